@@ -1,0 +1,557 @@
+// C ABI of the gfx950 ViSQOL library (include/visqol.h, include/visqol_hip.h).
+// Mirrors the reference's src/capi.zig: contexts, thread-local last error,
+// heap results owning every pointer they expose, per-pair batch status.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <fstream>
+#include <mutex>
+#include <sstream>
+
+#include "../../include/visqol.h"
+#include "../../include/visqol_hip.h"
+#include "engine.h"
+#include "wav.h"
+
+using namespace vq;
+
+struct visqol_ctx {
+    EngineOptions opts;
+    std::unique_ptr<Engine> engine;
+};
+
+namespace {
+
+thread_local char g_error[512] = {0};
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+}
+
+int invalid(const char* msg) {
+    set_error("%s", msg);
+    return VISQOL_E_INVALID_ARGUMENT;
+}
+
+// One allocation holding the visqol_result, its arrays and its patches
+// (capi.zig:65-70, 236-278).
+visqol_result* box(const PairResult& r) {
+    const size_t nb = r.fvnsim.size(), np = r.matches.size();
+    const size_t bytes = sizeof(visqol_result) + sizeof(double) * 5 * nb + sizeof(visqol_patch) * np +
+                         sizeof(double) * 3 * nb * np;
+    char* blk = (char*)calloc(1, bytes);
+    if (!blk) return nullptr;
+    visqol_result* out = (visqol_result*)blk;
+    double* arr = (double*)(blk + sizeof(visqol_result));
+    visqol_patch* patches = (visqol_patch*)(arr + 5 * nb);
+    double* band = (double*)(patches + np);
+    memcpy(arr, r.fvnsim.data(), sizeof(double) * nb);
+    memcpy(arr + nb, r.fvnsim10.data(), sizeof(double) * nb);
+    memcpy(arr + 2 * nb, r.fstdnsim.data(), sizeof(double) * nb);
+    memcpy(arr + 3 * nb, r.fvdegenergy.data(), sizeof(double) * nb);
+    memcpy(arr + 4 * nb, r.cf.data(), sizeof(double) * nb);
+    for (size_t i = 0; i < np; i++) {
+        const Match& m = r.matches[i];
+        double* d = band + i * 3 * nb;
+        memcpy(d, m.means, sizeof(double) * nb);
+        memcpy(d + nb, m.stddevs, sizeof(double) * nb);
+        memcpy(d + 2 * nb, m.energy, sizeof(double) * nb);
+        patches[i] = visqol_patch{m.similarity, m.ref_start, m.ref_end, m.deg_start, m.deg_end, d, d + nb, d + 2 * nb};
+    }
+    out->moslqo = r.moslqo;
+    out->vnsim = r.vnsim;
+    out->alignment_lag_s = r.lag_s;
+    out->num_bands = nb;
+    out->fvnsim = arr;
+    out->fvnsim10 = arr + nb;
+    out->fstdnsim = arr + 2 * nb;
+    out->fvdegenergy = arr + 3 * nb;
+    out->center_freq_bands = arr + 4 * nb;
+    out->num_patches = np;
+    out->patch_sims = patches;
+    return out;
+}
+
+std::string library_dir() {
+    Dl_info info;
+    if (dladdr((void*)&library_dir, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t s = p.rfind('/');
+        if (s != std::string::npos) return p.substr(0, s);
+    }
+    return ".";
+}
+
+bool read_file(const std::string& path, std::string& out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+bool device_ok(std::string& why) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        why = "no HIP device";
+        return false;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        why = "cannot query the HIP device";
+        return false;
+    }
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        why = std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950";
+        return false;
+    }
+    return true;
+}
+
+int run_pairs(visqol_ctx* c, const PairInput* pairs, size_t n, uint32_t sr, std::vector<PairResult>& res) {
+    res.assign(n, PairResult());
+    if (sr == 0) {
+        for (auto& r : res) { r.status = VISQOL_E_INVALID_ARGUMENT; r.error = "InvalidInput"; }
+        return VISQOL_OK;
+    }
+    try {
+        c->engine->run(pairs, (int)n, sr, res.data(), false);
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("OutOfMemory");
+        return VISQOL_E_OUT_OF_MEMORY;
+    }
+    return VISQOL_OK;
+}
+
+// Boxes per-pair results; status[i] per pair; the first failure is returned.
+int finish_batch(std::vector<PairResult>& res, visqol_result** out, int32_t* status) {
+    int first = VISQOL_OK;
+    for (size_t i = 0; i < res.size(); i++) {
+        out[i] = nullptr;
+        int rc = res[i].status;
+        if (rc == VISQOL_OK) {
+            out[i] = box(res[i]);
+            if (!out[i]) rc = VISQOL_E_OUT_OF_MEMORY;
+        }
+        if (rc != VISQOL_OK) set_error("%s", res[i].error.empty() ? "OutOfMemory" : res[i].error.c_str());
+        if (status) status[i] = rc;
+        if (rc != VISQOL_OK && first == VISQOL_OK) first = rc;
+    }
+    return first;
+}
+
+std::mutex g_stage_mu;
+std::unique_ptr<Engine> g_stage[2][2];  // [mode][fast]
+
+Engine* stage_engine(uint32_t mode, bool fast) {
+    auto& e = g_stage[mode ? 1 : 0][fast ? 1 : 0];
+    if (!e) {
+        EngineOptions o;
+        o.mode = mode ? 1 : 0;
+        o.fast = fast;
+        e.reset(new Engine(o, nullptr, nullptr));
+    }
+    return e.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t visqol_abi_version(void) { return VISQOL_ABI_VERSION; }
+
+const char* visqol_version(void) { return "0.1.0"; }
+
+const char* visqol_last_error(void) { return g_error; }
+
+uint32_t visqol_available_backends(void) {
+    std::string why;
+    return device_ok(why) ? (1u << VISQOL_BACKEND_HIP) : 0u;
+}
+
+void visqol_options_default(visqol_options* o) {
+    if (!o) return;
+    o->mode = VISQOL_MODE_AUDIO;
+    o->search_window_radius = 60;
+    o->use_unscaled_speech_mos_mapping = 0;
+    o->disable_global_alignment = 0;
+    o->disable_realignment = 0;
+    o->use_lattice_model = 1;
+    o->model_path = nullptr;
+    o->backend = VISQOL_BACKEND_HIP;
+    o->precision = VISQOL_PRECISION_AUTO;
+}
+
+int visqol_create(const visqol_options* in, visqol_ctx** out) {
+    if (!out) return invalid("out is NULL");
+    *out = nullptr;
+    visqol_options o;
+    visqol_options_default(&o);
+    if (in) o = *in;
+    if (o.mode > 1) {
+        set_error("invalid mode %u", o.mode);
+        return VISQOL_E_INVALID_ARGUMENT;
+    }
+    if (o.backend > VISQOL_BACKEND_HIP) {
+        set_error("invalid backend %u", o.backend);
+        return VISQOL_E_INVALID_ARGUMENT;
+    }
+    if (o.precision > VISQOL_PRECISION_AUTO) {
+        set_error("invalid precision %u", o.precision);
+        return VISQOL_E_INVALID_ARGUMENT;
+    }
+    if (o.backend != VISQOL_BACKEND_HIP) {
+        set_error("backend %u is not provided by this library; it implements VISQOL_BACKEND_HIP (4) on gfx950",
+                  o.backend);
+        return VISQOL_E_BACKEND;
+    }
+    std::string why;
+    if (!device_ok(why)) {
+        set_error("%s", why.c_str());
+        return VISQOL_E_BACKEND;
+    }
+    EngineOptions eo;
+    eo.mode = (int)o.mode;
+    eo.radius = o.search_window_radius > 0 ? o.search_window_radius : 60;
+    eo.unscaled_speech = o.use_unscaled_speech_mos_mapping != 0;
+    eo.no_global = o.disable_global_alignment != 0;
+    eo.no_realign = o.disable_realignment != 0;
+    eo.lattice = o.use_lattice_model != 0;
+    eo.fast = o.precision == VISQOL_PRECISION_FAST || o.precision == VISQOL_PRECISION_BALANCED;
+    std::unique_ptr<SvrModel> svr;
+    std::unique_ptr<LatticeModel, LatticeDeleter> lat;
+    if (eo.mode == 0 || eo.lattice) {
+        std::string path = o.model_path && o.model_path[0]
+                               ? std::string(o.model_path)
+                               : library_dir() + "/models/" + (eo.mode == 0 ? "libsvm_nu_svr_model.txt" : "lattice_speech.tflite");
+        std::string text, err;
+        if (!read_file(path, text)) {
+            set_error("cannot read model '%s'", path.c_str());
+            return VISQOL_E_IO;
+        }
+        if (eo.mode == 0) svr = SvrModel::parse(text, err);
+        else lat = load_lattice(text, kBandsSpeech, err);
+        if (!svr && !lat) {
+            set_error("invalid model '%s': %s", path.c_str(), err.c_str());
+            return VISQOL_E_MODEL;
+        }
+    }
+    visqol_ctx* c = new (std::nothrow) visqol_ctx();
+    if (!c) {
+        set_error("OutOfMemory");
+        return VISQOL_E_OUT_OF_MEMORY;
+    }
+    c->opts = eo;
+    try {
+        c->engine.reset(new Engine(eo, std::move(svr), std::move(lat)));
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        delete c;
+        return e.code;
+    }
+    *out = c;
+    return VISQOL_OK;
+}
+
+void visqol_destroy(visqol_ctx* c) { delete c; }
+
+void visqol_result_free(visqol_result* r) { free(r); }
+
+int visqol_measure(visqol_ctx* c, const double* ref, size_t ref_len, const double* deg, size_t deg_len,
+                   uint32_t sample_rate, visqol_result** out) {
+    if (!c) return invalid("ctx is NULL");
+    if (!out) return invalid("out is NULL");
+    *out = nullptr;
+    if ((!ref && ref_len) || (!deg && deg_len)) return invalid("signal pointer is NULL");
+    PairInput p{ref, (long long)ref_len, deg, (long long)deg_len, false};
+    std::vector<PairResult> res;
+    const int rc = run_pairs(c, &p, 1, sample_rate, res);
+    if (rc != VISQOL_OK) return rc;
+    return finish_batch(res, out, nullptr);
+}
+
+int visqol_measure_batch(visqol_ctx* c, size_t n, const double* const* refs, const size_t* ref_lens,
+                         const double* const* degs, const size_t* deg_lens, uint32_t sample_rate, uint32_t num_threads,
+                         visqol_result** out, int32_t* status) {
+    (void)num_threads;  // the device packs the whole batch
+    if (!c) return invalid("ctx is NULL");
+    if (n == 0) return VISQOL_OK;
+    if (!out) return invalid("out is NULL");
+    if (!refs) return invalid("refs is NULL");
+    if (!ref_lens) return invalid("ref_lens is NULL");
+    if (!degs) return invalid("degs is NULL");
+    if (!deg_lens) return invalid("deg_lens is NULL");
+    std::vector<PairInput> pairs(n);
+    for (size_t i = 0; i < n; i++) pairs[i] = PairInput{refs[i], (long long)ref_lens[i], degs[i], (long long)deg_lens[i], false};
+    std::vector<PairResult> res;
+    const int rc = run_pairs(c, pairs.data(), n, sample_rate, res);
+    if (rc != VISQOL_OK) {
+        for (size_t i = 0; i < n; i++) {
+            out[i] = nullptr;
+            if (status) status[i] = rc;
+        }
+        return rc;
+    }
+    return finish_batch(res, out, status);
+}
+
+int visqol_measure_files(visqol_ctx* c, const char* ref_path, const char* deg_path, visqol_result** out) {
+    if (!c) return invalid("ctx is NULL");
+    if (!out) return invalid("out is NULL");
+    *out = nullptr;
+    if (!ref_path) return invalid("ref_path is NULL");
+    if (!deg_path) return invalid("deg_path is NULL");
+    WavAudio ra, da;
+    std::string err;
+    if (!load_wav_mono(ref_path, ra, err)) { set_error("%s: %s", ref_path, err.c_str()); return wav_error_code(err); }
+    if (!load_wav_mono(deg_path, da, err)) { set_error("%s: %s", deg_path, err.c_str()); return wav_error_code(err); }
+    if (ra.sample_rate != da.sample_rate) {
+        set_error("Input audio signals have different sample rates! Reference audio sample rate: %u. Degraded audio sample rate: %u",
+                  ra.sample_rate, da.sample_rate);
+        return VISQOL_E_SIGNAL;
+    }
+    return visqol_measure(c, ra.samples.data(), ra.samples.size(), da.samples.data(), da.samples.size(), ra.sample_rate, out);
+}
+
+int visqol_measure_files_batch(visqol_ctx* c, size_t n, const char* const* ref_paths, const char* const* deg_paths,
+                               uint32_t num_threads, visqol_result** out, int32_t* status) {
+    (void)num_threads;
+    if (!c) return invalid("ctx is NULL");
+    if (n == 0) return VISQOL_OK;
+    if (!out) return invalid("out is NULL");
+    if (!ref_paths) return invalid("ref_paths is NULL");
+    if (!deg_paths) return invalid("deg_paths is NULL");
+    std::vector<WavAudio> refs(n), degs(n);
+    std::vector<int> codes(n, VISQOL_OK);
+    for (size_t i = 0; i < n; i++) {
+        out[i] = nullptr;
+        std::string err;
+        if (!load_wav_mono(ref_paths[i], refs[i], err)) {
+            set_error("%s: %s", ref_paths[i], err.c_str());
+            codes[i] = wav_error_code(err);
+            continue;
+        }
+        if (!load_wav_mono(deg_paths[i], degs[i], err)) {
+            set_error("%s: %s", deg_paths[i], err.c_str());
+            codes[i] = wav_error_code(err);
+            continue;
+        }
+        if (refs[i].sample_rate != degs[i].sample_rate) {
+            set_error("pair %zu: reference and degraded sample rates differ (%u vs %u)", i, refs[i].sample_rate,
+                      degs[i].sample_rate);
+            codes[i] = VISQOL_E_SIGNAL;
+        }
+    }
+    // One device batch per sample rate (capi.zig:429-448).
+    std::vector<bool> done(n);
+    for (size_t i = 0; i < n; i++) done[i] = codes[i] != VISQOL_OK;
+    for (size_t first = 0; first < n; first++) {
+        if (done[first]) continue;
+        const uint32_t rate = refs[first].sample_rate;
+        std::vector<size_t> idx;
+        std::vector<PairInput> pairs;
+        for (size_t i = first; i < n; i++) {
+            if (done[i] || refs[i].sample_rate != rate) continue;
+            idx.push_back(i);
+            pairs.push_back(PairInput{refs[i].samples.data(), (long long)refs[i].samples.size(), degs[i].samples.data(),
+                                      (long long)degs[i].samples.size(), false});
+            done[i] = true;
+        }
+        std::vector<PairResult> res;
+        const int rc = run_pairs(c, pairs.data(), pairs.size(), rate, res);
+        if (rc != VISQOL_OK) return rc;
+        for (size_t k = 0; k < idx.size(); k++) {
+            if (res[k].status == VISQOL_OK) {
+                out[idx[k]] = box(res[k]);
+                if (!out[idx[k]]) codes[idx[k]] = VISQOL_E_OUT_OF_MEMORY;
+            } else {
+                set_error("%s", res[k].error.c_str());
+                codes[idx[k]] = res[k].status;
+            }
+        }
+    }
+    int first_err = VISQOL_OK;
+    for (size_t i = 0; i < n; i++) {
+        if (status) status[i] = codes[i];
+        if (codes[i] != VISQOL_OK && first_err == VISQOL_OK) first_err = codes[i];
+    }
+    return first_err;
+}
+
+// ---------------------------------------------------------------- extensions
+int visqol_hip_measure_trace(visqol_ctx* c, const double* ref, size_t ref_len, const double* deg, size_t deg_len,
+                             uint32_t sample_rate, visqol_result** out, visqol_hip_trace* tr, double* ref_spec_out,
+                             double* deg_spec_out, double* sims_out) {
+    if (!c) return invalid("ctx is NULL");
+    if (!out) return invalid("out is NULL");
+    *out = nullptr;
+    if ((!ref && ref_len) || (!deg && deg_len)) return invalid("signal pointer is NULL");
+    if (sample_rate == 0) return invalid("InvalidInput");
+    PairInput p{ref, (long long)ref_len, deg, (long long)deg_len, false};
+    std::vector<PairResult> res(1);
+    try {
+        c->engine->run(&p, 1, sample_rate, res.data(), ref_spec_out || deg_spec_out || sims_out);
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    const PairResult& r = res[0];
+    if (tr) {
+        memset(tr, 0, sizeof *tr);
+        tr->lag = r.lag;
+        tr->spl_factor = r.spl_factor;
+        tr->nf_ref = r.nf_ref;
+        tr->nf_deg = r.nf_deg;
+        tr->num_patches = r.np;
+        for (int i = 0; i < 512; i++) tr->realign_lag[i] = INT64_MIN;
+        for (int i = 0; i < r.np && i < 512; i++) {
+            tr->ref_indices[i] = r.ref_idx[i];
+            if (i < (int)r.dp_offset.size()) tr->dp_offset[i] = r.dp_offset[i];
+            if (i < (int)r.match_valid.size()) tr->match_valid[i] = r.match_valid[i];
+            if (i < (int)r.realign_lag.size()) tr->realign_lag[i] = r.realign_lag[i];
+            if (i < (int)r.realign_frames.size()) tr->realign_frames[i] = r.realign_frames[i];
+            if (i < (int)r.realign_accepted.size()) tr->realign_accepted[i] = r.realign_accepted[i];
+        }
+    }
+    if (ref_spec_out && !r.ref_spec.empty()) memcpy(ref_spec_out, r.ref_spec.data(), sizeof(double) * r.ref_spec.size());
+    if (deg_spec_out && !r.deg_spec.empty()) memcpy(deg_spec_out, r.deg_spec.data(), sizeof(double) * r.deg_spec.size());
+    if (sims_out && !r.sims.empty()) memcpy(sims_out, r.sims.data(), sizeof(double) * r.sims.size());
+    return finish_batch(res, out, nullptr);
+}
+
+int visqol_hip_measure_batch_device(visqol_ctx* c, size_t n, const double* const* dev_refs, const size_t* ref_lens,
+                                    const double* const* dev_degs, const size_t* deg_lens, uint32_t sample_rate,
+                                    visqol_result** out, int32_t* status) {
+    if (!c) return invalid("ctx is NULL");
+    if (n == 0) return VISQOL_OK;
+    if (!out || !dev_refs || !ref_lens || !dev_degs || !deg_lens) return invalid("NULL argument");
+    std::vector<PairInput> pairs(n);
+    for (size_t i = 0; i < n; i++)
+        pairs[i] = PairInput{dev_refs[i], (long long)ref_lens[i], dev_degs[i], (long long)deg_lens[i], true};
+    std::vector<PairResult> res;
+    const int rc = run_pairs(c, pairs.data(), n, sample_rate, res);
+    if (rc != VISQOL_OK) {
+        for (size_t i = 0; i < n; i++) {
+            out[i] = nullptr;
+            if (status) status[i] = rc;
+        }
+        return rc;
+    }
+    return finish_batch(res, out, status);
+}
+
+int visqol_hip_profile(visqol_ctx* c, int enable, int reset) {
+    if (!c) return invalid("ctx is NULL");
+    c->engine->set_profiling(enable != 0);
+    if (reset) c->engine->reset_profile();
+    return VISQOL_OK;
+}
+
+int visqol_hip_profile_read(visqol_ctx* c, double* ms, double* flop, double* bytes, int64_t* launches) {
+    if (!c) return invalid("ctx is NULL");
+    const Engine::Profile p = c->engine->profile();
+    if (ms) *ms = p.gammatone_ms;
+    if (flop) *flop = p.gammatone_flop;
+    if (bytes) *bytes = p.gammatone_bytes;
+    if (launches) *launches = p.launches;
+    return VISQOL_OK;
+}
+
+int visqol_hip_alloc(size_t bytes, void** p) {
+    if (!p) return invalid("NULL argument");
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc of %zu bytes failed", bytes);
+        return VISQOL_E_OUT_OF_MEMORY;
+    }
+    return VISQOL_OK;
+}
+
+int visqol_hip_free(void* p) {
+    if (p && hipFree(p) != hipSuccess) {
+        set_error("hipFree failed");
+        return VISQOL_E_BACKEND;
+    }
+    return VISQOL_OK;
+}
+
+int visqol_hip_upload(void* dst, const void* src, size_t bytes) {
+    if (!dst || (!src && bytes)) return invalid("NULL argument");
+    if (hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("hipMemcpy failed");
+        return VISQOL_E_BACKEND;
+    }
+    return VISQOL_OK;
+}
+
+int visqol_hip_spectrogram(uint32_t mode, uint32_t sample_rate, uint32_t precision, const double* signal, size_t len,
+                           double* out, size_t* num_frames_out) {
+    if (!signal || !out || mode > 1 || sample_rate == 0) return invalid("invalid argument");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    try {
+        std::vector<double> spec;
+        int nf = 0;
+        stage_engine(mode, precision == VISQOL_PRECISION_FAST || precision == VISQOL_PRECISION_BALANCED)
+            ->spectrogram(sample_rate, signal, (long long)len, spec, nf);
+        memcpy(out, spec.data(), sizeof(double) * spec.size());
+        if (num_frames_out) *num_frames_out = (size_t)nf;
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return VISQOL_OK;
+}
+
+int visqol_hip_filters(uint32_t mode, uint32_t sample_rate, double* bands8, double* cf) {
+    if (mode > 1 || sample_rate == 0) return invalid("invalid argument");
+    const RateParams p = make_rate_params((int)mode, sample_rate);
+    if (bands8) memcpy(bands8, p.raw_band.data(), sizeof(double) * p.raw_band.size());
+    if (cf) memcpy(cf, p.cf.data(), sizeof(double) * p.cf.size());
+    return VISQOL_OK;
+}
+
+int visqol_hip_best_lag(const double* a, size_t la, const double* b, size_t lb, int64_t* lag) {
+    if ((!a && la) || (!b && lb) || !lag) return invalid("invalid argument");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    try {
+        *lag = stage_engine(0, false)->best_lag(a, (long long)la, b, (long long)lb);
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return VISQOL_OK;
+}
+
+int visqol_hip_math_probe(int op, const double* a, const double* b, double* out, size_t n) {
+    if (!a || !out || op < 0 || op > 2 || (op == 2 && !b)) return invalid("invalid argument");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    try {
+        stage_engine(0, false)->math_probe(op, a, b, out, (long long)n);
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return VISQOL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,880 @@
+// Batched device scoring pipeline (see engine.h).
+//
+// Phase A (device): envelope sums, packed envelopes, FFT cross-correlation,
+//   first-max lag; SPL energies of the reference and of the lag-aligned
+//   degraded signal; speech VAD chunk activity.  One readback.
+// Host: lag application, SPL factor, frame counts, patch indices, search
+//   windows (visqol.zig:489-555).
+// Phase B (device): aligned+scaled degraded signal, gammatone spectrograms,
+//   dB prep, deg planes, similarity table, DP.  Readback of the DP choice,
+//   then per-patch NSIM statistics (dp.zig:313-352).
+// Phase C (device, unless disabled): per-match slices, envelope lags,
+//   trimmed spectrograms (grid-aligned frames copied from the full
+//   spectrograms), dB prep and statistics; the host keeps a realigned
+//   patch unless it is strictly worse (realign.zig:294-460).
+// Host: aggregation and MOS mapping (visqol.zig:595-709).
+#include "engine.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+
+namespace vq {
+
+// ---------------------------------------------------------------- arenas
+DeviceArena::~DeviceArena() {
+    for (auto& c : chunks_) (void)hipFree(c.base);
+}
+
+void* DeviceArena::alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    for (auto& c : chunks_)
+        if (c.size - c.used >= bytes) {
+            void* p = c.base + c.used;
+            c.used += bytes;
+            total_used_ += bytes;
+            high_water_ = std::max(high_water_, total_used_);
+            return p;
+        }
+    size_t sz = std::max(bytes, (size_t)256 << 20);
+    if (!chunks_.empty()) sz = std::max(sz, chunks_.back().size);
+    char* base = nullptr;
+    if (hipMalloc(&base, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        throw BackendError{-2, "device out of memory"};
+    }
+    chunks_.push_back({base, sz, bytes});
+    total_used_ += bytes;
+    high_water_ = std::max(high_water_, total_used_);
+    return base;
+}
+
+// Between batches: one chunk of the high-water size, so steady-state
+// batches allocate nothing.
+void DeviceArena::reset() {
+    if (chunks_.size() > 1) {
+        for (auto& c : chunks_) (void)hipFree(c.base);
+        chunks_.clear();
+        char* base = nullptr;
+        const size_t sz = (high_water_ + ((size_t)1 << 20)) & ~(((size_t)1 << 20) - 1);
+        if (hipMalloc(&base, sz) == hipSuccess) chunks_.push_back({base, sz, 0});
+        else (void)hipGetLastError();
+    }
+    for (auto& c : chunks_) c.used = 0;
+    total_used_ = 0;
+}
+
+PinnedArena::~PinnedArena() {
+    for (auto& c : chunks_) (void)hipHostFree(c.base);
+}
+
+void* PinnedArena::alloc(size_t bytes) {
+    bytes = (bytes + 63) & ~(size_t)63;
+    if (bytes == 0) bytes = 64;
+    for (auto& c : chunks_)
+        if (c.size - c.used >= bytes) {
+            void* p = c.base + c.used;
+            c.used += bytes;
+            return p;
+        }
+    size_t sz = std::max(bytes, (size_t)16 << 20);
+    char* base = nullptr;
+    if (hipHostMalloc(&base, sz, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        throw BackendError{-2, "pinned host memory exhausted"};
+    }
+    chunks_.push_back({base, sz, bytes});
+    return base;
+}
+
+void PinnedArena::reset() {
+    for (auto& c : chunks_) c.used = 0;
+}
+
+// ---------------------------------------------------------------- engine
+Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
+               std::unique_ptr<LatticeModel, LatticeDeleter> lattice)
+    : opts_(o), svr_(std::move(svr)), lattice_(std::move(lattice)) {
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+Engine::~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& kv : rates_) {
+        (void)hipFree(kv.second->coef);
+        (void)hipFree(kv.second->hann);
+    }
+    for (auto& kv : twiddles_) (void)hipFree(kv.second);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw BackendError{-6, std::string(what) + ": " + hipGetErrorString(e)};
+    }
+}
+
+void Engine::sync() {
+    check(hipGetLastError(), "kernel launch");
+    check(hipStreamSynchronize(stream_), "device execution");
+}
+
+template <typename T>
+T* Engine::upload(const T* v, size_t count) {
+    if (count == 0) return nullptr;
+    const size_t bytes = sizeof(T) * count;
+    void* h = pinned_.alloc(bytes);
+    memcpy(h, v, bytes);
+    T* d = (T*)arena_.alloc(bytes);
+    check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_), "upload");
+    return d;
+}
+
+template <typename T>
+T* Engine::upload(const std::vector<T>& v) {
+    return upload(v.data(), v.size());
+}
+
+void Engine::readback(void* host_dst, const void* dev_src, size_t bytes) {
+    if (bytes == 0) return;
+    check(hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, stream_), "readback");
+}
+
+const RateParams& Engine::rate(uint32_t sr) { return dev_rate(sr).p; }
+
+Engine::DevRate& Engine::dev_rate(uint32_t sr) {
+    auto it = rates_.find(sr);
+    if (it != rates_.end()) return *it->second;
+    auto d = std::make_unique<DevRate>();
+    d->p = make_rate_params(opts_.mode, sr);
+    check(hipMalloc(&d->coef, sizeof(BandCoef) * d->p.nb), "hipMalloc");
+    check(hipMalloc(&d->hann, sizeof(double) * std::max(1, d->p.window)), "hipMalloc");
+    check(hipMemcpy(d->coef, d->p.coef.data(), sizeof(BandCoef) * d->p.nb, hipMemcpyHostToDevice), "upload");
+    check(hipMemcpy(d->hann, d->p.hann.data(), sizeof(double) * d->p.window, hipMemcpyHostToDevice), "upload");
+    auto& ref = *d;
+    rates_[sr] = std::move(d);
+    return ref;
+}
+
+const double2* Engine::twiddles(long long n) {
+    auto it = twiddles_.find(n);
+    if (it != twiddles_.end()) return it->second;
+    const std::vector<double> tw = make_twiddles(n);
+    double2* d = nullptr;
+    check(hipMalloc(&d, sizeof(double) * tw.size()), "hipMalloc twiddles");
+    check(hipMemcpy(d, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice), "upload twiddles");
+    twiddles_[n] = d;
+    return d;
+}
+
+// Envelope packing, FFT rows grouped by size, cross terms, inverse, lag scan.
+void Engine::lag_search(std::vector<LagJob>& jobs, LagJob* d_jobs, long long* d_lags) {
+    if (jobs.empty()) return;
+    long long max_n = 0;
+    std::map<long long, std::vector<double2*>> groups;
+    for (auto& j : jobs) {
+        max_n = std::max(max_n, j.n);
+        groups[j.n].push_back(j.z);
+    }
+    for (auto& g : groups) {
+        const double2* tw = twiddles(g.first);
+        double2** rows = upload(g.second);
+        launch_fft_dif(rows, (int)g.second.size(), g.first, tw, stream_);
+        launch_cross_terms(rows, (int)g.second.size(), g.first, stream_);
+        launch_fft_dit(rows, (int)g.second.size(), g.first, tw, stream_);
+    }
+    auto* scratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * 2 * jobs.size());
+    launch_lag_argmax(d_jobs, (int)jobs.size(), max_n, scratch, d_lags, stream_);
+}
+
+namespace {
+
+struct PairState {
+    bool live = true;
+    const double* dref = nullptr;
+    const double* ddeg = nullptr;
+    long long rl = 0, dl = 0;
+    int lagjob = -1, vadjob = -1;
+    long long lag = 0, shift = 0, dl2 = 0;
+    double lag_s = 0, factor = 1.0;
+    int nf_ref = 0, nf_deg = 0, np = 0, width = 0;
+    std::vector<int> idx, lo, hi;
+    VadPlan vp;
+    size_t vad_off = 0;
+    double* degs = nullptr;
+    double *spec_ref = nullptr, *spec_deg = nullptr, *raw_ref = nullptr, *raw_deg = nullptr;
+    double *plane = nullptr, *sims = nullptr, *dp = nullptr;
+    int *bt = nullptr, *d_idx = nullptr, *d_lo = nullptr, *d_hi = nullptr, *d_off = nullptr, *d_state = nullptr;
+};
+
+void fail(PairResult& r, PairState& s, int code, const char* msg) {
+    r.status = code;
+    r.error = msg;
+    s.live = false;
+}
+
+}  // namespace
+
+void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages) {
+    // Chunks bounded by samples so a batch of any size fits in HBM.
+    const long long kChunkSamples = 1LL << 28;
+    int start = 0;
+    while (start < n) {
+        long long samples = 0;
+        int end = start;
+        while (end < n && (end == start || (samples + pairs[end].ref_len + pairs[end].deg_len <= kChunkSamples && end - start < 512))) {
+            samples += pairs[end].ref_len + pairs[end].deg_len;
+            end++;
+        }
+        run_chunk(pairs + start, end - start, sr, out + start, want_stages);
+        start = end;
+    }
+}
+
+void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages) {
+    arena_.reset();
+    pinned_.reset();
+    DevRate& dr = dev_rate(sr);
+    const RateParams& rp = dr.p;
+    const int nb = rp.nb, W = rp.window, hop = rp.hop, S = rp.patch;
+    const bool serial = !opts_.fast;
+    std::vector<PairState> st(n);
+
+    // ------------------------------------------------------------ phase A
+    for (int i = 0; i < n; i++) {
+        out[i] = PairResult();
+        PairState& s = st[i];
+        s.rl = pairs[i].ref_len;
+        s.dl = pairs[i].deg_len;
+        if (pairs[i].on_device) {
+            s.dref = pairs[i].ref;
+            s.ddeg = pairs[i].deg;
+        } else {
+            double* r = (double*)arena_.alloc(sizeof(double) * std::max(1LL, s.rl));
+            double* d = (double*)arena_.alloc(sizeof(double) * std::max(1LL, s.dl));
+            if (s.rl) check(hipMemcpyAsync(r, pairs[i].ref, sizeof(double) * s.rl, hipMemcpyHostToDevice, stream_), "upload");
+            if (s.dl) check(hipMemcpyAsync(d, pairs[i].deg, sizeof(double) * s.dl, hipMemcpyHostToDevice, stream_), "upload");
+            s.dref = r;
+            s.ddeg = d;
+        }
+    }
+    double* d_sums = (double*)arena_.alloc(sizeof(double) * 6 * n);
+    check(hipMemsetAsync(d_sums, 0, sizeof(double) * 6 * n, stream_), "memset");
+    std::vector<SumJob> j1, j2, j3;
+    std::vector<LagJob> lj;
+    for (int i = 0; i < n; i++) {
+        PairState& s = st[i];
+        if (!opts_.no_global && s.rl > 0 && s.dl > 0) {
+            j1.push_back(SumJob{s.dref, s.rl, kSum, 6 * i, 0, 0, nullptr, 0});
+            j1.push_back(SumJob{s.ddeg, s.dl, kSum, 6 * i + 1, 0, 0, nullptr, 0});
+            j2.push_back(SumJob{s.dref, s.rl, kNyquist, 6 * i + 2, 6 * i, 0, nullptr, 0});
+            j2.push_back(SumJob{s.ddeg, s.dl, kNyquist, 6 * i + 3, 6 * i + 1, 0, nullptr, 0});
+            const long long nfft = fft_size_for(std::max(s.rl, s.dl));
+            double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
+            s.lagjob = (int)lj.size();
+            lj.push_back(LagJob{s.dref, s.ddeg, s.rl, s.dl, z, nfft, 6 * i, 6 * i + 1, 6 * i + 2, 6 * i + 3});
+        }
+        j1.push_back(SumJob{s.dref, s.rl, kSumSq, 6 * i + 4, 0, 0, nullptr, 0});
+    }
+    launch_sums(upload(j1), (int)j1.size(), d_sums, serial, stream_);
+    launch_sums(upload(j2), (int)j2.size(), d_sums, serial, stream_);
+    long long* d_lags = (long long*)arena_.alloc(sizeof(long long) * (lj.size() + 1));
+    long long* d_zero = d_lags + lj.size();
+    check(hipMemsetAsync(d_zero, 0, sizeof(long long), stream_), "memset");
+    if (!lj.empty()) {
+        LagJob* d_lj = upload(lj);
+        long long max_n = 0;
+        for (auto& j : lj) max_n = std::max(max_n, j.n);
+        launch_env_pack(d_lj, (int)lj.size(), max_n, d_sums, stream_);
+        lag_search(lj, d_lj, d_lags);
+    }
+    for (int i = 0; i < n; i++) {
+        PairState& s = st[i];
+        j3.push_back(SumJob{s.ddeg, s.dl, kSumSqFrom, 6 * i + 5, 0, 0, s.lagjob >= 0 ? d_lags + s.lagjob : d_zero, s.rl / 2});
+    }
+    launch_sums(upload(j3), (int)j3.size(), d_sums, serial, stream_);
+    std::vector<VadJob> vj;
+    size_t vad_bytes = 0;
+    if (opts_.mode == 1) {
+        for (int i = 0; i < n; i++) {
+            PairState& s = st[i];
+            if (s.rl <= W) continue;
+            s.vp = vad_plan(num_frames(s.rl, W, hop), S, W);
+            if (s.vp.n_chunks <= 0) continue;
+            s.vadjob = (int)vj.size();
+            s.vad_off = vad_bytes;
+            vad_bytes += (size_t)s.vp.n_chunks;
+            vj.push_back(VadJob{s.dref, s.rl, nullptr, s.vp.n_chunks, s.vp.frame_len, s.vp.first, 0});
+        }
+    }
+    unsigned char* d_active = vad_bytes ? (unsigned char*)arena_.alloc(vad_bytes) : nullptr;
+    if (!vj.empty()) {
+        int max_chunks = 0;
+        for (auto& v : vj) max_chunks = std::max(max_chunks, v.n_chunks);
+        for (int i = 0; i < n; i++)
+            if (st[i].vadjob >= 0) vj[st[i].vadjob].active = d_active + st[i].vad_off;
+        auto* scratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * vj.size());
+        launch_vad(upload(vj), (int)vj.size(), max_chunks, scratch, stream_);
+    }
+    std::vector<long long> h_lags(lj.size());
+    std::vector<double> h_sums(6 * n);
+    std::vector<unsigned char> h_active(vad_bytes);
+    {
+        long long* pl = (long long*)pinned_.alloc(sizeof(long long) * (lj.size() + 1));
+        double* ps = (double*)pinned_.alloc(sizeof(double) * 6 * n);
+        unsigned char* pa = (unsigned char*)pinned_.alloc(vad_bytes + 1);
+        readback(pl, d_lags, sizeof(long long) * lj.size());
+        readback(ps, d_sums, sizeof(double) * 6 * n);
+        readback(pa, d_active, vad_bytes);
+        sync();
+        memcpy(h_lags.data(), pl, sizeof(long long) * lj.size());
+        memcpy(h_sums.data(), ps, sizeof(double) * 6 * n);
+        memcpy(h_active.data(), pa, vad_bytes);
+    }
+
+    // ------------------------------------------------------------ host plan
+    for (int i = 0; i < n; i++) {
+        PairState& s = st[i];
+        PairResult& r = out[i];
+        s.lag = s.lagjob >= 0 ? h_lags[s.lagjob] : 0;
+        r.lag = s.lag;
+        // realign.zig:91-110 (applyLag)
+        s.dl2 = s.dl;
+        if (!opts_.no_global) {
+            const long long a = s.lag < 0 ? -s.lag : s.lag;
+            if (!(s.lag == 0 || a > s.rl / 2)) {
+                s.lag_s = (double)s.lag / (double)sr;
+                if (s.lag < 0) {
+                    const long long k = std::min(-s.lag, s.dl);
+                    s.shift = -k;
+                    s.dl2 = s.dl - k;
+                } else {
+                    s.shift = s.lag;
+                    s.dl2 = s.dl + s.lag;
+                }
+            }
+        }
+        // audio.zig:132-159 (scaleToMatchSpl)
+        s.factor = spl_factor(spl_of_energy(h_sums[6 * i + 4], s.rl), spl_of_energy(h_sums[6 * i + 5], s.dl2));
+        r.spl_factor = s.factor;
+        r.lag_s = s.lag_s;
+        if (s.rl <= W || s.dl2 <= W) { fail(r, s, -5, "SpectrogramTooShort"); continue; }
+        s.nf_ref = num_frames(s.rl, W, hop);
+        s.nf_deg = num_frames(s.dl2, W, hop);
+        r.nf_ref = s.nf_ref;
+        r.nf_deg = s.nf_deg;
+        if (opts_.mode == 0) {
+            if (!ref_patch_indices(s.nf_ref, S, s.idx)) { fail(r, s, -5, "ReferenceSpectrogramTooSmall"); continue; }
+        } else {
+            s.idx.clear();
+            if (s.vadjob >= 0) vad_patches(s.vp, h_active.data() + s.vad_off, S, s.idx);
+        }
+        if (s.idx.empty()) { fail(r, s, -5, "NoPatches"); continue; }
+        if (!plan_search(s.idx, s.nf_ref, s.nf_deg, S, opts_.radius, s.np, s.lo, s.hi)) {
+            fail(r, s, -5, "SignalsTooDifferent");
+            continue;
+        }
+        r.np = s.np;
+        r.ref_idx.assign(s.idx.begin(), s.idx.begin() + s.np);
+        s.width = s.nf_deg;
+        for (int p = 0; p < s.np; p++) s.width = std::max(s.width, s.hi[p] + S);
+    }
+
+    // ------------------------------------------------------------ phase B
+    std::vector<ScaleJob> scj;
+    std::vector<GtJob> gj;
+    std::vector<PrepJob> pj;
+    std::vector<PlaneJob> plj;
+    std::vector<SimJob> sj;
+    std::vector<DpJob> dj;
+    int max_nf = 0, max_width = 0, max_np = 0, max_win = 0;
+    long long max_len = 0;
+    for (int i = 0; i < n; i++) {
+        PairState& s = st[i];
+        if (!s.live) continue;
+        s.degs = (double*)arena_.alloc(sizeof(double) * s.dl2);
+        scj.push_back(ScaleJob{s.ddeg, s.degs, s.dl, s.dl2, s.shift, s.factor});
+        max_len = std::max(max_len, s.dl2);
+        s.spec_ref = (double*)arena_.alloc(sizeof(double) * nb * s.nf_ref);
+        s.spec_deg = (double*)arena_.alloc(sizeof(double) * nb * s.nf_deg);
+        gj.push_back(GtJob{s.dref, s.spec_ref, s.nf_ref, 0});
+        gj.push_back(GtJob{s.degs, s.spec_deg, s.nf_deg, 0});
+        max_nf = std::max(max_nf, std::max(s.nf_ref, s.nf_deg));
+        if (!opts_.no_realign || want_stages) {
+            s.raw_ref = (double*)arena_.alloc(sizeof(double) * nb * s.nf_ref);
+            s.raw_deg = (double*)arena_.alloc(sizeof(double) * nb * s.nf_deg);
+        }
+        pj.push_back(PrepJob{s.spec_ref, s.spec_deg, s.nf_ref, s.nf_deg});
+        s.plane = (double*)arena_.alloc(sizeof(double) * 5 * nb * s.width);
+        plj.push_back(PlaneJob{s.spec_deg, s.plane, s.nf_deg, s.width});
+        max_width = std::max(max_width, s.width);
+        const size_t cells = (size_t)s.np * s.nf_deg;
+        s.sims = (double*)arena_.alloc(sizeof(double) * cells);
+        s.dp = (double*)arena_.alloc(sizeof(double) * cells);
+        s.bt = (int*)arena_.alloc(sizeof(int) * cells);
+        check(hipMemsetAsync(s.sims, 0, sizeof(double) * cells, stream_), "memset");
+        check(hipMemsetAsync(s.dp, 0, sizeof(double) * cells, stream_), "memset");
+        check(hipMemsetAsync(s.bt, 0, sizeof(int) * cells, stream_), "memset");
+        s.d_idx = upload(s.idx.data(), s.np);
+        s.d_lo = upload(s.lo);
+        s.d_hi = upload(s.hi);
+        s.d_off = (int*)arena_.alloc(sizeof(int) * s.np);
+        s.d_state = (int*)arena_.alloc(sizeof(int) * s.np);
+        sj.push_back(SimJob{s.spec_ref, s.plane, s.sims, s.d_idx, s.d_lo, s.d_hi, s.np, s.nf_ref, s.nf_deg, s.width});
+        dj.push_back(DpJob{s.sims, s.dp, s.bt, s.d_lo, s.d_hi, s.d_off, s.d_state, s.np, s.nf_deg});
+        max_np = std::max(max_np, s.np);
+        for (int p = 0; p < s.np; p++) max_win = std::max(max_win, s.hi[p] - s.lo[p] + 1);
+    }
+    bool timed = false;
+    if (!scj.empty()) {
+        launch_scale(upload(scj), (int)scj.size(), max_len, stream_);
+        GtJob* d_gj = upload(gj);
+        if (profiling_) {
+            if (!ev_[0]) {
+                check(hipEventCreate(&ev_[0]), "hipEventCreate");
+                check(hipEventCreate(&ev_[1]), "hipEventCreate");
+            }
+            check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+        }
+        launch_gammatone(d_gj, (int)gj.size(), max_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, stream_);
+        if (profiling_) {
+            check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+            timed = true;
+            for (const GtJob& g : gj) {
+                prof_.gammatone_flop += 30.0 * W * nb * (double)g.nf;
+                prof_.gammatone_bytes += 8.0 * ((double)(g.nf - 1) * hop + W) + 8.0 * nb * g.nf;
+            }
+            prof_.launches++;
+        }
+        for (int i = 0; i < n; i++) {
+            PairState& s = st[i];
+            if (!s.live || !s.raw_ref) continue;
+            check(hipMemcpyAsync(s.raw_ref, s.spec_ref, sizeof(double) * nb * s.nf_ref, hipMemcpyDeviceToDevice, stream_), "copy");
+            check(hipMemcpyAsync(s.raw_deg, s.spec_deg, sizeof(double) * nb * s.nf_deg, hipMemcpyDeviceToDevice, stream_), "copy");
+        }
+        auto* prep_scratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * pj.size());
+        launch_prep(upload(pj), (int)pj.size(), nb, max_nf, prep_scratch, stream_);
+        launch_planes(upload(plj), (int)plj.size(), nb, max_width, stream_);
+        double* terms = (double*)arena_.alloc(sizeof(double) * 3 * (size_t)nb * S * max_np * sj.size());
+        launch_sim_table(upload(sj), (int)sj.size(), nb, S, max_np, max_win, terms, stream_);
+        launch_dp(upload(dj), (int)dj.size(), stream_);
+    }
+    // Readback of the DP choice (and stage outputs on request).
+    std::vector<std::vector<int>> h_off(n), h_state(n);
+    {
+        std::vector<std::pair<int*, int*>> pins(n, {nullptr, nullptr});
+        std::vector<double*> pr(n, nullptr), pd(n, nullptr), pss(n, nullptr);
+        for (int i = 0; i < n; i++) {
+            PairState& s = st[i];
+            if (!s.live) continue;
+            pins[i].first = (int*)pinned_.alloc(sizeof(int) * s.np);
+            pins[i].second = (int*)pinned_.alloc(sizeof(int) * s.np);
+            readback(pins[i].first, s.d_off, sizeof(int) * s.np);
+            readback(pins[i].second, s.d_state, sizeof(int) * s.np);
+            if (want_stages) {
+                pr[i] = (double*)pinned_.alloc(sizeof(double) * nb * s.nf_ref);
+                pd[i] = (double*)pinned_.alloc(sizeof(double) * nb * s.nf_deg);
+                pss[i] = (double*)pinned_.alloc(sizeof(double) * s.np * s.nf_deg);
+                readback(pr[i], s.raw_ref, sizeof(double) * nb * s.nf_ref);
+                readback(pd[i], s.raw_deg, sizeof(double) * nb * s.nf_deg);
+                readback(pss[i], s.sims, sizeof(double) * s.np * s.nf_deg);
+            }
+        }
+        sync();
+        if (timed) {
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+            prof_.gammatone_ms += ms;
+        }
+        for (int i = 0; i < n; i++) {
+            if (!pins[i].first) continue;
+            h_off[i].assign(pins[i].first, pins[i].first + st[i].np);
+            h_state[i].assign(pins[i].second, pins[i].second + st[i].np);
+            if (want_stages) {
+                out[i].ref_spec.assign(pr[i], pr[i] + (size_t)nb * st[i].nf_ref);
+                out[i].deg_spec.assign(pd[i], pd[i] + (size_t)nb * st[i].nf_deg);
+                out[i].sims.assign(pss[i], pss[i] + (size_t)st[i].np * st[i].nf_deg);
+            }
+        }
+    }
+
+    // Per-patch NSIM statistics of the chosen patches (dp.zig:313-352).
+    std::vector<StatJob> stj;
+    std::vector<std::pair<int, int>> stat_of;  // (pair, patch)
+    double* d_stats = nullptr;
+    const int stat_len = 3 * nb + 1;
+    {
+        int total = 0;
+        for (int i = 0; i < n; i++)
+            if (st[i].live)
+                for (int p = 0; p < st[i].np; p++)
+                    if (h_state[i][p] != 0) total++;
+        d_stats = (double*)arena_.alloc(sizeof(double) * stat_len * std::max(1, total));
+        for (int i = 0; i < n; i++) {
+            PairState& s = st[i];
+            if (!s.live) continue;
+            for (int p = 0; p < s.np; p++) {
+                if (h_state[i][p] == 0) continue;
+                double* o = d_stats + (size_t)stat_len * stj.size();
+                stj.push_back(StatJob{s.spec_ref, s.spec_deg, s.nf_ref, s.nf_deg, s.idx[p], h_off[i][p], s.nf_deg, S, o});
+                stat_of.emplace_back(i, p);
+            }
+        }
+        launch_stats(upload(stj), (int)stj.size(), nb, S, stream_);
+        std::vector<double> h(stat_len * stj.size());
+        double* ph = (double*)pinned_.alloc(sizeof(double) * (h.size() + 1));
+        readback(ph, d_stats, sizeof(double) * h.size());
+        sync();
+        memcpy(h.data(), ph, sizeof(double) * h.size());
+        for (int i = 0; i < n; i++)
+            if (st[i].live) {
+                out[i].matches.assign(st[i].np, Match());
+                out[i].dp_offset.assign(st[i].np, 0);
+                out[i].match_valid.assign(st[i].np, 1);
+                out[i].realign_lag.assign(st[i].np, INT64_MIN);
+                out[i].realign_frames.assign(st[i].np, 0);
+                out[i].realign_accepted.assign(st[i].np, 0);
+            }
+        const double pdur = rp.frame_duration * (double)S;
+        for (size_t k = 0; k < stj.size(); k++) {
+            const int i = stat_of[k].first, p = stat_of[k].second;
+            Match& m = out[i].matches[p];
+            const double* v = &h[k * stat_len];
+            for (int b = 0; b < nb; b++) {
+                m.means[b] = v[b];
+                m.stddevs[b] = v[nb + b];
+                m.energy[b] = v[2 * nb + b];
+            }
+            m.similarity = v[3 * nb];
+            const int off = h_off[i][p];
+            out[i].dp_offset[p] = off;
+            if (h_state[i][p] == 2) {
+                m.deg_start = 0.0;
+                m.deg_end = 0.0;
+                m.similarity = 0.0;
+                for (int b = 0; b < nb; b++) m.means[b] = 0.0;
+                m.valid = false;
+            } else {
+                m.deg_start = (double)off * rp.frame_duration;
+                m.deg_end = m.deg_start + pdur;
+                m.valid = true;
+            }
+            m.ref_start = (double)st[i].idx[p] * rp.frame_duration;
+            m.ref_end = m.ref_start + pdur;
+            out[i].match_valid[p] = m.valid ? 1 : 0;
+        }
+    }
+
+    // ------------------------------------------------------------ phase C
+    if (!opts_.no_realign) {
+        struct RJ {
+            int pair, match;
+            Layout rl, dl;
+            long long rlen, dlen;
+            double *sref, *sdeg;
+            int lagjob;
+            Geometry geo;
+            int nf = 0;
+            double *spec_r = nullptr, *spec_d = nullptr;
+        };
+        std::vector<RJ> rjs;
+        std::vector<SliceJob> slj;
+        long long max_slice = 0;
+        for (int i = 0; i < n; i++) {
+            PairState& s = st[i];
+            if (!s.live) continue;
+            for (int p = 0; p < s.np; p++) {
+                const Match& m = out[i].matches[p];
+                if (!m.valid) continue;
+                RJ rj;
+                rj.pair = i;
+                rj.match = p;
+                rj.rl = slice_layout(s.rl, sr, m.ref_start, m.ref_end);
+                rj.dl = slice_layout(s.dl2, sr, m.deg_start, m.deg_end);
+                rj.rlen = rj.rl.pad + rj.rl.len;
+                rj.dlen = rj.dl.pad + rj.dl.len;
+                if (rj.rlen < W || rj.dlen < W) continue;
+                rj.sref = (double*)arena_.alloc(sizeof(double) * rj.rlen);
+                rj.sdeg = (double*)arena_.alloc(sizeof(double) * rj.dlen);
+                slj.push_back(SliceJob{s.dref, rj.sref, rj.rl.pad, rj.rl.base, rj.rl.len, rj.rlen});
+                slj.push_back(SliceJob{s.degs, rj.sdeg, rj.dl.pad, rj.dl.base, rj.dl.len, rj.dlen});
+                max_slice = std::max(max_slice, std::max(rj.rlen, rj.dlen));
+                rjs.push_back(rj);
+            }
+        }
+        if (!rjs.empty()) {
+            launch_slices(upload(slj), (int)slj.size(), max_slice, stream_);
+            double* d_rs = (double*)arena_.alloc(sizeof(double) * 4 * rjs.size());
+            std::vector<SumJob> a1, a2;
+            std::vector<LagJob> rlj;
+            for (size_t k = 0; k < rjs.size(); k++) {
+                RJ& rj = rjs[k];
+                const int b = 4 * (int)k;
+                a1.push_back(SumJob{rj.sref, rj.rlen, kSum, b, 0, 0, nullptr, 0});
+                a1.push_back(SumJob{rj.sdeg, rj.dlen, kSum, b + 1, 0, 0, nullptr, 0});
+                a2.push_back(SumJob{rj.sref, rj.rlen, kNyquist, b + 2, b, 0, nullptr, 0});
+                a2.push_back(SumJob{rj.sdeg, rj.dlen, kNyquist, b + 3, b + 1, 0, nullptr, 0});
+                const long long nfft = fft_size_for(std::max(rj.rlen, rj.dlen));
+                double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
+                rj.lagjob = (int)rlj.size();
+                rlj.push_back(LagJob{rj.sref, rj.sdeg, rj.rlen, rj.dlen, z, nfft, b, b + 1, b + 2, b + 3});
+            }
+            launch_sums(upload(a1), (int)a1.size(), d_rs, serial, stream_);
+            launch_sums(upload(a2), (int)a2.size(), d_rs, serial, stream_);
+            LagJob* d_rlj = upload(rlj);
+            long long max_n = 0;
+            for (auto& j : rlj) max_n = std::max(max_n, j.n);
+            launch_env_pack(d_rlj, (int)rlj.size(), max_n, d_rs, stream_);
+            long long* d_rlags = (long long*)arena_.alloc(sizeof(long long) * rlj.size());
+            lag_search(rlj, d_rlj, d_rlags);
+            std::vector<long long> h_rl(rlj.size());
+            long long* pl = (long long*)pinned_.alloc(sizeof(long long) * rlj.size());
+            readback(pl, d_rlags, sizeof(long long) * rlj.size());
+            sync();
+            memcpy(h_rl.data(), pl, sizeof(long long) * rlj.size());
+
+            // Trim geometry, frame reuse, spectrograms of what is kept.
+            std::vector<GtJob> rg;
+            std::vector<SliceJob> tslj;
+            std::vector<PrepJob> rpj;
+            std::vector<StatJob> rstj;
+            int rmax_nf = 0;
+            long long max_t = 0;
+            struct Copy {
+                double* dst;
+                const double* src;
+                int src_stride, nf;
+            };
+            std::vector<Copy> copies;
+            for (size_t k = 0; k < rjs.size(); k++) {
+                RJ& rj = rjs[k];
+                PairState& s = st[rj.pair];
+                const long long lag = h_rl[rj.lagjob];
+                out[rj.pair].realign_lag[rj.match] = lag;
+                rj.geo = trim_geometry(rj.rlen, rj.dlen, lag, sr);
+                rj.nf = num_frames(rj.geo.len, W, hop);
+                out[rj.pair].realign_frames[rj.match] = rj.nf;
+                if (rj.nf == 0) continue;
+                rj.spec_r = (double*)arena_.alloc(sizeof(double) * nb * rj.nf);
+                rj.spec_d = (double*)arena_.alloc(sizeof(double) * nb * rj.nf);
+                rmax_nf = std::max(rmax_nf, rj.nf);
+                // realign.zig:262-281 (RawSpectrograms.reuse)
+                auto reuse = [&](const Layout& l, long long off, int nf_global, long long& g0) {
+                    if (off < l.pad) return false;
+                    const long long start = l.base + off - l.pad;
+                    if (start % hop != 0) return false;
+                    g0 = start / hop;
+                    return g0 + rj.nf <= nf_global;
+                };
+                long long g0 = 0;
+                if (reuse(rj.rl, rj.geo.ref_off, s.nf_ref, g0)) {
+                    copies.push_back(Copy{rj.spec_r, s.raw_ref + g0, s.nf_ref, rj.nf});
+                } else {
+                    rg.push_back(GtJob{rj.sref + rj.geo.ref_off, rj.spec_r, rj.nf, 0});
+                }
+                if (reuse(rj.dl, rj.geo.deg_off, s.nf_deg, g0)) {
+                    copies.push_back(Copy{rj.spec_d, s.raw_deg + g0, s.nf_deg, rj.nf});
+                } else {
+                    const long long off = rj.geo.deg_off;
+                    if (off >= 0 && off + rj.geo.len <= rj.dlen) {
+                        rg.push_back(GtJob{rj.sdeg + off, rj.spec_d, rj.nf, 0});
+                    } else {
+                        double* t = (double*)arena_.alloc(sizeof(double) * rj.geo.len);
+                        const long long pad = off < 0 ? std::min(-off, rj.geo.len) : 0;
+                        const long long base = off < 0 ? 0 : off;
+                        const long long avail = std::max(0LL, std::min(rj.geo.len - pad, rj.dlen - base));
+                        tslj.push_back(SliceJob{rj.sdeg, t, pad, base, avail, rj.geo.len});
+                        max_t = std::max(max_t, rj.geo.len);
+                        rg.push_back(GtJob{t, rj.spec_d, rj.nf, 0});
+                    }
+                }
+                rpj.push_back(PrepJob{rj.spec_r, rj.spec_d, rj.nf, rj.nf});
+            }
+            launch_slices(upload(tslj), (int)tslj.size(), max_t, stream_);
+            launch_gammatone(upload(rg), (int)rg.size(), rmax_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, stream_);
+            for (auto& c : copies)
+                check(hipMemcpy2DAsync(c.dst, sizeof(double) * c.nf, c.src, sizeof(double) * c.src_stride,
+                                       sizeof(double) * c.nf, nb, hipMemcpyDeviceToDevice, stream_),
+                      "copy frames");
+            auto* rscratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * std::max<size_t>(1, rpj.size()));
+            launch_prep(upload(rpj), (int)rpj.size(), nb, rmax_nf, rscratch, stream_);
+            double* d_rstats = (double*)arena_.alloc(sizeof(double) * stat_len * std::max<size_t>(1, rjs.size()));
+            std::vector<int> rstat_of;
+            for (size_t k = 0; k < rjs.size(); k++) {
+                RJ& rj = rjs[k];
+                if (rj.nf == 0) continue;
+                rstj.push_back(StatJob{rj.spec_r, rj.spec_d, rj.nf, rj.nf, 0, 0, rj.nf, rj.nf,
+                                       d_rstats + (size_t)stat_len * rstj.size()});
+                rstat_of.push_back((int)k);
+            }
+            launch_stats(upload(rstj), (int)rstj.size(), nb, rmax_nf, stream_);
+            std::vector<double> h(stat_len * rstj.size());
+            double* ph = (double*)pinned_.alloc(sizeof(double) * (h.size() + 1));
+            readback(ph, d_rstats, sizeof(double) * h.size());
+            sync();
+            memcpy(h.data(), ph, sizeof(double) * h.size());
+            // realign.zig:398-460 (finishRealign)
+            for (size_t q = 0; q < rstj.size(); q++) {
+                const RJ& rj = rjs[rstat_of[q]];
+                Match& m = out[rj.pair].matches[rj.match];
+                const double* v = &h[q * stat_len];
+                const double mean = v[3 * nb];
+                if (!(mean < m.similarity)) {
+                    for (int b = 0; b < nb; b++) {
+                        m.means[b] = v[b];
+                        m.stddevs[b] = v[nb + b];
+                        m.energy[b] = v[2 * nb + b];
+                    }
+                    m.similarity = mean;
+                    const double new_dur = (double)rj.geo.len / (double)sr;
+                    if (rj.geo.lag_seconds > 0.0) m.ref_start += rj.geo.lag_seconds;
+                    else m.deg_start -= rj.geo.lag_seconds;
+                    m.ref_end = m.ref_start + new_dur;
+                    m.deg_end = m.deg_start + new_dur;
+                    out[rj.pair].realign_accepted[rj.match] = 1;
+                }
+            }
+        }
+    }
+
+    for (int i = 0; i < n; i++)
+        if (st[i].live) finish(out[i], rp);
+}
+
+// visqol.zig:595-709 (buildResult) and visqol.zig:81-98 (mapping).
+void Engine::finish(PairResult& r, const RateParams& rp) {
+    const int nb = rp.nb;
+    const size_t n = r.matches.size();
+    const double n_f = (double)n;
+    r.cf = rp.cf;
+    r.fvnsim.assign(nb, 0.0);
+    r.fvdegenergy.assign(nb, 0.0);
+    r.fvnsim10.assign(nb, 0.0);
+    r.fstdnsim.assign(nb, 0.0);
+    for (const Match& m : r.matches)
+        for (int b = 0; b < nb; b++) {
+            r.fvnsim[b] += m.means[b];
+            r.fvdegenergy[b] += m.energy[b];
+        }
+    for (int b = 0; b < nb; b++) {
+        r.fvnsim[b] /= n_f;
+        r.fvdegenergy[b] /= n_f;
+    }
+    const size_t k = std::max<size_t>(1, (size_t)trunc(n_f * 0.10));
+    std::vector<double> col(n);
+    for (int b = 0; b < nb; b++) {
+        for (size_t i = 0; i < n; i++) col[i] = r.matches[i].means[b];
+        std::sort(col.begin(), col.end());
+        double acc = 0.0;
+        for (size_t i = 0; i < k && i < n; i++) acc += col[i];
+        r.fvnsim10[b] = acc / (double)k;
+    }
+    std::vector<double> contrib(nb, 0.0);
+    long long total = 0;
+    for (const Match& m : r.matches) {
+        const long long fc_i = (long long)ceil((m.ref_end - m.ref_start) / rp.frame_duration);
+        total += fc_i;
+        const double fc = (double)fc_i;
+        for (int b = 0; b < nb; b++) {
+            contrib[b] += (fc - 1.0) * m.stddevs[b] * m.stddevs[b];
+            contrib[b] += fc * m.means[b] * m.means[b];
+        }
+    }
+    const double total_f = (double)total;
+    for (int b = 0; b < nb; b++) {
+        const double v = (contrib[b] - r.fvnsim[b] * r.fvnsim[b] * total_f) / (total_f - 1.0);
+        r.fstdnsim[b] = v < 0.0 ? 0.0 : sqrt(v);
+    }
+    double sum = 0.0;
+    for (int b = 0; b < nb; b++) sum += r.fvnsim[b];
+    r.vnsim = sum / (double)nb;
+    double mos = 1.0;
+    if (opts_.mode == 0) {
+        mos = fmax(1.0, fmin(svr_->predict(r.fvnsim.data()), 5.0));
+    } else if (opts_.lattice) {
+        if (!lattice_predict(*lattice_, r.fvnsim.data(), r.fvnsim10.data(), r.fstdnsim.data(), r.fvdegenergy.data(), mos)) {
+            r.status = -4;
+            r.error = "lattice model evaluation failed";
+            return;
+        }
+    } else {
+        double s2 = 0.0;
+        for (int b = 0; b < nb; b++) s2 += r.fvnsim[b];
+        const double mean = s2 / (double)nb;
+        const double m = -262.847869 + exp(0.0154302525 * (mean - -361.063949));
+        const double scale = opts_.unscaled_speech ? 1.0 : (double)1.245063f;
+        mos = fmax(1.0, fmin(m * scale, 5.0));
+    }
+    if (r.vnsim < 0.15) mos = 1.0;
+    r.moslqo = mos;
+}
+
+// ---------------------------------------------------------------- stage access
+void Engine::spectrogram(uint32_t sr, const double* host_sig, long long len, std::vector<double>& out, int& nf) {
+    arena_.reset();
+    pinned_.reset();
+    DevRate& dr = dev_rate(sr);
+    nf = num_frames(len, dr.p.window, dr.p.hop);
+    out.assign((size_t)dr.p.nb * nf, 0.0);
+    if (nf == 0) return;
+    double* d = (double*)arena_.alloc(sizeof(double) * len);
+    check(hipMemcpyAsync(d, host_sig, sizeof(double) * len, hipMemcpyHostToDevice, stream_), "upload");
+    double* o = (double*)arena_.alloc(sizeof(double) * out.size());
+    std::vector<GtJob> j{GtJob{d, o, nf, 0}};
+    launch_gammatone(upload(j), 1, nf, dr.coef, dr.hann, dr.p.nb, dr.p.window, dr.p.hop, opts_.fast, stream_);
+    double* p = (double*)pinned_.alloc(sizeof(double) * out.size());
+    readback(p, o, sizeof(double) * out.size());
+    sync();
+    memcpy(out.data(), p, sizeof(double) * out.size());
+}
+
+long long Engine::best_lag(const double* a, long long la, const double* b, long long lb) {
+    arena_.reset();
+    pinned_.reset();
+    if (std::max(la, lb) == 0) return 0;
+    double* da = (double*)arena_.alloc(sizeof(double) * std::max(1LL, la));
+    double* db = (double*)arena_.alloc(sizeof(double) * std::max(1LL, lb));
+    if (la) check(hipMemcpyAsync(da, a, sizeof(double) * la, hipMemcpyHostToDevice, stream_), "upload");
+    if (lb) check(hipMemcpyAsync(db, b, sizeof(double) * lb, hipMemcpyHostToDevice, stream_), "upload");
+    double* sums = (double*)arena_.alloc(sizeof(double) * 4);
+    check(hipMemsetAsync(sums, 0, sizeof(double) * 4, stream_), "memset");
+    std::vector<SumJob> s1{SumJob{da, la, kSum, 0, 0, 0, nullptr, 0}, SumJob{db, lb, kSum, 1, 0, 0, nullptr, 0}};
+    std::vector<SumJob> s2{SumJob{da, la, kNyquist, 2, 0, 0, nullptr, 0}, SumJob{db, lb, kNyquist, 3, 1, 0, nullptr, 0}};
+    launch_sums(upload(s1), 2, sums, !opts_.fast, stream_);
+    launch_sums(upload(s2), 2, sums, !opts_.fast, stream_);
+    const long long nfft = fft_size_for(std::max(la, lb));
+    double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
+    std::vector<LagJob> lj{LagJob{da, db, la, lb, z, nfft, 0, 1, 2, 3}};
+    LagJob* d_lj = upload(lj);
+    launch_env_pack(d_lj, 1, nfft, sums, stream_);
+    long long* d_lag = (long long*)arena_.alloc(sizeof(long long));
+    lag_search(lj, d_lj, d_lag);
+    long long* p = (long long*)pinned_.alloc(sizeof(long long));
+    readback(p, d_lag, sizeof(long long));
+    sync();
+    return *p;
+}
+
+void Engine::math_probe(int op, const double* a, const double* b, double* out, long long n) {
+    arena_.reset();
+    pinned_.reset();
+    if (n == 0) return;
+    double* da = (double*)arena_.alloc(sizeof(double) * n);
+    double* db = (double*)arena_.alloc(sizeof(double) * n);
+    double* dout = (double*)arena_.alloc(sizeof(double) * n);
+    check(hipMemcpyAsync(da, a, sizeof(double) * n, hipMemcpyHostToDevice, stream_), "upload");
+    if (b) check(hipMemcpyAsync(db, b, sizeof(double) * n, hipMemcpyHostToDevice, stream_), "upload");
+    launch_math_probe(op, da, db, dout, n, stream_);
+    double* p = (double*)pinned_.alloc(sizeof(double) * n);
+    readback(p, dout, sizeof(double) * n);
+    sync();
+    memcpy(out, p, sizeof(double) * n);
+}
+
+}  // namespace vq

@@ -1,0 +1,159 @@
+// The device scoring pipeline: batches of pairs at one sample rate run
+// stage by stage on one gfx950 device (visqol.zig:394-487's eight stages),
+// with host planning between the phases whose shapes depend on device
+// results (the global lag, the DP choice, the realignment lags).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host.h"
+#include "kernels.h"
+
+namespace vq {
+
+struct EngineOptions {
+    int mode = 0;
+    int radius = 60;
+    bool unscaled_speech = false;
+    bool no_global = false;
+    bool no_realign = false;
+    bool lattice = true;
+    bool fast = false;
+};
+
+struct PairInput {
+    const double* ref;
+    long long ref_len;
+    const double* deg;
+    long long deg_len;
+    bool on_device;  // pointers are device memory
+};
+
+struct Match {
+    double similarity = 0, ref_start = 0, ref_end = 0, deg_start = 0, deg_end = 0;
+    double means[kMaxBands] = {0}, stddevs[kMaxBands] = {0}, energy[kMaxBands] = {0};
+    bool valid = true;
+};
+
+struct PairResult {
+    int status = 0;  // visqol.h code
+    std::string error;
+    double moslqo = 0, vnsim = 0, lag_s = 0;
+    std::vector<double> fvnsim, fvnsim10, fstdnsim, fvdegenergy, cf;
+    std::vector<Match> matches;
+    // Trace of the discrete decisions.
+    long long lag = 0;
+    double spl_factor = 1.0;
+    int nf_ref = 0, nf_deg = 0, np = 0;
+    std::vector<int> ref_idx, dp_offset;
+    std::vector<unsigned char> match_valid, realign_accepted;
+    std::vector<long long> realign_lag;
+    std::vector<int> realign_frames;
+    // Optional stage outputs.
+    std::vector<double> ref_spec, deg_spec, sims;
+};
+
+// Device memory in chunks; allocations stay put until reset().
+class DeviceArena {
+   public:
+    ~DeviceArena();
+    void* alloc(size_t bytes);
+    void reset();
+
+   private:
+    struct Chunk {
+        char* base;
+        size_t size, used;
+    };
+    std::vector<Chunk> chunks_;
+    size_t total_used_ = 0, high_water_ = 0;
+};
+
+// Pinned host staging for job tables and readbacks.
+class PinnedArena {
+   public:
+    ~PinnedArena();
+    void* alloc(size_t bytes);
+    void reset();
+
+   private:
+    struct Chunk {
+        char* base;
+        size_t size, used;
+    };
+    std::vector<Chunk> chunks_;
+};
+
+struct BackendError {
+    int code;
+    std::string msg;
+};
+
+class Engine {
+   public:
+    Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
+           std::unique_ptr<LatticeModel, LatticeDeleter> lattice);
+    ~Engine();
+    // Scores n pairs; per-pair outcome in out[i].status.  Throws
+    // BackendError for device failures that poison the whole batch.
+    void run(const PairInput* pairs, int n, uint32_t sample_rate, PairResult* out, bool want_stages);
+    const EngineOptions& options() const { return opts_; }
+
+    // Stage access for checks.
+    void spectrogram(uint32_t sample_rate, const double* host_sig, long long len, std::vector<double>& out, int& nf);
+    long long best_lag(const double* a, long long la, const double* b, long long lb);
+    void math_probe(int op, const double* a, const double* b, double* out, long long n);
+    const RateParams& rate(uint32_t sample_rate);
+
+    // Device time of the full-signal gammatone launches, measured with HIP
+    // events on the engine's stream, and their algorithmic work.
+    struct Profile {
+        double gammatone_ms = 0;
+        double gammatone_flop = 0;  // 30 per (sample, band) of every frame
+        double gammatone_bytes = 0; // signal samples read + spectrogram written
+        long long launches = 0;
+    };
+    void set_profiling(bool on) { profiling_ = on; }
+    Profile profile() const { return prof_; }
+    void reset_profile() { prof_ = Profile(); }
+
+   private:
+    bool profiling_ = false;
+    Profile prof_;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    struct DevRate {
+        RateParams p;
+        BandCoef* coef = nullptr;
+        double* hann = nullptr;
+    };
+    DevRate& dev_rate(uint32_t sample_rate);
+    const double2* twiddles(long long n);
+    template <typename T>
+    T* upload(const std::vector<T>& v);
+    template <typename T>
+    T* upload(const T* v, size_t count);
+    void readback(void* host_dst, const void* dev_src, size_t bytes);
+    void sync();
+    void check(hipError_t e, const char* what);
+    // One chunk of pairs (bounded HBM footprint).
+    void run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages);
+    // FFT cross-correlation lags of jobs whose packed rows are set up.
+    void lag_search(std::vector<LagJob>& jobs, LagJob* d_jobs, long long* d_lags);
+    void finish(PairResult& r, const RateParams& rp);
+
+    EngineOptions opts_;
+    std::unique_ptr<SvrModel> svr_;
+    std::unique_ptr<LatticeModel, LatticeDeleter> lattice_;
+    hipStream_t stream_ = nullptr;
+    DeviceArena arena_;
+    PinnedArena pinned_;
+    std::map<uint32_t, std::unique_ptr<DevRate>> rates_;
+    std::map<long long, double2*> twiddles_;
+};
+
+}  // namespace vq

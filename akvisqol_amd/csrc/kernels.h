@@ -1,0 +1,157 @@
+// Launch interface of the gfx950 kernels (host side).  Every launcher takes
+// device pointers and a stream; job tables live in device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "dev_common.h"
+
+namespace vq {
+
+// One signal's spectrogram: out[b * nf + f] for frames f < nf.
+struct GtJob {
+    const double* sig;
+    double* out;
+    int nf;
+    int pad_;
+};
+
+void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const double* d_hann, int nb,
+                      int window, int hop, bool fast, hipStream_t stream);
+
+// ---------------------------------------------------------------- reductions
+// Serial (reference-order) or tree reductions of one signal each.
+enum SumKind : int {
+    kSum = 0,       // Σ x
+    kSumSq = 1,     // Σ x·x
+    kNyquist = 2,   // Σ (-1)^i (x - mean), mean = out[aux] / len
+    kSumSqFrom = 3  // Σ x·x over x[start..len), start from a lag slot (see SumJob)
+};
+struct SumJob {
+    const double* x;
+    long long len;
+    int kind;
+    int slot;              // result goes to out[slot]
+    int aux;               // kNyquist: slot holding the plain sum
+    int pad_;
+    const long long* lag;  // kSumSqFrom: global lag; start = -lag if 0 < -lag <= half, else 0
+    long long half;        // kSumSqFrom: ref_len / 2
+};
+// out[job.slot] = reduction of each job.  `serial` keeps the reference's
+// left-to-right order (bit-exact); otherwise a deterministic tree order.
+void launch_sums(const SumJob* d_jobs, int njobs, double* d_out, bool serial, hipStream_t stream);
+
+// ---------------------------------------------------------------- envelope + FFT xcorr
+// One lag search: z[i] = (env(a)[i], env(b)[i]) zero padded to n (a power of two).
+struct LagJob {
+    const double* a;
+    const double* b;
+    long long la, lb;
+    double2* z;
+    long long n;
+    int sum_a, sum_b;  // indices of the plain sums in the sums output
+    int nyq_a, nyq_b;  // indices of the nyquist sums
+};
+void launch_env_pack(const LagJob* d_jobs, int njobs, long long max_n, const double* d_sums, hipStream_t stream);
+// Radix-2 DIF forward / DIT inverse over rows of the same n; tw = W_n^k, k < n/2.
+void launch_fft_dif(double2* const* d_rows, int nrows, long long n, const double2* d_tw, hipStream_t stream);
+void launch_fft_dit(double2* const* d_rows, int nrows, long long n, const double2* d_tw, hipStream_t stream);
+void launch_cross_terms(double2* const* d_rows, int nrows, long long n, hipStream_t stream);
+// lag = first argmax of |Re z| in the order -max_lag..-1, 0..max_lag.
+void launch_lag_argmax(const LagJob* d_jobs, int njobs, long long max_n, unsigned long long* d_scratch,
+                       long long* d_lags, hipStream_t stream);
+
+// ---------------------------------------------------------------- signal prep
+// dst[i] = src-aligned sample i times factor: negative lag drops -lag
+// leading samples, positive lag prepends zeros (realign.zig:91-110).
+struct ScaleJob {
+    const double* src;
+    double* dst;
+    long long src_len, dst_len;
+    long long shift;  // dst[i] = src[i - shift]
+    double factor;
+};
+void launch_scale(const ScaleJob* d_jobs, int njobs, long long max_len, hipStream_t stream);
+
+// A zero-padded window of a signal, `total` long: dst[i] = src[base + i - pad]
+// for pad <= i < pad + len, else 0.
+struct SliceJob {
+    const double* src;
+    double* dst;
+    long long pad, base, len, total;
+};
+void launch_slices(const SliceJob* d_jobs, int njobs, long long max_len, hipStream_t stream);
+
+// Speech-mode VAD chunk activity (vad.zig:31-48): per chunk c,
+// RMS of trunc(clamp(x / max(ref) * 32768)) over frame_len samples starting
+// at first + c·frame_len, compared with 5000.
+struct VadJob {
+    const double* ref;
+    long long len;
+    unsigned char* active;
+    int n_chunks, frame_len, first, pad_;
+};
+void launch_vad(const VadJob* d_jobs, int njobs, int max_chunks, unsigned long long* d_scratch, hipStream_t stream);
+
+// ---------------------------------------------------------------- spectrogram prep
+struct PrepJob {
+    double* ref;
+    double* deg;
+    int nfr, nfd;
+};
+// dB, per-frame floor, joint-minimum subtract (spectrogram.zig:69-80).
+void launch_prep(const PrepJob* d_jobs, int njobs, int nb, int max_nf, unsigned long long* d_scratch,
+                 hipStream_t stream);
+
+// ---------------------------------------------------------------- NSIM
+struct PlaneJob {
+    const double* spec;  // [nb][nf]
+    double* plane;       // 5 planes of [nb][width]: pad, sq, mu, dms, sds
+    int nf, width;
+};
+void launch_planes(const PlaneJob* d_jobs, int njobs, int nb, int max_width, hipStream_t stream);
+
+struct SimJob {
+    const double* ref_spec;  // [nb][nf_ref], comparison-ready dB
+    const double* plane;     // from launch_planes
+    double* sims;            // [np][nf_deg]
+    const int* ref_idx;      // [np]
+    const int* lo;           // [np]
+    const int* hi;           // [np]
+    int np, nf_ref, nf_deg, width;
+};
+// sims[p][off] for off in [lo[p], hi[p]] (accel.zig:126-146); d_terms holds
+// 3 * nb * patch * max_np * njobs doubles of ref-patch terms.
+void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int max_np, int max_window, double* d_terms,
+                      hipStream_t stream);
+
+struct DpJob {
+    const double* sims;
+    double* dp;     // [np][nf_deg], zeroed
+    int* bt;        // [np][nf_deg], zeroed
+    const int* lo;
+    const int* hi;
+    int* offsets;   // [np] chosen deg start frame
+    int* state;     // [np] 0 untouched, 1 valid, 2 no real match
+    int np, nf_deg;
+};
+void launch_dp(const DpJob* d_jobs, int njobs, hipStream_t stream);
+
+// Per-band NSIM statistics of one patch pair (nsim.zig:427-510).
+struct StatJob {
+    const double* ref;  // ref spectrogram base
+    const double* deg;  // deg spectrogram base
+    int ref_stride, deg_stride;  // row strides (frames per band)
+    int ref_col, deg_col;        // first column; deg columns >= deg_valid read 0
+    int deg_valid;
+    int ncols;
+    double* out;  // [3*nb + 1]: means, stddevs, energy, mean of means
+};
+void launch_stats(const StatJob* d_jobs, int njobs, int nb, int max_cols, hipStream_t stream);
+
+// Device math checks (log10, sqrt, division) against the host.
+void launch_math_probe(int op, const double* a, const double* b, double* out, long long n, hipStream_t stream);
+
+}  // namespace vq

@@ -1,0 +1,775 @@
+// MOS mapping models on the host: the libsvm nu-SVR (svr.zig) and the
+// TF-Lattice speech model (lattice.zig) through a small TFLite flatbuffer
+// interpreter for the 18 builtin ops the model uses (tflite.zig).
+//
+// The lattice is one graph walk per pair (about 0.25 M f32 multiply-adds):
+// cheaper on the host than a device round trip, so it stays here.  Dots
+// are sequential f32 fused multiply-adds, the TFLite builtin order the
+// reference reproduces (tflite.zig:665-671).
+#include <math.h>
+#include <string.h>
+
+#include <cstdlib>
+#include <sstream>
+
+#include "host.h"
+
+namespace vq {
+
+// ---------------------------------------------------------------- SVR
+static std::string trim(const std::string& s) {
+    size_t a = 0, b = s.size();
+    while (a < b && (s[a] == ' ' || s[a] == '\t')) a++;
+    while (b > a && (s[b - 1] == ' ' || s[b - 1] == '\t')) b--;
+    return s.substr(a, b - a);
+}
+
+static bool parse_double(const std::string& s, double& v) {
+    if (s.empty()) return false;
+    char* end = nullptr;
+    v = strtod(s.c_str(), &end);
+    return end && *end == 0;
+}
+
+std::unique_ptr<SvrModel> SvrModel::parse(const std::string& text, std::string& err) {
+    auto m = std::unique_ptr<SvrModel>(new SvrModel());
+    std::vector<std::vector<std::string>> rows;
+    bool in_sv = false;
+    size_t max_feat = 0;
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.empty()) continue;
+        if (!in_sv) {
+            if (line == "SV") { in_sv = true; continue; }
+            const size_t sp = line.find(' ');
+            if (sp == std::string::npos) continue;
+            const std::string key = line.substr(0, sp), val = trim(line.substr(sp + 1));
+            if (key == "svm_type" && val != "nu_svr" && val != "epsilon_svr") { err = "UnsupportedSvmType"; return nullptr; }
+            if (key == "kernel_type" && val != "rbf") { err = "UnsupportedKernel"; return nullptr; }
+            if (key == "gamma" && !parse_double(val, m->gamma_)) { err = "InvalidModelFile"; return nullptr; }
+            if (key == "rho" && !parse_double(val, m->rho_)) { err = "InvalidModelFile"; return nullptr; }
+            continue;
+        }
+        std::vector<std::string> toks;
+        std::string tok;
+        std::istringstream ls(line);
+        while (ls >> tok) toks.push_back(tok);
+        for (size_t i = 1; i < toks.size(); i++) {
+            const size_t colon = toks[i].find(':');
+            if (colon == std::string::npos || colon == 0) continue;
+            const std::string is = toks[i].substr(0, colon);
+            if (is.find_first_not_of("0123456789") != std::string::npos) continue;
+            max_feat = std::max(max_feat, (size_t)strtoull(is.c_str(), nullptr, 10));
+        }
+        rows.push_back(std::move(toks));
+    }
+    if (rows.empty()) { err = "NoSupportVectors"; return nullptr; }
+    if (max_feat == 0) { err = "InvalidModelFile"; return nullptr; }
+    m->nf_ = (int)max_feat;
+    m->sv_.assign(rows.size() * max_feat, 0.0);
+    m->coef_.resize(rows.size());
+    for (size_t r = 0; r < rows.size(); r++) {
+        if (rows[r].empty() || !parse_double(rows[r][0], m->coef_[r])) { err = "InvalidModelFile"; return nullptr; }
+        for (size_t i = 1; i < rows[r].size(); i++) {
+            const size_t colon = rows[r][i].find(':');
+            if (colon == std::string::npos) continue;
+            const std::string is = rows[r][i].substr(0, colon);
+            if (is.empty() || is.find_first_not_of("0123456789") != std::string::npos) continue;
+            double v;
+            if (!parse_double(rows[r][i].substr(colon + 1), v)) continue;
+            const size_t idx = strtoull(is.c_str(), nullptr, 10);
+            if (idx == 0 || idx > max_feat) { err = "InvalidFeatureIndex"; return nullptr; }
+            m->sv_[r * max_feat + idx - 1] = v;
+        }
+    }
+    return m;
+}
+
+double SvrModel::predict(const double* x) const {
+    double sum = 0.0;
+    for (size_t i = 0; i < coef_.size(); i++) {
+        const double* sv = &sv_[i * nf_];
+        double dist = 0.0;
+        for (int k = 0; k < nf_; k++) {
+            const double d = sv[k] - x[k];
+            dist += d * d;
+        }
+        sum += coef_[i] * exp(-gamma_ * dist);
+    }
+    return sum - rho_;
+}
+
+// ---------------------------------------------------------------- TFLite
+namespace {
+
+struct FbError {};
+
+// Bounds-checked little-endian flatbuffer reader (tflite.zig:15-82).
+struct Fb {
+    const uint8_t* p;
+    size_t n;
+    template <typename T>
+    T at(size_t pos) const {
+        if (pos > n || n - pos < sizeof(T)) throw FbError{};
+        T v;
+        memcpy(&v, p + pos, sizeof(T));
+        return v;
+    }
+    // Absolute position of a table field, 0 if absent.
+    size_t field(size_t tab, int id) const {
+        const long long vt = (long long)tab - at<int32_t>(tab);
+        if (vt < 0) throw FbError{};
+        const uint16_t vsize = at<uint16_t>((size_t)vt);
+        const size_t slot = 4 + 2 * (size_t)id;
+        if (slot + 2 > vsize) return 0;
+        const uint16_t off = at<uint16_t>((size_t)vt + slot);
+        return off ? tab + off : 0;
+    }
+    template <typename T>
+    T scalar(size_t tab, int id, T dflt) const {
+        const size_t f = field(tab, id);
+        return f ? at<T>(f) : dflt;
+    }
+    size_t deref(size_t pos) const { return pos + at<uint32_t>(pos); }
+    size_t table(size_t tab, int id) const {
+        const size_t f = field(tab, id);
+        return f ? deref(f) : 0;
+    }
+    // (start, len) of a vector field; len < 0 if absent.
+    std::pair<size_t, long long> vec(size_t tab, int id) const {
+        const size_t f = field(tab, id);
+        if (!f) return {0, -1};
+        const size_t v = deref(f);
+        return {v + 4, (long long)at<uint32_t>(v)};
+    }
+    std::string str(size_t tab, int id) const {
+        auto v = vec(tab, id);
+        if (v.second < 0) return std::string();
+        if (v.first > n || n - v.first < (size_t)v.second) throw FbError{};
+        return std::string((const char*)p + v.first, (size_t)v.second);
+    }
+};
+
+enum class Op { Add, Sub, Mul, Min, Max, Equal, Relu, Relu6, Abs, Cast, Concat, FullyConnected, BatchMatMul, Reshape,
+                ExpandDims, Split, Shape, Fill, Unknown };
+enum class Dt { None, F32, I32, Bool };
+
+Op op_of(int code) {
+    switch (code) {
+        case 0: return Op::Add;
+        case 2: return Op::Concat;
+        case 9: return Op::FullyConnected;
+        case 18: return Op::Mul;
+        case 19: return Op::Relu;
+        case 21: return Op::Relu6;
+        case 22: return Op::Reshape;
+        case 41: return Op::Sub;
+        case 49: return Op::Split;
+        case 53: return Op::Cast;
+        case 55: return Op::Max;
+        case 57: return Op::Min;
+        case 70: return Op::ExpandDims;
+        case 71: return Op::Equal;
+        case 77: return Op::Shape;
+        case 94: return Op::Fill;
+        case 101: return Op::Abs;
+        case 126: return Op::BatchMatMul;
+        default: return Op::Unknown;
+    }
+}
+
+// A tensor value: shape plus one of three typed payloads (f32 / i32 / bool
+// stored as bytes).  Payloads are shared so reshape / expand are views.
+struct Val {
+    bool set = false;
+    Dt dt = Dt::None;
+    std::vector<size_t> shape;
+    std::shared_ptr<std::vector<float>> f;
+    std::shared_ptr<std::vector<int32_t>> i;
+    std::shared_ptr<std::vector<uint8_t>> b;
+    size_t count() const {
+        size_t c = 1;
+        for (size_t d : shape) c *= d;
+        return c;
+    }
+};
+
+struct Node {
+    Op op;
+    std::vector<int> in, out;
+    int act = 0;
+    int axis = 0;
+    bool adj_x = false, adj_y = false, keep_dims = false;
+    bool has_shape = false;
+    std::vector<int32_t> new_shape;
+};
+
+inline float act_apply(int act, float x) {
+    switch (act) {
+        case 1: return fmaxf(x, 0.0f);
+        case 2: return fmaxf(-1.0f, fminf(x, 1.0f));
+        case 3: return fmaxf(0.0f, fminf(x, 6.0f));
+        default: return x;
+    }
+}
+
+struct GraphError {};
+
+std::vector<size_t> bshape(const std::vector<size_t>& a, const std::vector<size_t>& b) {
+    const size_t r = std::max(a.size(), b.size());
+    std::vector<size_t> o(r);
+    for (size_t k = 0; k < r; k++) {
+        const size_t da = k + a.size() >= r ? a[k + a.size() - r] : 1;
+        const size_t db = k + b.size() >= r ? b[k + b.size() - r] : 1;
+        if (da != db && da != 1 && db != 1) throw GraphError{};
+        o[k] = da == 1 ? db : da;
+    }
+    return o;
+}
+
+// Flat source index of every output element under numpy-style broadcast.
+std::vector<size_t> bindex(const std::vector<size_t>& src, const std::vector<size_t>& out) {
+    const size_t r = out.size();
+    std::vector<size_t> stride(r, 0);
+    size_t acc = 1;
+    for (size_t k = src.size(); k-- > 0;) {
+        const size_t ok = k + r - src.size();
+        stride[ok] = src[k] == 1 ? 0 : acc;
+        acc *= src[k];
+    }
+    size_t total = 1;
+    for (size_t d : out) total *= d;
+    std::vector<size_t> idx(total);
+    std::vector<size_t> pos(r, 0);
+    size_t cur = 0;
+    for (size_t e = 0; e < total; e++) {
+        idx[e] = cur;
+        for (size_t k = r; k-- > 0;) {
+            pos[k]++;
+            cur += stride[k];
+            if (pos[k] < out[k]) break;
+            cur -= stride[k] * out[k];
+            pos[k] = 0;
+        }
+    }
+    return idx;
+}
+
+}  // namespace
+
+class LatticeModel {
+   public:
+    std::vector<Dt> dtypes;
+    std::vector<Val> consts;
+    std::vector<Node> nodes;
+    std::vector<std::pair<std::string, int>> inputs, outputs;
+    std::vector<int> feat;
+    int tau = -1, output = -1, nb = 0;
+
+    int find(const std::vector<std::pair<std::string, int>>& l, const std::string& name) const {
+        for (auto& e : l)
+            if (e.first == name) return e.second;
+        return -1;
+    }
+
+    void run(std::vector<Val>& v) const {
+        for (const Node& n : nodes) exec(n, v);
+    }
+
+   private:
+    static const Val& in(const std::vector<Val>& v, const Node& n, size_t k) {
+        if (k >= n.in.size() || n.in[k] < 0 || !v[n.in[k]].set) throw GraphError{};
+        return v[n.in[k]];
+    }
+    static Val floats(std::vector<size_t> shape, std::vector<float> data) {
+        Val r;
+        r.set = true;
+        r.dt = Dt::F32;
+        r.shape = std::move(shape);
+        r.f = std::make_shared<std::vector<float>>(std::move(data));
+        return r;
+    }
+    static int norm_axis(int axis, size_t rank) {
+        const int a = axis < 0 ? axis + (int)rank : axis;
+        if (a < 0 || a >= (int)rank) throw GraphError{};
+        return a;
+    }
+
+    void exec(const Node& n, std::vector<Val>& v) const {
+        if (n.out.empty()) throw GraphError{};
+        Val r;
+        switch (n.op) {
+            case Op::Add: case Op::Sub: case Op::Mul: case Op::Min: case Op::Max: {
+                const Val &x = in(v, n, 0), &y = in(v, n, 1);
+                if (x.dt != Dt::F32 || y.dt != Dt::F32) throw GraphError{};
+                auto shape = bshape(x.shape, y.shape);
+                auto ix = bindex(x.shape, shape), iy = bindex(y.shape, shape);
+                std::vector<float> o(ix.size());
+                for (size_t e = 0; e < o.size(); e++) {
+                    const float p = (*x.f)[ix[e]], q = (*y.f)[iy[e]];
+                    float z;
+                    switch (n.op) {
+                        case Op::Add: z = p + q; break;
+                        case Op::Sub: z = p - q; break;
+                        case Op::Mul: z = p * q; break;
+                        case Op::Min: z = fminf(p, q); break;
+                        default: z = fmaxf(p, q); break;
+                    }
+                    o[e] = act_apply(n.act, z);
+                }
+                r = floats(shape, std::move(o));
+                break;
+            }
+            case Op::Equal: {
+                const Val &x = in(v, n, 0), &y = in(v, n, 1);
+                if (x.dt != y.dt || x.dt == Dt::Bool) throw GraphError{};
+                auto shape = bshape(x.shape, y.shape);
+                auto ix = bindex(x.shape, shape), iy = bindex(y.shape, shape);
+                auto o = std::make_shared<std::vector<uint8_t>>(ix.size());
+                for (size_t e = 0; e < ix.size(); e++)
+                    (*o)[e] = x.dt == Dt::F32 ? (*x.f)[ix[e]] == (*y.f)[iy[e]] : (*x.i)[ix[e]] == (*y.i)[iy[e]];
+                r.set = true;
+                r.dt = Dt::Bool;
+                r.shape = shape;
+                r.b = o;
+                break;
+            }
+            case Op::Relu: case Op::Relu6: case Op::Abs: {
+                const Val& x = in(v, n, 0);
+                if (x.dt != Dt::F32) throw GraphError{};
+                std::vector<float> o(x.f->size());
+                for (size_t e = 0; e < o.size(); e++) {
+                    const float a = (*x.f)[e];
+                    o[e] = n.op == Op::Relu ? fmaxf(a, 0.0f) : n.op == Op::Relu6 ? fmaxf(0.0f, fminf(a, 6.0f)) : fabsf(a);
+                }
+                r = floats(x.shape, std::move(o));
+                break;
+            }
+            case Op::Cast: {
+                const Val& x = in(v, n, 0);
+                const Dt to = dtypes[n.out[0]];
+                const size_t cnt = x.count();
+                r.set = true;
+                r.dt = to;
+                r.shape = x.shape;
+                auto get = [&](size_t e) -> double {
+                    if (x.dt == Dt::F32) return (*x.f)[e];
+                    if (x.dt == Dt::I32) return (*x.i)[e];
+                    return (*x.b)[e];
+                };
+                if (to == Dt::F32) {
+                    r.f = std::make_shared<std::vector<float>>(cnt);
+                    for (size_t e = 0; e < cnt; e++) (*r.f)[e] = x.dt == Dt::F32 ? (*x.f)[e] : (float)get(e);
+                } else if (to == Dt::I32) {
+                    r.i = std::make_shared<std::vector<int32_t>>(cnt);
+                    for (size_t e = 0; e < cnt; e++) {
+                        if (x.dt == Dt::I32) { (*r.i)[e] = (*x.i)[e]; continue; }
+                        const double t = trunc(get(e));  // toward zero, saturating (tflite.zig:595-596)
+                        (*r.i)[e] = isnan(t) ? 0 : t >= 2147483647.0 ? 2147483647 : t <= -2147483648.0 ? INT32_MIN : (int32_t)t;
+                    }
+                } else if (to == Dt::Bool) {
+                    r.b = std::make_shared<std::vector<uint8_t>>(cnt);
+                    for (size_t e = 0; e < cnt; e++) (*r.b)[e] = get(e) != 0;
+                } else {
+                    throw GraphError{};
+                }
+                break;
+            }
+            case Op::Concat: {
+                if (n.in.empty()) throw GraphError{};
+                const Val& first = in(v, n, 0);
+                const int ax = norm_axis(n.axis, first.shape.size());
+                std::vector<size_t> shape = first.shape;
+                shape[ax] = 0;
+                for (size_t k = 0; k < n.in.size(); k++) {
+                    const Val& p = in(v, n, k);
+                    if (p.dt != Dt::F32 || p.shape.size() != shape.size()) throw GraphError{};
+                    for (size_t d = 0; d < shape.size(); d++)
+                        if ((int)d != ax && p.shape[d] != first.shape[d]) throw GraphError{};
+                    shape[ax] += p.shape[ax];
+                }
+                size_t outer = 1, inner = 1;
+                for (int d = 0; d < ax; d++) outer *= shape[d];
+                for (size_t d = ax + 1; d < shape.size(); d++) inner *= shape[d];
+                std::vector<float> o;
+                o.reserve(outer * shape[ax] * inner);
+                for (size_t oi = 0; oi < outer; oi++)
+                    for (size_t k = 0; k < n.in.size(); k++) {
+                        const Val& p = in(v, n, k);
+                        const size_t chunk = p.shape[ax] * inner;
+                        for (size_t e = 0; e < chunk; e++) o.push_back(act_apply(n.act, (*p.f)[oi * chunk + e]));
+                    }
+                r = floats(shape, std::move(o));
+                break;
+            }
+            case Op::FullyConnected: {
+                const Val &x = in(v, n, 0), &w = in(v, n, 1);
+                const Val* bias = (n.in.size() > 2 && n.in[2] >= 0) ? &in(v, n, 2) : nullptr;
+                if (x.dt != Dt::F32 || w.dt != Dt::F32 || w.shape.size() != 2) throw GraphError{};
+                const size_t nout = w.shape[0], nin = w.shape[1];
+                if (nin == 0 || x.f->size() % nin) throw GraphError{};
+                if (bias && (bias->dt != Dt::F32 || bias->f->size() != nout)) throw GraphError{};
+                const size_t batch = x.f->size() / nin;
+                std::vector<float> o(batch * nout);
+                for (size_t rr = 0; rr < batch; rr++)
+                    for (size_t oo = 0; oo < nout; oo++) {
+                        const float* xr = x.f->data() + rr * nin;
+                        const float* wr = w.f->data() + oo * nin;
+                        float acc = 0.0f;
+                        for (size_t k = 0; k < nin; k++) acc = fmaf(xr[k], wr[k], acc);
+                        if (bias) acc += (*bias->f)[oo];
+                        o[rr * nout + oo] = act_apply(n.act, acc);
+                    }
+                std::vector<size_t> shape;
+                if (n.keep_dims) {
+                    shape = x.shape;
+                    shape.back() = nout;
+                } else {
+                    shape = {batch, nout};
+                }
+                r = floats(shape, std::move(o));
+                break;
+            }
+            case Op::BatchMatMul: {
+                const Val &x = in(v, n, 0), &y = in(v, n, 1);
+                if (x.dt != Dt::F32 || y.dt != Dt::F32 || x.shape.size() < 2 || y.shape.size() < 2) throw GraphError{};
+                const size_t xr = x.shape.size(), yr = y.shape.size();
+                const size_t m = n.adj_x ? x.shape[xr - 1] : x.shape[xr - 2];
+                const size_t kk = n.adj_x ? x.shape[xr - 2] : x.shape[xr - 1];
+                const size_t k2 = n.adj_y ? y.shape[yr - 1] : y.shape[yr - 2];
+                const size_t nn = n.adj_y ? y.shape[yr - 2] : y.shape[yr - 1];
+                if (kk != k2) throw GraphError{};
+                std::vector<size_t> xb(x.shape.begin(), x.shape.end() - 2), yb(y.shape.begin(), y.shape.end() - 2);
+                auto ob = bshape(xb, yb);
+                auto ix = bindex(xb, ob), iy = bindex(yb, ob);
+                std::vector<float> o(ix.size() * m * nn);
+                for (size_t bi = 0; bi < ix.size(); bi++) {
+                    const float* xm = x.f->data() + ix[bi] * m * kk;
+                    const float* ym = y.f->data() + iy[bi] * kk * nn;
+                    for (size_t a = 0; a < m; a++)
+                        for (size_t c = 0; c < nn; c++) {
+                            float acc = 0.0f;
+                            for (size_t l = 0; l < kk; l++) {
+                                const float p = n.adj_x ? xm[l * m + a] : xm[a * kk + l];
+                                const float q = n.adj_y ? ym[c * kk + l] : ym[l * nn + c];
+                                acc = fmaf(p, q, acc);
+                            }
+                            o[bi * m * nn + a * nn + c] = acc;
+                        }
+                }
+                if (ob.size() + 2 > 6) throw GraphError{};
+                ob.push_back(m);
+                ob.push_back(nn);
+                r = floats(ob, std::move(o));
+                break;
+            }
+            case Op::Reshape: {
+                const Val& x = in(v, n, 0);
+                std::vector<int32_t> spec;
+                if (n.in.size() > 1 && n.in[1] >= 0) {
+                    const Val& s = in(v, n, 1);
+                    if (s.dt != Dt::I32) throw GraphError{};
+                    spec = *s.i;
+                } else if (n.has_shape) {
+                    spec = n.new_shape;
+                } else {
+                    throw GraphError{};
+                }
+                if (spec.size() > 6) throw GraphError{};
+                const size_t total = x.count();
+                size_t known = 1;
+                int infer = -1;
+                std::vector<size_t> shape(spec.size());
+                for (size_t k = 0; k < spec.size(); k++) {
+                    if (spec[k] == -1) {
+                        if (infer >= 0) throw GraphError{};
+                        infer = (int)k;
+                    } else if (spec[k] < 0) {
+                        throw GraphError{};
+                    } else {
+                        shape[k] = (size_t)spec[k];
+                        known *= shape[k];
+                    }
+                }
+                if (infer >= 0) {
+                    if (known == 0 || total % known) throw GraphError{};
+                    shape[infer] = total / known;
+                }
+                r = x;
+                r.shape = shape;
+                if (r.count() != total) throw GraphError{};
+                break;
+            }
+            case Op::ExpandDims: {
+                const Val &x = in(v, n, 0), &a = in(v, n, 1);
+                if (a.dt != Dt::I32 || a.i->size() != 1) throw GraphError{};
+                const int rank = (int)x.shape.size();
+                const int ax = (*a.i)[0] < 0 ? (*a.i)[0] + rank + 1 : (*a.i)[0];
+                if (ax < 0 || ax > rank || rank >= 6) throw GraphError{};
+                r = x;
+                r.shape.insert(r.shape.begin() + ax, 1);
+                break;
+            }
+            case Op::Shape: {
+                const Val& x = in(v, n, 0);
+                r.set = true;
+                r.dt = Dt::I32;
+                r.shape = {x.shape.size()};
+                r.i = std::make_shared<std::vector<int32_t>>();
+                for (size_t d : x.shape) r.i->push_back((int32_t)d);
+                break;
+            }
+            case Op::Fill: {
+                const Val &d = in(v, n, 0), &x = in(v, n, 1);
+                if (d.dt != Dt::I32 || d.i->size() > 6 || x.count() != 1) throw GraphError{};
+                r.set = true;
+                r.dt = x.dt;
+                for (int32_t k : *d.i) {
+                    if (k < 0) throw GraphError{};
+                    r.shape.push_back((size_t)k);
+                }
+                const size_t cnt = r.count();
+                if (x.dt == Dt::F32) r.f = std::make_shared<std::vector<float>>(cnt, (*x.f)[0]);
+                else if (x.dt == Dt::I32) r.i = std::make_shared<std::vector<int32_t>>(cnt, (*x.i)[0]);
+                else r.b = std::make_shared<std::vector<uint8_t>>(cnt, (*x.b)[0]);
+                break;
+            }
+            case Op::Split: {
+                const Val &a = in(v, n, 0), &x = in(v, n, 1);
+                if (a.dt != Dt::I32 || a.i->size() != 1 || x.dt != Dt::F32) throw GraphError{};
+                const int ax = norm_axis((*a.i)[0], x.shape.size());
+                const size_t parts = n.out.size();
+                if (parts == 0 || x.shape[ax] % parts) throw GraphError{};
+                std::vector<size_t> shape = x.shape;
+                shape[ax] /= parts;
+                size_t outer = 1, inner = 1;
+                for (int d = 0; d < ax; d++) outer *= x.shape[d];
+                for (size_t d = ax + 1; d < x.shape.size(); d++) inner *= x.shape[d];
+                const size_t chunk = shape[ax] * inner;
+                for (size_t p = 0; p < parts; p++) {
+                    std::vector<float> o(outer * chunk);
+                    for (size_t oi = 0; oi < outer; oi++)
+                        memcpy(&o[oi * chunk], x.f->data() + (oi * parts + p) * chunk, sizeof(float) * chunk);
+                    v[n.out[p]] = floats(shape, std::move(o));
+                }
+                return;
+            }
+            default: throw GraphError{};
+        }
+        v[n.out[0]] = r;
+    }
+};
+
+void LatticeDeleter::operator()(LatticeModel* m) const { delete m; }
+
+static Dt dt_of(int t) { return t == 0 ? Dt::F32 : t == 2 ? Dt::I32 : t == 6 ? Dt::Bool : Dt::None; }
+
+// tflite.zig:232-360 and lattice.zig:23-41
+std::unique_ptr<LatticeModel, LatticeDeleter> load_lattice(const std::string& bytes, int nb, std::string& err) {
+    std::unique_ptr<LatticeModel, LatticeDeleter> m(new LatticeModel());
+    try {
+        Fb fb{(const uint8_t*)bytes.data(), bytes.size()};
+        const size_t root = fb.at<uint32_t>(0);
+        auto codes_v = fb.vec(root, 1), subgraphs = fb.vec(root, 2), buffers = fb.vec(root, 4);
+        if (codes_v.second < 0 || subgraphs.second < 1 || buffers.second < 0) throw FbError{};
+        const size_t g = fb.deref(subgraphs.first);
+        std::vector<Op> codes(codes_v.second);
+        for (long long i = 0; i < codes_v.second; i++) {
+            const size_t t = fb.deref(codes_v.first + 4 * i);
+            if (fb.field(t, 1)) { err = "UnsupportedOp"; return nullptr; }
+            const int dep = fb.scalar<int8_t>(t, 0, 0), bi = fb.scalar<int32_t>(t, 3, 0);
+            codes[i] = op_of(std::max(dep, bi));
+        }
+        auto tens = fb.vec(g, 0);
+        if (tens.second < 0) throw FbError{};
+        const size_t nt = (size_t)tens.second;
+        std::vector<std::string> names(nt);
+        m->dtypes.resize(nt);
+        m->consts.resize(nt);
+        for (size_t i = 0; i < nt; i++) {
+            const size_t t = fb.deref(tens.first + 4 * i);
+            names[i] = fb.str(t, 3);
+            m->dtypes[i] = dt_of(fb.scalar<int8_t>(t, 1, 0));
+            const uint32_t bidx = fb.scalar<uint32_t>(t, 2, 0);
+            if (bidx == 0 || bidx >= (uint64_t)buffers.second) continue;
+            const size_t bt = fb.deref(buffers.first + 4 * (size_t)bidx);
+            const uint8_t* data = nullptr;
+            size_t dlen = 0;
+            auto dv = fb.vec(bt, 0);
+            if (dv.second > 0) {
+                if (dv.first > fb.n || fb.n - dv.first < (size_t)dv.second) throw FbError{};
+                data = fb.p + dv.first;
+                dlen = (size_t)dv.second;
+            }
+            if (dlen == 0) {
+                const uint64_t off = fb.scalar<uint64_t>(bt, 1, 0), size = fb.scalar<uint64_t>(bt, 2, 0);
+                if (off > 1 && size > 0) {
+                    if (off > fb.n || fb.n - off < size) throw FbError{};
+                    data = fb.p + off;
+                    dlen = (size_t)size;
+                }
+            }
+            if (dlen == 0) continue;
+            Val c;
+            c.set = true;
+            c.dt = m->dtypes[i];
+            auto sv = fb.vec(t, 0);
+            if (sv.second > 6) { err = "UnsupportedType"; return nullptr; }
+            for (long long d = 0; d < sv.second; d++) {
+                const int32_t x = fb.at<int32_t>(sv.first + 4 * d);
+                if (x < 0) throw FbError{};
+                c.shape.push_back((size_t)x);
+            }
+            const size_t cnt = c.count();
+            if (c.dt == Dt::None) { err = "UnsupportedType"; return nullptr; }
+            if (c.dt == Dt::Bool) {
+                if (dlen != cnt) throw FbError{};
+                c.b = std::make_shared<std::vector<uint8_t>>(cnt);
+                for (size_t e = 0; e < cnt; e++) (*c.b)[e] = data[e] != 0;
+            } else {
+                if (dlen != cnt * 4) throw FbError{};
+                if (c.dt == Dt::F32) {
+                    c.f = std::make_shared<std::vector<float>>(cnt);
+                    memcpy(c.f->data(), data, 4 * cnt);
+                } else {
+                    c.i = std::make_shared<std::vector<int32_t>>(cnt);
+                    memcpy(c.i->data(), data, 4 * cnt);
+                }
+            }
+            m->consts[i] = c;
+        }
+        auto ops = fb.vec(g, 3);
+        if (ops.second < 0) throw FbError{};
+        auto tensor_list = [&](size_t t, int id) {
+            std::vector<int> l;
+            auto v = fb.vec(t, id);
+            for (long long k = 0; k < v.second; k++) {
+                const int32_t x = fb.at<int32_t>(v.first + 4 * k);
+                if (x >= (int64_t)nt) throw FbError{};
+                l.push_back(x);
+            }
+            return l;
+        };
+        for (long long i = 0; i < ops.second; i++) {
+            const size_t t = fb.deref(ops.first + 4 * i);
+            const uint32_t ci = fb.scalar<uint32_t>(t, 0, 0);
+            if (ci >= codes.size()) throw FbError{};
+            Node n;
+            n.op = codes[ci];
+            if (n.op == Op::Unknown) { err = "UnsupportedOp"; return nullptr; }
+            n.in = tensor_list(t, 1);
+            n.out = tensor_list(t, 2);
+            for (int o : n.out)
+                if (o < 0) throw FbError{};
+            const size_t opt = fb.table(t, 4);
+            auto act = [&](int id) {
+                const int a = fb.scalar<int8_t>(opt, id, 0);
+                if (a < 0 || a > 3) throw FbError{};
+                return a;
+            };
+            if (opt) switch (n.op) {
+                    case Op::Add: case Op::Sub: case Op::Mul: n.act = act(0); break;
+                    case Op::Concat:
+                        n.axis = fb.scalar<int32_t>(opt, 0, 0);
+                        n.act = act(1);
+                        break;
+                    case Op::FullyConnected:
+                        n.act = act(0);
+                        if (fb.scalar<int8_t>(opt, 1, 0) != 0) { err = "UnsupportedOp"; return nullptr; }
+                        n.keep_dims = fb.scalar<uint8_t>(opt, 2, 0) != 0;
+                        break;
+                    case Op::BatchMatMul:
+                        n.adj_x = fb.scalar<uint8_t>(opt, 0, 0) != 0;
+                        n.adj_y = fb.scalar<uint8_t>(opt, 1, 0) != 0;
+                        break;
+                    case Op::Reshape: {
+                        auto v = fb.vec(opt, 0);
+                        if (v.second >= 0) {
+                            n.has_shape = true;
+                            for (long long d = 0; d < v.second; d++) n.new_shape.push_back(fb.at<int32_t>(v.first + 4 * d));
+                        }
+                        break;
+                    }
+                    default: break;
+                }
+            m->nodes.push_back(std::move(n));
+        }
+        auto sigs = fb.vec(root, 7);
+        for (long long s = 0; s < sigs.second; s++) {
+            const size_t st = fb.deref(sigs.first + 4 * s);
+            if (fb.str(st, 2) != "predict" && sigs.second > 1) continue;
+            if (fb.scalar<uint32_t>(st, 4, 0) != 0) { err = "UnsupportedOp"; return nullptr; }
+            for (int which = 0; which < 2; which++) {
+                auto mv = fb.vec(st, which);
+                for (long long k = 0; k < mv.second; k++) {
+                    const size_t e = fb.deref(mv.first + 4 * k);
+                    const uint32_t ti = fb.scalar<uint32_t>(e, 1, 0);
+                    if (ti >= nt) throw FbError{};
+                    (which == 0 ? m->inputs : m->outputs).emplace_back(fb.str(e, 0), (int)ti);
+                }
+            }
+            break;
+        }
+        if (m->inputs.empty()) {
+            for (int which = 1; which <= 2; which++) {
+                auto v = fb.vec(g, which);
+                for (long long k = 0; k < v.second; k++) {
+                    const int32_t ti = fb.at<int32_t>(v.first + 4 * k);
+                    if (ti < 0 || (size_t)ti >= nt) throw FbError{};
+                    (which == 1 ? m->inputs : m->outputs).emplace_back(names[ti], ti);
+                }
+            }
+        }
+    } catch (const FbError&) {
+        err = "InvalidModel";
+        return nullptr;
+    }
+    static const char* kFeatures[4] = {"fvnsim", "fvnsim10_", "fstdnsim", "fvdegenergy"};
+    m->nb = nb;
+    for (int f = 0; f < 4; f++)
+        for (int b = 0; b < nb; b++) {
+            const int t = m->find(m->inputs, std::string(kFeatures[f]) + std::to_string(b));
+            if (t < 0) { err = "MissingInput"; return nullptr; }
+            m->feat.push_back(t);
+        }
+    m->tau = m->find(m->inputs, "tau");
+    m->output = m->find(m->outputs, "predictions");
+    if (m->tau < 0 || m->output < 0) { err = "MissingInput"; return nullptr; }
+    return m;
+}
+
+bool lattice_predict(const LatticeModel& m, const double* f0, const double* f1, const double* f2, const double* f3,
+                     double& out) {
+    std::vector<Val> v = m.consts;
+    const double* feats[4] = {f0, f1, f2, f3};
+    for (int f = 0; f < 4; f++)
+        for (int b = 0; b < m.nb; b++) {
+            Val s;
+            s.set = true;
+            s.dt = Dt::F32;
+            s.shape = {1};
+            s.f = std::make_shared<std::vector<float>>(1, (float)feats[f][b]);
+            v[m.feat[f * m.nb + b]] = s;
+        }
+    Val t;
+    t.set = true;
+    t.dt = Dt::F32;
+    t.shape = {1};
+    t.f = std::make_shared<std::vector<float>>(1, 0.5f);
+    v[m.tau] = t;
+    try {
+        m.run(v);
+    } catch (const GraphError&) {
+        return false;
+    } catch (const FbError&) {
+        return false;
+    }
+    const Val& o = v[m.output];
+    if (!o.set || o.dt != Dt::F32 || o.f->empty()) return false;
+    out = (double)(*o.f)[0];
+    return true;
+}
+
+}  // namespace vq

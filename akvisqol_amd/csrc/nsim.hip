@@ -1,0 +1,384 @@
+// Spectrogram preparation, NSIM similarity table, DP patch search and
+// per-band patch statistics on gfx950.
+//
+// References: spectrogram.zig:14-80 (dB / floors / joint minimum),
+// nsim.zig:119-510 (taps, DegPlane, measureSimilarityAt,
+// measurePatchSimilarity), dp.zig:205-355 (DP + backtrack),
+// accel.zig:126-146 (table over the search windows).
+#include <float.h>
+
+#include "dev_common.h"
+#include "kernels.h"
+
+namespace vq {
+
+// ---------------------------------------------------------------- dB prep
+__global__ __launch_bounds__(256) void k_db(const PrepJob* __restrict__ jobs, int nb) {
+    const PrepJob j = jobs[blockIdx.y];
+    const long long nr = (long long)nb * j.nfr, nd = (long long)nb * j.nfd;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nr + nd) return;
+    double* p = e < nr ? j.ref + e : j.deg + (e - nr);
+    const double v = *p;
+    const double x = v == 0.0 ? DBL_EPSILON : fabs(v);
+    const double db = 10.0 * fdlibm_log10(x);
+    *p = fmax(db, -45.0);
+}
+
+__global__ __launch_bounds__(256) void k_frame_floor(const PrepJob* __restrict__ jobs, int nb) {
+    const PrepJob j = jobs[blockIdx.y];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mf = j.nfr < j.nfd ? j.nfr : j.nfd;
+    if (c >= mf) return;
+    double col_max = -INFINITY;
+    for (int r = 0; r < nb; r++) {
+        const double m = fmax(j.ref[(long long)r * j.nfr + c], j.deg[(long long)r * j.nfd + c]);
+        if (m > col_max) col_max = m;
+    }
+    const double floor_db = col_max - 45.0;
+    for (int r = 0; r < nb; r++) {
+        double* a = &j.ref[(long long)r * j.nfr + c];
+        double* b = &j.deg[(long long)r * j.nfd + c];
+        if (*a < floor_db) *a = floor_db;
+        if (*b < floor_db) *b = floor_db;
+    }
+}
+
+__global__ void k_min_init(unsigned long long* scratch, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) scratch[i] = ~0ULL;
+}
+
+__global__ __launch_bounds__(256) void k_min(const PrepJob* __restrict__ jobs, int nb, unsigned long long* scratch) {
+    const PrepJob j = jobs[blockIdx.y];
+    const long long nr = (long long)nb * j.nfr, nd = (long long)nb * j.nfd;
+    unsigned long long best = ~0ULL;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nr + nd; e += (long long)gridDim.x * blockDim.x) {
+        const unsigned long long k = order_key(e < nr ? j.ref[e] : j.deg[e - nr]);
+        best = k < best ? k : best;
+    }
+    __shared__ unsigned long long red[256];
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] < red[threadIdx.x + w] ? red[threadIdx.x] : red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMin(&scratch[blockIdx.y], red[0]);
+}
+
+__global__ __launch_bounds__(256) void k_sub_min(const PrepJob* __restrict__ jobs, int nb, const unsigned long long* scratch) {
+    const PrepJob j = jobs[blockIdx.y];
+    const long long nr = (long long)nb * j.nfr, nd = (long long)nb * j.nfd;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nr + nd) return;
+    const double lo = key_value(scratch[blockIdx.y]);
+    double* p = e < nr ? j.ref + e : j.deg + (e - nr);
+    *p = *p - lo;
+}
+
+void launch_prep(const PrepJob* d_jobs, int njobs, int nb, int max_nf, unsigned long long* d_scratch, hipStream_t stream) {
+    if (njobs == 0) return;
+    const long long cells = 2LL * nb * max_nf;
+    const dim3 ge((unsigned)((cells + 255) / 256), njobs);
+    hipLaunchKernelGGL(k_db, ge, dim3(256), 0, stream, d_jobs, nb);
+    hipLaunchKernelGGL(k_frame_floor, dim3((unsigned)((max_nf + 255) / 256), njobs), dim3(256), 0, stream, d_jobs, nb);
+    hipLaunchKernelGGL(k_min_init, dim3((njobs + 255) / 256), dim3(256), 0, stream, d_scratch, njobs);
+    unsigned mb = (unsigned)((cells + 255) / 256);
+    if (mb > 64) mb = 64;
+    hipLaunchKernelGGL(k_min, dim3(mb, njobs), dim3(256), 0, stream, d_jobs, nb, d_scratch);
+    hipLaunchKernelGGL(k_sub_min, ge, dim3(256), 0, stream, d_jobs, nb, d_scratch);
+}
+
+// ---------------------------------------------------------------- deg plane
+// nsim.zig:236-263: the deg spectrogram zero padded to `width` columns, its
+// square, and their replicate-padded 3x3 convolutions.
+__global__ __launch_bounds__(256) void k_plane_fill(const PlaneJob* __restrict__ jobs, int nb) {
+    const PlaneJob j = jobs[blockIdx.y];
+    const long long n = (long long)nb * j.width;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int r = (int)(e / j.width), c = (int)(e % j.width);
+    const double v = c < j.nf ? j.spec[(long long)r * j.nf + c] : 0.0;
+    j.plane[e] = v;
+    j.plane[n + e] = v * v;
+}
+
+__global__ __launch_bounds__(256) void k_plane_conv(const PlaneJob* __restrict__ jobs, int nb) {
+    const PlaneJob j = jobs[blockIdx.y];
+    const long long n = (long long)nb * j.width;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int w = j.width;
+    const int r = (int)(e / w), c = (int)(e % w);
+    const int rm = r == 0 ? 0 : r - 1, rp = r + 1 >= nb ? nb - 1 : r + 1;
+    int cm, cp;
+    if (w == 1) { cm = 0; cp = 0; }
+    else { cm = c == 0 ? 0 : c - 1; cp = c + 1 >= w ? w - 1 : c + 1; }
+    const double* pad = j.plane;
+    const double* sq = j.plane + n;
+    const double mu = tap9([&](int rr, int cc) { return pad[(long long)rr * w + cc]; }, rm, r, rp, cm, c, cp);
+    const double csq = tap9([&](int rr, int cc) { return sq[(long long)rr * w + cc]; }, rm, r, rp, cm, c, cp);
+    const double dms = mu * mu;
+    j.plane[2 * n + e] = mu;
+    j.plane[3 * n + e] = dms;
+    j.plane[4 * n + e] = csq - dms;
+}
+
+void launch_planes(const PlaneJob* d_jobs, int njobs, int nb, int max_width, hipStream_t stream) {
+    if (njobs == 0) return;
+    const dim3 g((unsigned)(((long long)nb * max_width + 255) / 256), njobs);
+    hipLaunchKernelGGL(k_plane_fill, g, dim3(256), 0, stream, d_jobs, nb);
+    hipLaunchKernelGGL(k_plane_conv, g, dim3(256), 0, stream, d_jobs, nb);
+}
+
+// ---------------------------------------------------------------- similarity table
+// Ref-patch terms (nsim.zig:318-329): mu_ref, mu_ref², conv(ref²) - mu_ref²,
+// with the patch's own replicate padding; terms[(p*3 + t)*nb*S + r*S + c].
+__global__ __launch_bounds__(256) void k_ref_terms(const SimJob* __restrict__ jobs, int nb, int S, double* __restrict__ terms,
+                                                   int max_np) {
+    const SimJob j = jobs[blockIdx.z];
+    const int p = blockIdx.y;
+    if (p >= j.np) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * S) return;
+    const int r = e / S, c = e % S;
+    const int rm = r == 0 ? 0 : r - 1, rp = r + 1 >= nb ? nb - 1 : r + 1;
+    const int cm = S == 1 ? 0 : (c == 0 ? 0 : c - 1), cp = S == 1 ? 0 : (c + 1 >= S ? S - 1 : c + 1);
+    const double* ref = j.ref_spec + j.ref_idx[p];
+    const long long st = j.nf_ref;
+    const double mu = tap9([&](int rr, int cc) { return ref[rr * st + cc]; }, rm, r, rp, cm, c, cp);
+    const double crs = tap9([&](int rr, int cc) { const double v = ref[rr * st + cc]; return v * v; }, rm, r, rp, cm, c, cp);
+    const double rms = mu * mu;
+    double* t = terms + ((long long)blockIdx.z * max_np + p) * 3 * nb * S;
+    t[e] = mu;
+    t[nb * S + e] = rms;
+    t[2 * nb * S + e] = crs - rms;
+}
+
+// measureSimilarityAt (nsim.zig:364-424) for one (patch, offset) per lane:
+// cells in row-major order, summed serially as the reference sums them.
+template <bool FAST>
+__global__ __launch_bounds__(64) void k_sim_table(const SimJob* __restrict__ jobs, int nb, int S,
+                                                  const double* __restrict__ terms, int max_np) {
+    const SimJob j = jobs[blockIdx.z];
+    const int p = blockIdx.y;
+    if (p >= j.np) return;
+    const int lo = j.lo[p], hi = j.hi[p];
+    const int off = lo + blockIdx.x * 64 + threadIdx.x;
+    if (lo + (int)blockIdx.x * 64 > hi) return;
+    const bool active = off <= hi;
+    const int w = j.width;
+    const long long n = (long long)nb * w;
+    const double* pad = j.plane;
+    const double* sq = j.plane + n;
+    const double* mu = j.plane + 2 * n;
+    const double* dmsp = j.plane + 3 * n;
+    const double* sdsp = j.plane + 4 * n;
+    const double* ref = j.ref_spec + j.ref_idx[p];
+    const long long rst = j.nf_ref;
+    const double* t = terms + ((long long)blockIdx.z * max_np + p) * 3 * nb * S;
+    const int o = active ? off : lo;  // idle lanes shadow a valid column
+    double sum = 0.0;
+    for (int r = 0; r < nb; r++) {
+        const int rm = r == 0 ? 0 : r - 1, rp = r + 1 >= nb ? nb - 1 : r + 1;
+        for (int c = 0; c < S; c++) {
+            const int cm = S == 1 ? 0 : (c == 0 ? 0 : c - 1);
+            const int cp = S == 1 ? 0 : (c + 1 >= S ? S - 1 : c + 1);
+            const double crd = tap9(
+                [&](int rr, int cc) { return ref[rr * rst + cc] * pad[(long long)rr * w + o + cc]; }, rm, r, rp, cm, c, cp);
+            double md, dms, sds;
+            if (c == 0 || c == S - 1) {
+                md = tap9([&](int rr, int cc) { return pad[(long long)rr * w + o + cc]; }, rm, r, rp, cm, c, cp);
+                const double cdsq = tap9([&](int rr, int cc) { return sq[(long long)rr * w + o + cc]; }, rm, r, rp, cm, c, cp);
+                dms = md * md;
+                sds = cdsq - dms;
+            } else {
+                const long long q = (long long)r * w + o + c;
+                md = mu[q];
+                dms = dmsp[q];
+                sds = sdsp[q];
+            }
+            const int e = r * S + c;
+            sum += sim_cell(t[e], t[nb * S + e], t[2 * nb * S + e], md, dms, sds, crd);
+        }
+    }
+    if (active) j.sims[(long long)p * j.nf_deg + off] = sum / (double)(nb * S);
+}
+
+// ---------------------------------------------------------------- DP
+// dp.zig:227-352 for one pair per block.  Row p's running max over
+// dp[p-1][lo_{p-1} .. s) with `>=` (the later index wins ties) is an
+// exclusive scan under the associative "last argmax" operator, so each
+// row is one block-wide scan plus an elementwise update; the decisions are
+// exact because max never rounds.
+struct ArgMax {
+    double v;
+    int i;
+};
+__device__ inline ArgMax later_max(ArgMax a, ArgMax b) { return b.v >= a.v ? b : a; }
+
+__global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
+    const DpJob j = jobs[blockIdx.x];
+    const int cols = j.nf_deg;
+    __shared__ double agg_v[256];
+    __shared__ int agg_i[256];
+    for (int p = 0; p < j.np; p++) {
+        const int lo = j.lo[p], hi = j.hi[p];
+        double* row = j.dp + (long long)p * cols;
+        int* brow = j.bt + (long long)p * cols;
+        const double* srow = j.sims + (long long)p * cols;
+        if (p == 0) {
+            for (int s = lo + threadIdx.x; s <= hi; s += blockDim.x) {
+                row[s] = srow[s];
+                brow[s] = -1;
+            }
+        } else {
+            const double* prev = j.dp + (long long)(p - 1) * cols;
+            const int t0 = j.lo[p - 1];
+            const int m = hi - t0 + 1;  // scan elements t0..hi
+            const int per = (m + blockDim.x - 1) / blockDim.x;
+            const int a = t0 + threadIdx.x * per;
+            const int b = min(a + per, hi + 1);
+            ArgMax acc = {-INFINITY, -1};
+            for (int t = a; t < b; t++) acc = later_max(acc, ArgMax{prev[t], t});
+            agg_v[threadIdx.x] = acc.v;
+            agg_i[threadIdx.x] = acc.i;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                ArgMax run = {-INFINITY, -1};
+                for (int k = 0; k < (int)blockDim.x; k++) {
+                    const ArgMax cur = {agg_v[k], agg_i[k]};
+                    agg_v[k] = run.v;
+                    agg_i[k] = run.i;
+                    run = later_max(run, cur);
+                }
+            }
+            __syncthreads();
+            ArgMax run = {agg_v[threadIdx.x], agg_i[threadIdx.x]};
+            for (int t = a; t < b; t++) {
+                if (t >= lo) {
+                    double sim = srow[t] + run.v;
+                    int past = run.i;
+                    const double no_move = prev[t];
+                    if (no_move > sim) {
+                        sim = no_move;
+                        past = t;
+                    }
+                    row[t] = sim;
+                    brow[t] = past < 0 ? -1 : past;
+                }
+                run = later_max(run, ArgMax{prev[t], t});
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int last = j.np - 1;
+        const double* row = j.dp + (long long)last * cols;
+        double best = -INFINITY;
+        int cur = 0;
+        for (int s = j.lo[last]; s <= j.hi[last]; s++)
+            if (row[s] > best) {
+                best = row[s];
+                cur = s;
+            }
+        for (int p = 0; p < j.np; p++) {
+            j.offsets[p] = 0;
+            j.state[p] = 0;
+        }
+        for (int p = last; p >= 0; p--) {
+            const int back = j.bt[(long long)p * cols + cur];
+            j.offsets[p] = cur;
+            j.state[p] = back == cur ? 2 : 1;
+            if (back < 0) break;
+            cur = back;
+        }
+    }
+}
+
+void launch_dp(const DpJob* d_jobs, int njobs, hipStream_t stream) {
+    if (njobs == 0) return;
+    hipLaunchKernelGGL(k_dp, dim3(njobs), dim3(256), 0, stream, d_jobs);
+}
+
+// ---------------------------------------------------------------- patch statistics
+// nsim.zig:427-510 on one (ref patch, deg patch) pair per block; lane r
+// owns band r and keeps the reference's per-band summation order.
+__global__ __launch_bounds__(64) void k_stats(const StatJob* __restrict__ jobs, int nb) {
+    extern __shared__ double lds[];
+    const StatJob j = jobs[blockIdx.x];
+    const int S = j.ncols;
+    double* R = lds;
+    double* D = lds + nb * S;
+    double* SIM = lds + 2 * nb * S;
+    for (int e = threadIdx.x; e < nb * S; e += blockDim.x) {
+        const int r = e / S, c = e % S;
+        R[e] = j.ref[(long long)r * j.ref_stride + j.ref_col + c];
+        const int dc = j.deg_col + c;
+        D[e] = dc < j.deg_valid ? j.deg[(long long)r * j.deg_stride + dc] : 0.0;
+    }
+    __syncthreads();
+    const int r = threadIdx.x;
+    if (r < nb) {
+        const double inv_ncols = 1.0 / (double)S;
+        double deg_sum = 0.0;
+        for (int c = 0; c < S; c++) deg_sum += D[r * S + c];
+        j.out[2 * nb + r] = deg_sum * inv_ncols;
+        const int rm = r == 0 ? 0 : r - 1, rp = r + 1 >= nb ? nb - 1 : r + 1;
+        double sim_sum = 0.0;
+        for (int c = 0; c < S; c++) {
+            const int cm = S == 1 ? 0 : (c == 0 ? 0 : c - 1), cp = S == 1 ? 0 : (c + 1 >= S ? S - 1 : c + 1);
+            const double mr = tap9([&](int rr, int cc) { return R[rr * S + cc]; }, rm, r, rp, cm, c, cp);
+            const double md = tap9([&](int rr, int cc) { return D[rr * S + cc]; }, rm, r, rp, cm, c, cp);
+            const double crs = tap9([&](int rr, int cc) { const double v = R[rr * S + cc]; return v * v; }, rm, r, rp, cm, c, cp);
+            const double cds = tap9([&](int rr, int cc) { const double v = D[rr * S + cc]; return v * v; }, rm, r, rp, cm, c, cp);
+            const double crd = tap9([&](int rr, int cc) { return R[rr * S + cc] * D[rr * S + cc]; }, rm, r, rp, cm, c, cp);
+            const double rms = mr * mr, dms = md * md, mrd = mr * md;
+            const double srs = crs - rms, sds = cds - dms, srd = crd - mrd;
+            const double c1 = nsim_c1(), c3 = nsim_c3();
+            const double intensity = (2.0 * mrd + c1) / (rms + dms + c1);
+            const double prod = srs * sds;
+            const double den = prod < 0.0 ? c3 : sqrt(prod) + c3;
+            const double structure = (srd + c3) / den;
+            const double sim = intensity * structure;
+            sim_sum += sim;
+            SIM[r * S + c] = sim;
+        }
+        const double mean = sim_sum * inv_ncols;
+        j.out[r] = mean;
+        if (S <= 1) {
+            j.out[nb + r] = 0.0;
+        } else {
+            double ss = 0.0;
+            for (int c = 0; c < S; c++) {
+                const double diff = SIM[r * S + c] - mean;
+                ss += diff * diff;
+            }
+            j.out[nb + r] = sqrt(ss / ((double)S - 1.0));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int b = 0; b < nb; b++) acc += j.out[b];
+        j.out[3 * nb] = acc / (double)nb;
+    }
+}
+
+void launch_stats(const StatJob* d_jobs, int njobs, int nb, int max_cols, hipStream_t stream) {
+    if (njobs == 0) return;
+    const size_t lds = sizeof(double) * 3 * nb * (size_t)max_cols;
+    hipLaunchKernelGGL(k_stats, dim3(njobs), dim3(64), lds, stream, d_jobs, nb);
+}
+
+void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int max_np, int max_window, double* d_terms,
+                      hipStream_t stream) {
+    if (njobs == 0 || max_np == 0) return;
+    hipLaunchKernelGGL(k_ref_terms, dim3((nb * patch + 255) / 256, max_np, njobs), dim3(256), 0, stream, d_jobs, nb, patch,
+                       d_terms, max_np);
+    hipLaunchKernelGGL(k_sim_table<false>, dim3((max_window + 63) / 64, max_np, njobs), dim3(64), 0, stream, d_jobs, nb,
+                       patch, (const double*)d_terms, max_np);
+}
+
+}  // namespace vq

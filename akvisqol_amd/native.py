@@ -1,0 +1,396 @@
+"""ctypes binding to libvisqol.so (include/visqol.h, include/visqol_hip.h).
+
+Mirrors the reference's Python binding (src/python/visqol/_native.py:16-234):
+the same structs, error codes and Context.measure / measure_batch
+behaviour, with "hip" added to the backend table.  The library must have
+been built (``python -m akvisqol_amd.build``); there is no fallback: a
+missing library raises ImportError, and scoring needs a gfx950 device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_uint8, c_uint32, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+MODE_AUDIO = 0
+MODE_SPEECH = 1
+
+# _native.py:19 plus this library's backend.
+BACKENDS = {"cpu": 0, "metal": 1, "cuda": 2, "vulkan": 3, "hip": 4}
+PRECISIONS = {"exact": 0, "fast": 1, "balanced": 2, "auto": 3}
+
+OK = 0
+E_INVALID_ARGUMENT = -1
+E_OUT_OF_MEMORY = -2
+E_IO = -3
+E_MODEL = -4
+E_SIGNAL = -5
+E_BACKEND = -6
+E_INTERNAL = -7
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_uint32),
+        ("search_window_radius", c_int32),
+        ("use_unscaled_speech_mos_mapping", c_uint8),
+        ("disable_global_alignment", c_uint8),
+        ("disable_realignment", c_uint8),
+        ("use_lattice_model", c_uint8),
+        ("model_path", c_char_p),
+        ("backend", c_uint32),
+        ("precision", c_uint32),
+    ]
+
+
+class Patch(ctypes.Structure):
+    _fields_ = [
+        ("similarity", c_double),
+        ("ref_patch_start_time", c_double),
+        ("ref_patch_end_time", c_double),
+        ("deg_patch_start_time", c_double),
+        ("deg_patch_end_time", c_double),
+        ("freq_band_means", POINTER(c_double)),
+        ("freq_band_stddevs", POINTER(c_double)),
+        ("freq_band_deg_energy", POINTER(c_double)),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("moslqo", c_double),
+        ("vnsim", c_double),
+        ("alignment_lag_s", c_double),
+        ("num_bands", c_size_t),
+        ("fvnsim", POINTER(c_double)),
+        ("fvnsim10", POINTER(c_double)),
+        ("fstdnsim", POINTER(c_double)),
+        ("fvdegenergy", POINTER(c_double)),
+        ("center_freq_bands", POINTER(c_double)),
+        ("num_patches", c_size_t),
+        ("patch_sims", POINTER(Patch)),
+    ]
+
+
+class Trace(ctypes.Structure):
+    _fields_ = [
+        ("lag", c_int64),
+        ("spl_factor", c_double),
+        ("nf_ref", c_size_t),
+        ("nf_deg", c_size_t),
+        ("num_patches", c_size_t),
+        ("ref_indices", c_size_t * 512),
+        ("realign_lag", c_int64 * 512),
+        ("realign_frames", c_size_t * 512),
+        ("realign_accepted", c_uint8 * 512),
+        ("match_valid", c_uint8 * 512),
+        ("dp_offset", c_size_t * 512),
+    ]
+
+
+LIB_PATH = Path(os.environ.get("VISQOL_LIBRARY", Path(__file__).resolve().parent / "libvisqol.so"))
+
+
+def _load() -> ctypes.CDLL:
+    if not LIB_PATH.is_file():
+        raise ImportError(f"libvisqol.so not found at {LIB_PATH}; build it with `python -m akvisqol_amd.build`")
+    L = ctypes.CDLL(str(LIB_PATH))
+    D = POINTER(c_double)
+    S = POINTER(c_size_t)
+    sig = {
+        "visqol_abi_version": (c_uint32, []),
+        "visqol_version": (c_char_p, []),
+        "visqol_last_error": (c_char_p, []),
+        "visqol_available_backends": (c_uint32, []),
+        "visqol_options_default": (None, [POINTER(Options)]),
+        "visqol_create": (c_int, [POINTER(Options), POINTER(c_void_p)]),
+        "visqol_destroy": (None, [c_void_p]),
+        "visqol_measure": (c_int, [c_void_p, D, c_size_t, D, c_size_t, c_uint32, POINTER(POINTER(Result))]),
+        "visqol_measure_files": (c_int, [c_void_p, c_char_p, c_char_p, POINTER(POINTER(Result))]),
+        "visqol_measure_batch": (c_int, [c_void_p, c_size_t, POINTER(D), S, POINTER(D), S, c_uint32, c_uint32,
+                                         POINTER(POINTER(Result)), POINTER(c_int32)]),
+        "visqol_measure_files_batch": (c_int, [c_void_p, c_size_t, POINTER(c_char_p), POINTER(c_char_p), c_uint32,
+                                               POINTER(POINTER(Result)), POINTER(c_int32)]),
+        "visqol_result_free": (None, [POINTER(Result)]),
+        "visqol_hip_measure_trace": (c_int, [c_void_p, D, c_size_t, D, c_size_t, c_uint32, POINTER(POINTER(Result)),
+                                             POINTER(Trace), D, D, D]),
+        "visqol_hip_measure_batch_device": (c_int, [c_void_p, c_size_t, POINTER(c_void_p), S, POINTER(c_void_p), S,
+                                                    c_uint32, POINTER(POINTER(Result)), POINTER(c_int32)]),
+        "visqol_hip_profile": (c_int, [c_void_p, c_int, c_int]),
+        "visqol_hip_profile_read": (c_int, [c_void_p, D, D, D, POINTER(c_int64)]),
+        "visqol_hip_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
+        "visqol_hip_free": (c_int, [c_void_p]),
+        "visqol_hip_upload": (c_int, [c_void_p, c_void_p, c_size_t]),
+        "visqol_hip_spectrogram": (c_int, [c_uint32, c_uint32, c_uint32, D, c_size_t, D, S]),
+        "visqol_hip_filters": (c_int, [c_uint32, c_uint32, D, D]),
+        "visqol_hip_best_lag": (c_int, [D, c_size_t, D, c_size_t, POINTER(c_int64)]),
+        "visqol_hip_math_probe": (c_int, [c_int, D, D, D, c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+ABI_VERSION = 1
+if lib.visqol_abi_version() != ABI_VERSION:
+    raise ImportError(f"libvisqol ABI {lib.visqol_abi_version()} does not match the bindings ({ABI_VERSION})")
+
+__version__ = lib.visqol_version().decode()
+
+
+class NativeError(RuntimeError):
+    """A visqol.h error code with the thread's last error message."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"visqol error {code}: {message}")
+        self.code = code
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise NativeError(rc, lib.visqol_last_error().decode(errors="replace"))
+
+
+def _f64(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def result_dict(r: Result) -> dict:
+    """A visqol_result as plain Python / numpy values (copied)."""
+    nb, npch = r.num_bands, r.num_patches
+
+    def arr(p, n):
+        return np.ctypeslib.as_array(p, shape=(n,)).copy() if n else np.zeros(0)
+
+    patches = []
+    for i in range(npch):
+        p = r.patch_sims[i]
+        patches.append(dict(
+            similarity=p.similarity, ref_patch_start_time=p.ref_patch_start_time,
+            ref_patch_end_time=p.ref_patch_end_time, deg_patch_start_time=p.deg_patch_start_time,
+            deg_patch_end_time=p.deg_patch_end_time, freq_band_means=arr(p.freq_band_means, nb),
+            freq_band_stddevs=arr(p.freq_band_stddevs, nb), freq_band_deg_energy=arr(p.freq_band_deg_energy, nb)))
+    return dict(moslqo=r.moslqo, vnsim=r.vnsim, alignment_lag_s=r.alignment_lag_s, num_bands=nb,
+                fvnsim=arr(r.fvnsim, nb), fvnsim10=arr(r.fvnsim10, nb), fstdnsim=arr(r.fstdnsim, nb),
+                fvdegenergy=arr(r.fvdegenergy, nb), center_freq_bands=arr(r.center_freq_bands, nb),
+                patch_sims=patches)
+
+
+def default_options() -> Options:
+    o = Options()
+    lib.visqol_options_default(ctypes.byref(o))
+    return o
+
+
+class Context:
+    """One visqol_ctx (_native.py:165-234); calls are serialised per context."""
+
+    def __init__(self, mode: int = MODE_AUDIO, *, use_lattice_model: bool = True, search_window_radius: int = 60,
+                 disable_global_alignment: bool = False, disable_realignment: bool = False,
+                 use_unscaled_speech_mos_mapping: bool = False, model_path: str | None = None,
+                 backend: str | int = "hip", precision: str | int = "exact"):
+        o = default_options()
+        o.mode = mode
+        o.use_lattice_model = int(use_lattice_model)
+        o.search_window_radius = search_window_radius
+        o.disable_global_alignment = int(disable_global_alignment)
+        o.disable_realignment = int(disable_realignment)
+        o.use_unscaled_speech_mos_mapping = int(use_unscaled_speech_mos_mapping)
+        o.model_path = model_path.encode() if model_path else None
+        o.backend = BACKENDS[backend] if isinstance(backend, str) else backend
+        o.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+        h = c_void_p()
+        _check(lib.visqol_create(ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._lock = threading.Lock()
+        self.mode = mode
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.visqol_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def measure(self, ref, deg, sample_rate: int) -> dict:
+        ref, deg = _f64(ref), _f64(deg)
+        out = POINTER(Result)()
+        with self._lock:
+            rc = lib.visqol_measure(self._h, _dptr(ref), ref.size, _dptr(deg), deg.size, sample_rate,
+                                    ctypes.byref(out))
+            _check(rc)
+        try:
+            return result_dict(out.contents)
+        finally:
+            lib.visqol_result_free(out)
+
+    def measure_trace(self, ref, deg, sample_rate: int, want_stages: bool = False):
+        """visqol_hip_measure_trace: (result, trace dict with optional stages)."""
+        ref, deg = _f64(ref), _f64(deg)
+        out = POINTER(Result)()
+        tr = Trace()
+        rs = ds = sims = None
+        if want_stages:
+            nb = 32 if self.mode == 0 else 21
+            hop = int(int(np.floor(sample_rate * 0.08 + 0.5)) * 0.25)
+            patch = 30 if self.mode == 0 else 20
+            nfr = ref.size // hop + 1
+            nfd = (deg.size + ref.size // 2) // hop + 1
+            rs, ds = np.zeros(nb * nfr), np.zeros(nb * nfd)
+            sims = np.zeros((nfr // patch + 2) * nfd)
+        with self._lock:
+            rc = lib.visqol_hip_measure_trace(self._h, _dptr(ref), ref.size, _dptr(deg), deg.size, sample_rate,
+                                              ctypes.byref(out), ctypes.byref(tr),
+                                              _dptr(rs) if rs is not None else None,
+                                              _dptr(ds) if ds is not None else None,
+                                              _dptr(sims) if sims is not None else None)
+            _check(rc)
+        try:
+            res = result_dict(out.contents)
+        finally:
+            lib.visqol_result_free(out)
+        npch = tr.num_patches
+        trace = dict(lag=tr.lag, spl_factor=tr.spl_factor, nf_ref=tr.nf_ref, nf_deg=tr.nf_deg, num_patches=npch,
+                     ref_indices=list(tr.ref_indices[:npch]), realign_lag=list(tr.realign_lag[:npch]),
+                     realign_frames=list(tr.realign_frames[:npch]),
+                     realign_accepted=list(tr.realign_accepted[:npch]), match_valid=list(tr.match_valid[:npch]),
+                     dp_offset=list(tr.dp_offset[:npch]))
+        if want_stages:
+            nb = 32 if self.mode == 0 else 21
+            trace["ref_spec"] = rs[: nb * tr.nf_ref].reshape(nb, tr.nf_ref).copy()
+            trace["deg_spec"] = ds[: nb * tr.nf_deg].reshape(nb, tr.nf_deg).copy()
+            trace["sims"] = sims[: npch * tr.nf_deg].reshape(npch, tr.nf_deg).copy()
+        return res, trace
+
+    def profile(self, enable: bool = True, reset: bool = True) -> None:
+        _check(lib.visqol_hip_profile(self._h, int(enable), int(reset)))
+
+    def profile_read(self) -> dict:
+        ms, fl, by = c_double(), c_double(), c_double()
+        n = c_int64()
+        _check(lib.visqol_hip_profile_read(self._h, ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by),
+                                           ctypes.byref(n)))
+        return dict(ms=ms.value, flop=fl.value, bytes=by.value, launches=n.value)
+
+    def measure_batch(self, refs, degs, sample_rate: int, num_threads: int = 0):
+        """(first error code, [result dict or None], [status]) as _native.py:215-225."""
+        refs = [_f64(r) for r in refs]
+        degs = [_f64(d) for d in degs]
+        n = len(refs)
+        D = POINTER(c_double)
+        rp = (D * n)(*[_dptr(r) for r in refs])
+        dp = (D * n)(*[_dptr(d) for d in degs])
+        rl = (c_size_t * n)(*[r.size for r in refs])
+        dl = (c_size_t * n)(*[d.size for d in degs])
+        out = (POINTER(Result) * n)()
+        st = (c_int32 * n)()
+        with self._lock:
+            rc = lib.visqol_measure_batch(self._h, n, rp, rl, dp, dl, sample_rate, 0, out, st)
+        results = []
+        for i in range(n):
+            if out[i]:
+                results.append(result_dict(out[i].contents))
+                lib.visqol_result_free(out[i])
+            else:
+                results.append(None)
+        return rc, results, list(st)
+
+
+class DeviceBuffer:
+    """A device allocation holding one f64 signal (visqol_hip_alloc)."""
+
+    def __init__(self, x):
+        x = _f64(x)
+        self.size = x.size
+        p = c_void_p()
+        _check(lib.visqol_hip_alloc(max(1, x.nbytes), ctypes.byref(p)))
+        self.ptr = p
+        _check(lib.visqol_hip_upload(p, x.ctypes.data_as(c_void_p), x.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib.visqol_hip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def measure_batch_device(ctx: Context, refs: list[DeviceBuffer], degs: list[DeviceBuffer], sample_rate: int):
+    """visqol_hip_measure_batch_device over device-resident signals."""
+    n = len(refs)
+    rp = (c_void_p * n)(*[r.ptr for r in refs])
+    dp = (c_void_p * n)(*[d.ptr for d in degs])
+    rl = (c_size_t * n)(*[r.size for r in refs])
+    dl = (c_size_t * n)(*[d.size for d in degs])
+    out = (POINTER(Result) * n)()
+    st = (c_int32 * n)()
+    with ctx._lock:
+        rc = lib.visqol_hip_measure_batch_device(ctx._h, n, rp, rl, dp, dl, sample_rate, out, st)
+    results = []
+    for i in range(n):
+        if out[i]:
+            results.append(result_dict(out[i].contents))
+            lib.visqol_result_free(out[i])
+        else:
+            results.append(None)
+    return rc, results, list(st)
+
+
+def spectrogram(signal, sample_rate: int, mode: int = MODE_AUDIO, precision: str = "exact") -> np.ndarray:
+    x = _f64(signal)
+    nb = 32 if mode == 0 else 21
+    hop = int(int(np.floor(sample_rate * 0.08 + 0.5)) * 0.25)
+    out = np.zeros(nb * (x.size // max(1, hop) + 1))
+    nf = c_size_t()
+    _check(lib.visqol_hip_spectrogram(mode, sample_rate, PRECISIONS[precision], _dptr(x), x.size, _dptr(out),
+                                      ctypes.byref(nf)))
+    return out[: nb * nf.value].reshape(nb, nf.value)
+
+
+def filters(sample_rate: int, mode: int = MODE_AUDIO):
+    nb = 32 if mode == 0 else 21
+    b = np.zeros(8 * nb)
+    cf = np.zeros(nb)
+    _check(lib.visqol_hip_filters(mode, sample_rate, _dptr(b), _dptr(cf)))
+    return b.reshape(nb, 8), cf
+
+
+def best_lag(a, b) -> int:
+    a, b = _f64(a), _f64(b)
+    lag = c_int64()
+    _check(lib.visqol_hip_best_lag(_dptr(a), a.size, _dptr(b), b.size, ctypes.byref(lag)))
+    return lag.value
+
+
+def math_probe(op: int, a, b=None) -> np.ndarray:
+    a = _f64(a)
+    bb = _f64(b) if b is not None else None
+    out = np.zeros(a.size)
+    _check(lib.visqol_hip_math_probe(op, _dptr(a), _dptr(bb) if bb is not None else None, _dptr(out), a.size))
+    return out

@@ -1,0 +1,82 @@
+/* visqol_hip.h — additive entry points of the MI355X ViSQOL library.
+ *
+ * None of these has a counterpart in jdek/akvisqol's visqol.h; they are
+ * for callers that keep signals in device memory (benchmarks, serving
+ * loops) and for stage-by-stage parity checks of the device path.  All
+ * take plain pointers and sizes; results use visqol.h's types.
+ */
+#ifndef VISQOL_HIP_H
+#define VISQOL_HIP_H
+
+#include "visqol.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Discrete decisions and shapes of one scored pair: the global lag, the
+ * SPL factor applied to the degraded signal, frame counts, reference patch
+ * starts, the DP's chosen degraded starts and validity, and per match the
+ * realignment lag (INT64_MIN when not realigned), realigned frame count
+ * and whether the realigned score replaced the DP score. */
+typedef struct {
+    int64_t lag;
+    double spl_factor;
+    size_t nf_ref, nf_deg, num_patches;
+    size_t ref_indices[512];
+    int64_t realign_lag[512];
+    size_t realign_frames[512];
+    uint8_t realign_accepted[512];
+    uint8_t match_valid[512];
+    size_t dp_offset[512];
+} visqol_hip_trace;
+
+/* visqol_measure plus the trace and, when non-NULL, the linear gammatone
+ * spectrograms ([bands][frames], nf_ref / nf_deg frames) and the DP
+ * similarity table ([patches][nf_deg]).  The output buffers must hold
+ * bands * (ref_len / hop + 1), bands * ((deg_len + ref_len / 2) / hop + 1)
+ * and (ref_len / hop / patch + 2) * ((deg_len + ref_len / 2) / hop + 1)
+ * doubles. */
+int visqol_hip_measure_trace(visqol_ctx *ctx, const double *ref, size_t ref_len, const double *deg, size_t deg_len,
+                             uint32_t sample_rate, visqol_result **out, visqol_hip_trace *trace,
+                             double *ref_spec_out, double *deg_spec_out, double *sims_out);
+
+/* visqol_measure_batch over signals already resident in device memory
+ * (hipMalloc'd on the context's device): no host-to-device copy of the
+ * samples.  Per-pair failures land in status[i]; returns the first. */
+int visqol_hip_measure_batch_device(visqol_ctx *ctx, size_t n, const double *const *dev_refs, const size_t *ref_lens,
+                                    const double *const *dev_degs, const size_t *deg_lens, uint32_t sample_rate,
+                                    visqol_result **out, int32_t *status);
+
+/* Per-context kernel timing: when enabled, HIP events on the context's
+ * stream bracket every full-signal gammatone launch; *ms is their summed
+ * device time, *flop the algorithmic work (30 flop per sample and band of
+ * every frame), *bytes the signal read plus spectrogram written, *launches
+ * the launch count since the last reset. */
+int visqol_hip_profile(visqol_ctx *ctx, int enable, int reset);
+int visqol_hip_profile_read(visqol_ctx *ctx, double *ms, double *flop, double *bytes, int64_t *launches);
+
+/* Device-resident buffers for the batch entry point above. */
+int visqol_hip_alloc(size_t bytes, void **dev_ptr);
+int visqol_hip_free(void *dev_ptr);
+int visqol_hip_upload(void *dev_dst, const void *host_src, size_t bytes);
+
+/* Stage checks, each on host arrays. */
+/* Linear gammatone spectrogram of one signal ([bands][frames]); precision
+ * as visqol.h; returns the frame count through *num_frames. */
+int visqol_hip_spectrogram(uint32_t mode, uint32_t sample_rate, uint32_t precision, const double *signal, size_t len,
+                           double *out, size_t *num_frames);
+/* Filter-bank design shared with the device: 8 doubles per band
+ * (a0 a11 a12 a13 a14 b1 b2 gain), bands ascending, plus centre
+ * frequencies. */
+int visqol_hip_filters(uint32_t mode, uint32_t sample_rate, double *bands8, double *center_freqs);
+/* FFT cross-correlation lag of the upper envelopes of a and b. */
+int visqol_hip_best_lag(const double *a, size_t la, const double *b, size_t lb, int64_t *lag);
+/* Device arithmetic probes: op 0 log10(a), 1 sqrt(a), 2 a / b. */
+int visqol_hip_math_probe(int op, const double *a, const double *b, double *out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VISQOL_HIP_H */

@@ -291,9 +291,9 @@ size_t vo_vad_patch_indices(const double *ref, size_t len, size_t nf, size_t pat
         for (size_t j = 0; j < frame_len; j++) {
             size_t idx = first + c * frame_len + j;
             double x = idx < len ? ref[idx] / max_val : 0.0;
-            double scaled = x * 32768.0;
-            if (scaled > 32767.0) scaled = 32767.0;
-            if (scaled < -32768.0) scaled = -32768.0;
+            /* std.math.clamp = @max(lo, @min(v, hi)); Zig's float min/max
+             * return the non-NaN operand, as fmin/fmax do. */
+            double scaled = fmax(-32768.0, fmin(x * 32768.0, 32767.0));
             double s = trunc(scaled);
             square += s * s;
         }

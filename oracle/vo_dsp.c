@@ -268,7 +268,7 @@ void vo_raise_floor_per_frame(double *ref, double *deg, size_t nb, size_t nfr, s
         double col_max = -INFINITY;
         for (size_t r = 0; r < nb; r++) {
             double sv = ref[r * nfr + c], ov = deg[r * nfd + c];
-            double m = sv > ov ? sv : ov;
+            double m = fmax(sv, ov); /* Zig @max: the non-NaN operand */
             if (m > col_max) col_max = m;
         }
         double floor_db = col_max - rel;

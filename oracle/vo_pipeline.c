@@ -144,7 +144,7 @@ static int map_predict(const struct vo_ctx *c, const double *fvnsim, const doubl
                        const double *fvdegenergy, size_t nb, double *mos) {
     if (c->mapper == MAP_SVR) {
         double y = vo_svr_eval(c->svr, fvnsim, nb);
-        *mos = y < 1.0 ? 1.0 : y > 5.0 ? 5.0 : y;
+        *mos = fmax(1.0, fmin(y, 5.0)); /* std.math.clamp */
         return VO_OK;
     }
     if (c->mapper == MAP_LATTICE) return vo_lattice_eval(c->lat, fvnsim, fvnsim10, fstdnsim, fvdegenergy, mos);
@@ -155,7 +155,7 @@ static int map_predict(const struct vo_ctx *c, const double *fvnsim, const doubl
     double m = a + exp(b * (nsim_mean - x0));
     double scale = c->opts.use_unscaled_speech_mos_mapping ? 1.0 : (double)1.245063f;
     double v = m * scale;
-    *mos = v < 1.0 ? 1.0 : v > 5.0 ? 5.0 : v;
+    *mos = fmax(1.0, fmin(v, 5.0));
     return VO_OK;
 }
 

@@ -1,0 +1,210 @@
+"""Parity of the gfx950 path against the CPU f64 oracle, through the C ABI.
+
+Exact mode (VISQOL_PRECISION_EXACT) is held to bit equality on every
+stage and on the final scores: the device uses the oracle's operation
+order, host-designed coefficients and the same log10 algorithm, so any
+difference is a bug.  (The stated tolerance is |dMOS| <= 1e-10 with
+identical alignments; the tests assert the stronger bit equality.)  Fast
+mode is held to |dMOS| <= 1e-6.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EXACT_MOS_TOL = 1e-10
+FAST_MOS_TOL = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def native():
+    from akvisqol_amd import native as n
+
+    return n
+
+
+@pytest.fixture(scope="module")
+def vo():
+    from oracle import vo as v
+
+    return v
+
+
+@pytest.fixture(scope="module")
+def synth():
+    from akvisqol_amd import synth as s
+
+    return s
+
+
+# ---------------------------------------------------------------- device arithmetic
+def test_device_log10_sqrt_div_bit_exact(native, vo):
+    say("probe: log10/sqrt/div on 200k values")
+    rng = np.random.default_rng(1)
+    a = np.concatenate([np.exp(rng.uniform(-700, 700, 100000)), rng.uniform(1e-18, 1e3, 100000),
+                        [2.220446049250313e-16, 1.0, 10.0, 1e-310, 5e-324]])
+    got = native.math_probe(0, a)
+    want = np.array([vo.lib().vo_log10(float(x)) for x in a])
+    assert np.array_equal(got, want)
+    got = native.math_probe(1, a)
+    assert np.array_equal(got, np.sqrt(a))
+    b = rng.uniform(-1e3, 1e3, a.size)
+    got = native.math_probe(2, a, b)
+    assert np.array_equal(got, a / b)
+
+
+# ---------------------------------------------------------------- stages
+@pytest.mark.parametrize("mode,sr,n", [(0, 48000, 3840 * 3 + 960 * 70 + 17), (1, 16000, 1280 * 2 + 320 * 41 + 5)])
+def test_spectrogram_exact_bit_identical(native, vo, mode, sr, n):
+    say(f"spectrogram exact mode={mode} sr={sr} n={n}")
+    rng = np.random.default_rng(sr + n)
+    x = rng.uniform(-1, 1, n)
+    got = native.spectrogram(x, sr, mode, "exact")
+    want = vo.spectrogram(x, sr, 32 if mode == 0 else 21, None if mode == 0 else 8000.0)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+def test_spectrogram_fast_close(native, vo):
+    say("spectrogram fast 48k")
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, 3840 + 960 * 40)
+    got = native.spectrogram(x, 48000, 0, "fast")
+    want = vo.spectrogram(x, 48000, 32)
+    rel = np.max(np.abs(got - want) / np.maximum(want, 1e-300))
+    say("fast spectrogram max rel err", rel)
+    assert rel < 1e-9
+
+
+REF14 = [2.0, 2.0, 1.0, 0.1, -3.0, 0.1, 1.0, 2.0, 2.0, 6.0, 8.0, 6.0, 2.0, 2.0]
+
+
+def test_best_lag_matches_oracle(native, vo):
+    say("best lag toys + random")
+    cases = [
+        (REF14, [1.2, 0.1, -3.3, 0.1, 1.1, 2.2, 2.1, 7.1, 8.3, 6.8, 2.4, 2.2, 2.2, 2.1]),
+        (REF14, [2.0, 2.0, 2.0, 2.0, 1.0, 0.1, -3.0, 0.1, 1.0, 2.0, 2.0, 6.0, 8.0, 6.0]),
+    ]
+    rng = np.random.default_rng(3)
+    for _ in range(8):
+        l1, l2 = int(rng.integers(50, 9000)), int(rng.integers(50, 9000))
+        a = rng.uniform(-1, 1, l1)
+        b = np.roll(np.resize(a, l2), int(rng.integers(0, 40))) + 0.1 * rng.standard_normal(l2)
+        cases.append((a, b))
+    for a, b in cases:
+        a, b = np.asarray(a, float), np.asarray(b, float)
+        ea, eb = np.zeros(a.size), np.zeros(b.size)
+        vo.lib().vo_upper_envelope(vo.dptr(vo.f64(a)), a.size, vo.dptr(ea))
+        vo.lib().vo_upper_envelope(vo.dptr(vo.f64(b)), b.size, vo.dptr(eb))
+        want = vo.lib().vo_best_lag(vo.dptr(ea), a.size, vo.dptr(eb), b.size)
+        assert native.best_lag(a, b) == want
+
+
+# ---------------------------------------------------------------- whole pipeline
+def _compare(res, tr, ores, otr, exact=True, tol=EXACT_MOS_TOL):
+    assert tr["lag"] == otr["lag"]
+    assert tr["nf_ref"] == otr["nf_ref"] and tr["nf_deg"] == otr["nf_deg"]
+    assert tr["num_patches"] == otr["num_patches"]
+    assert tr["ref_indices"] == otr["ref_indices"]
+    assert tr["dp_offset"] == otr["dp_offset"]
+    assert tr["match_valid"] == otr["match_valid"]
+    assert tr["realign_lag"] == otr["realign_lag"]
+    assert tr["realign_frames"] == otr["realign_frames"]
+    assert tr["realign_accepted"] == otr["realign_accepted"]
+    assert len(res["patch_sims"]) == len(ores["patch_sims"])
+    for p, q in zip(res["patch_sims"], ores["patch_sims"]):
+        for k in ("ref_patch_start_time", "ref_patch_end_time", "deg_patch_start_time", "deg_patch_end_time"):
+            assert p[k] == q[k], k
+    assert abs(res["moslqo"] - ores["moslqo"]) <= tol
+    if exact:
+        assert tr["spl_factor"] == otr["spl_factor"]
+        assert res["moslqo"] == ores["moslqo"]
+        assert res["vnsim"] == ores["vnsim"]
+        for k in ("fvnsim", "fvnsim10", "fstdnsim", "fvdegenergy", "center_freq_bands"):
+            assert np.array_equal(res[k], ores[k]), k
+        for p, q in zip(res["patch_sims"], ores["patch_sims"]):
+            assert p["similarity"] == q["similarity"]
+            for k in ("freq_band_means", "freq_band_stddevs", "freq_band_deg_energy"):
+                assert np.array_equal(p[k], q[k]), k
+    assert res["alignment_lag_s"] == ores["alignment_lag_s"]
+
+
+def test_audio_config0_exact_stages_and_score(native, vo, synth):
+    say("config-0 audio pair, exact, stage by stage")
+    ref, deg = synth.audio_pair(0)
+    ctx = native.Context(native.MODE_AUDIO, precision="exact")
+    res, tr = ctx.measure_trace(ref, deg, 48000, want_stages=True)
+    ores, otr = vo.Oracle(mode=0).measure(ref, deg, 48000, want_spec=True)
+    say("gpu mos", res["moslqo"], "oracle mos", ores["moslqo"], "lag", tr["lag"])
+    assert np.array_equal(tr["ref_spec"], otr["ref_spec"])
+    assert np.array_equal(tr["deg_spec"], otr["deg_spec"])
+    assert np.array_equal(tr["sims"], otr["sims"])
+    _compare(res, tr, ores, otr)
+    assert tr["lag"] == -37
+
+
+@pytest.mark.parametrize("lattice", [True, False])
+def test_speech_exact(native, vo, synth, lattice):
+    say(f"speech pair exact lattice={lattice}")
+    ref, deg = synth.speech_pair(3)
+    ctx = native.Context(native.MODE_SPEECH, use_lattice_model=lattice, precision="exact")
+    res, tr = ctx.measure_trace(ref, deg, 16000, want_stages=True)
+    ores, otr = vo.Oracle(mode=1, use_lattice_model=lattice).measure(ref, deg, 16000, want_spec=True)
+    say("gpu mos", res["moslqo"], "oracle mos", ores["moslqo"])
+    assert np.array_equal(tr["ref_spec"], otr["ref_spec"])
+    assert np.array_equal(tr["deg_spec"], otr["deg_spec"])
+    _compare(res, tr, ores, otr)
+
+
+def test_options_exact(native, vo, synth):
+    say("options: no global alignment / no realignment / radius")
+    ref, deg = synth.audio_pair(4, duration=6.0, delay=500, snr_db=15.0)
+    for kw in (dict(disable_global_alignment=True), dict(disable_realignment=True), dict(search_window_radius=5)):
+        res, tr = native.Context(native.MODE_AUDIO, precision="exact", **kw).measure_trace(ref, deg, 48000)
+        ores, otr = vo.Oracle(mode=0, **kw).measure(ref, deg, 48000)
+        say(kw, res["moslqo"], ores["moslqo"])
+        _compare(res, tr, ores, otr)
+
+
+def test_batch_equals_single_and_isolates_failures(native, vo, synth):
+    say("batch == single; a too-short pair fails alone")
+    pairs = [synth.audio_pair(s, duration=d) for s, d in ((10, 5.0), (11, 7.5), (12, 6.0))]
+    short = (np.zeros(1000), np.zeros(1000))
+    refs = [p[0] for p in pairs] + [short[0]]
+    degs = [p[1] for p in pairs] + [short[1]]
+    ctx = native.Context(native.MODE_AUDIO, precision="exact")
+    rc, results, status = ctx.measure_batch(refs, degs, 48000)
+    assert rc == native.E_SIGNAL and status == [0, 0, 0, native.E_SIGNAL] and results[3] is None
+    for (r, d), got in zip(pairs, results[:3]):
+        alone = ctx.measure(r, d, 48000)
+        assert got["moslqo"] == alone["moslqo"]
+        assert np.array_equal(got["fvnsim"], alone["fvnsim"])
+        ores, _ = vo.Oracle(mode=0).measure(r, d, 48000)
+        assert got["moslqo"] == ores["moslqo"]
+
+
+def test_errors(native):
+    say("error paths")
+    ctx = native.Context(native.MODE_AUDIO)
+    with pytest.raises(native.NativeError) as e:
+        ctx.measure(np.zeros(100), np.zeros(100), 48000)
+    assert e.value.code == native.E_SIGNAL
+    with pytest.raises(native.NativeError) as e:
+        ctx.measure(np.zeros(100000), np.zeros(100000), 0)
+    assert e.value.code == native.E_INVALID_ARGUMENT
+
+
+def test_fast_mode_close(native, vo, synth):
+    say("fast mode |dMOS| <= 1e-6")
+    ref, deg = synth.audio_pair(0)
+    res, tr = native.Context(native.MODE_AUDIO, precision="fast").measure_trace(ref, deg, 48000)
+    ores, otr = vo.Oracle(mode=0).measure(ref, deg, 48000)
+    say("fast mos", res["moslqo"], "oracle", ores["moslqo"], "diff", res["moslqo"] - ores["moslqo"])
+    _compare(res, tr, ores, otr, exact=False, tol=FAST_MOS_TOL)
