@@ -51,7 +51,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (pairs would shard "
-              "across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+              "across GPUs with no exchange between them)", file=sys.stderr)
         return 2
 
     from akvisqol_amd import native
