@@ -540,6 +540,20 @@ int visqol_hip_best_lag(const double* a, size_t la, const double* b, size_t lb, 
     return VISQOL_OK;
 }
 
+int visqol_hip_fft(double* z, size_t n, int dit) {
+    if (!z || n == 0 || (n & (n - 1))) return invalid("n must be a power of two");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    try {
+        stage_engine(0, false)->fft_probe(z, (long long)n, dit != 0);
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return VISQOL_OK;
+}
+
 int visqol_hip_math_probe(int op, const double* a, const double* b, double* out, size_t n) {
     if (!a || !out || op < 0 || op > 2 || (op == 2 && !b)) return invalid("invalid argument");
     std::string why;

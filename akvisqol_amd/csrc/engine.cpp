@@ -395,37 +395,77 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     std::vector<DpJob> dj;
     int max_nf = 0, max_width = 0, max_np = 0, max_win = 0;
     long long max_len = 0;
+    // Chunk-wide contiguous blocks: one allocation, upload, memset and
+    // readback each instead of one per pair.
+    size_t spec_total = 0, cells_total = 0, np_total = 0, plane_total = 0, degs_total = 0;
+    std::vector<size_t> spec_off(n), cell_off(n), np_off(n), plane_off(n), degs_off(n);
     for (int i = 0; i < n; i++) {
         PairState& s = st[i];
         if (!s.live) continue;
-        s.degs = (double*)arena_.alloc(sizeof(double) * s.dl2);
+        spec_off[i] = spec_total;
+        spec_total += (size_t)nb * (s.nf_ref + s.nf_deg);
+        cell_off[i] = cells_total;
+        cells_total += (size_t)s.np * s.nf_deg;
+        np_off[i] = np_total;
+        np_total += (size_t)s.np;
+        plane_off[i] = plane_total;
+        plane_total += (size_t)5 * nb * s.width;
+        degs_off[i] = degs_total;
+        degs_total += (size_t)s.dl2;
+    }
+    double* d_spec = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, spec_total));
+    double* d_raw = nullptr;
+    if (!opts_.no_realign || want_stages) d_raw = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, spec_total));
+    double* d_degs = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, degs_total));
+    double* d_planes = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, plane_total));
+    double* d_sims = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, cells_total));
+    double* d_dp = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, cells_total));
+    int* d_bt = (int*)arena_.alloc(sizeof(int) * std::max<size_t>(1, cells_total));
+    int* d_offs = (int*)arena_.alloc(sizeof(int) * std::max<size_t>(1, np_total));
+    int* d_states = (int*)arena_.alloc(sizeof(int) * std::max<size_t>(1, np_total));
+    std::vector<int> h_ilh(3 * np_total);  // idx | lo | hi
+    for (int i = 0; i < n; i++) {
+        PairState& s = st[i];
+        if (!s.live) continue;
+        for (int p = 0; p < s.np; p++) {
+            h_ilh[np_off[i] + p] = s.idx[p];
+            h_ilh[np_total + np_off[i] + p] = s.lo[p];
+            h_ilh[2 * np_total + np_off[i] + p] = s.hi[p];
+        }
+    }
+    int* d_ilh = upload(h_ilh);
+    if (cells_total) {
+        check(hipMemsetAsync(d_sims, 0, sizeof(double) * cells_total, stream_), "memset");
+        check(hipMemsetAsync(d_dp, 0, sizeof(double) * cells_total, stream_), "memset");
+        check(hipMemsetAsync(d_bt, 0, sizeof(int) * cells_total, stream_), "memset");
+    }
+    for (int i = 0; i < n; i++) {
+        PairState& s = st[i];
+        if (!s.live) continue;
+        s.degs = d_degs + degs_off[i];
         scj.push_back(ScaleJob{s.ddeg, s.degs, s.dl, s.dl2, s.shift, s.factor});
         max_len = std::max(max_len, s.dl2);
-        s.spec_ref = (double*)arena_.alloc(sizeof(double) * nb * s.nf_ref);
-        s.spec_deg = (double*)arena_.alloc(sizeof(double) * nb * s.nf_deg);
+        s.spec_ref = d_spec + spec_off[i];
+        s.spec_deg = s.spec_ref + (size_t)nb * s.nf_ref;
         gj.push_back(GtJob{s.dref, s.spec_ref, s.nf_ref, 0});
         gj.push_back(GtJob{s.degs, s.spec_deg, s.nf_deg, 0});
         max_nf = std::max(max_nf, std::max(s.nf_ref, s.nf_deg));
-        if (!opts_.no_realign || want_stages) {
-            s.raw_ref = (double*)arena_.alloc(sizeof(double) * nb * s.nf_ref);
-            s.raw_deg = (double*)arena_.alloc(sizeof(double) * nb * s.nf_deg);
+        if (d_raw) {
+            s.raw_ref = d_raw + spec_off[i];
+            s.raw_deg = s.raw_ref + (size_t)nb * s.nf_ref;
         }
         pj.push_back(PrepJob{s.spec_ref, s.spec_deg, s.nf_ref, s.nf_deg});
-        s.plane = (double*)arena_.alloc(sizeof(double) * 5 * nb * s.width);
+        s.plane = d_planes + plane_off[i];
         plj.push_back(PlaneJob{s.spec_deg, s.plane, s.nf_deg, s.width});
         max_width = std::max(max_width, s.width);
-        const size_t cells = (size_t)s.np * s.nf_deg;
-        s.sims = (double*)arena_.alloc(sizeof(double) * cells);
-        s.dp = (double*)arena_.alloc(sizeof(double) * cells);
-        s.bt = (int*)arena_.alloc(sizeof(int) * cells);
-        check(hipMemsetAsync(s.sims, 0, sizeof(double) * cells, stream_), "memset");
-        check(hipMemsetAsync(s.dp, 0, sizeof(double) * cells, stream_), "memset");
-        check(hipMemsetAsync(s.bt, 0, sizeof(int) * cells, stream_), "memset");
-        s.d_idx = upload(s.idx.data(), s.np);
-        s.d_lo = upload(s.lo);
-        s.d_hi = upload(s.hi);
-        s.d_off = (int*)arena_.alloc(sizeof(int) * s.np);
-        s.d_state = (int*)arena_.alloc(sizeof(int) * s.np);
+        s.sims = d_sims + cell_off[i];
+        s.dp = d_dp + cell_off[i];
+        s.bt = d_bt + cell_off[i];
+        s.d_idx = d_ilh + np_off[i];
+        s.d_lo = d_ilh + np_total + np_off[i];
+        s.d_hi = d_ilh + 2 * np_total + np_off[i];
+        s.d_off = d_offs + np_off[i];
+        s.d_state = d_states + np_off[i];
         sj.push_back(SimJob{s.spec_ref, s.plane, s.sims, s.d_idx, s.d_lo, s.d_hi, s.np, s.nf_ref, s.nf_deg, s.width});
         dj.push_back(DpJob{s.sims, s.dp, s.bt, s.d_lo, s.d_hi, s.d_off, s.d_state, s.np, s.nf_deg});
         max_np = std::max(max_np, s.np);
@@ -452,12 +492,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             }
             prof_.launches++;
         }
-        for (int i = 0; i < n; i++) {
-            PairState& s = st[i];
-            if (!s.live || !s.raw_ref) continue;
-            check(hipMemcpyAsync(s.raw_ref, s.spec_ref, sizeof(double) * nb * s.nf_ref, hipMemcpyDeviceToDevice, stream_), "copy");
-            check(hipMemcpyAsync(s.raw_deg, s.spec_deg, sizeof(double) * nb * s.nf_deg, hipMemcpyDeviceToDevice, stream_), "copy");
-        }
+        if (d_raw && spec_total)
+            check(hipMemcpyAsync(d_raw, d_spec, sizeof(double) * spec_total, hipMemcpyDeviceToDevice, stream_), "copy");
         auto* prep_scratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * pj.size());
         launch_prep(upload(pj), (int)pj.size(), nb, max_nf, prep_scratch, stream_);
         launch_planes(upload(plj), (int)plj.size(), nb, max_width, stream_);
@@ -468,15 +504,14 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     // Readback of the DP choice (and stage outputs on request).
     std::vector<std::vector<int>> h_off(n), h_state(n);
     {
-        std::vector<std::pair<int*, int*>> pins(n, {nullptr, nullptr});
         std::vector<double*> pr(n, nullptr), pd(n, nullptr), pss(n, nullptr);
+        int* p_offs = (int*)pinned_.alloc(sizeof(int) * (np_total + 1));
+        int* p_states = (int*)pinned_.alloc(sizeof(int) * (np_total + 1));
+        readback(p_offs, d_offs, sizeof(int) * np_total);
+        readback(p_states, d_states, sizeof(int) * np_total);
         for (int i = 0; i < n; i++) {
             PairState& s = st[i];
             if (!s.live) continue;
-            pins[i].first = (int*)pinned_.alloc(sizeof(int) * s.np);
-            pins[i].second = (int*)pinned_.alloc(sizeof(int) * s.np);
-            readback(pins[i].first, s.d_off, sizeof(int) * s.np);
-            readback(pins[i].second, s.d_state, sizeof(int) * s.np);
             if (want_stages) {
                 pr[i] = (double*)pinned_.alloc(sizeof(double) * nb * s.nf_ref);
                 pd[i] = (double*)pinned_.alloc(sizeof(double) * nb * s.nf_deg);
@@ -493,9 +528,9 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             prof_.gammatone_ms += ms;
         }
         for (int i = 0; i < n; i++) {
-            if (!pins[i].first) continue;
-            h_off[i].assign(pins[i].first, pins[i].first + st[i].np);
-            h_state[i].assign(pins[i].second, pins[i].second + st[i].np);
+            if (!st[i].live) continue;
+            h_off[i].assign(p_offs + np_off[i], p_offs + np_off[i] + st[i].np);
+            h_state[i].assign(p_states + np_off[i], p_states + np_off[i] + st[i].np);
             if (want_stages) {
                 out[i].ref_spec.assign(pr[i], pr[i] + (size_t)nb * st[i].nf_ref);
                 out[i].deg_spec.assign(pd[i], pd[i] + (size_t)nb * st[i].nf_deg);
@@ -646,12 +681,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             std::vector<StatJob> rstj;
             int rmax_nf = 0;
             long long max_t = 0;
-            struct Copy {
-                double* dst;
-                const double* src;
-                int src_stride, nf;
-            };
-            std::vector<Copy> copies;
+            std::vector<FrameCopyJob> copies;
             for (size_t k = 0; k < rjs.size(); k++) {
                 RJ& rj = rjs[k];
                 PairState& s = st[rj.pair];
@@ -674,12 +704,12 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                 };
                 long long g0 = 0;
                 if (reuse(rj.rl, rj.geo.ref_off, s.nf_ref, g0)) {
-                    copies.push_back(Copy{rj.spec_r, s.raw_ref + g0, s.nf_ref, rj.nf});
+                    copies.push_back(FrameCopyJob{rj.spec_r, s.raw_ref + g0, s.nf_ref, rj.nf});
                 } else {
                     rg.push_back(GtJob{rj.sref + rj.geo.ref_off, rj.spec_r, rj.nf, 0});
                 }
                 if (reuse(rj.dl, rj.geo.deg_off, s.nf_deg, g0)) {
-                    copies.push_back(Copy{rj.spec_d, s.raw_deg + g0, s.nf_deg, rj.nf});
+                    copies.push_back(FrameCopyJob{rj.spec_d, s.raw_deg + g0, s.nf_deg, rj.nf});
                 } else {
                     const long long off = rj.geo.deg_off;
                     if (off >= 0 && off + rj.geo.len <= rj.dlen) {
@@ -698,10 +728,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             }
             launch_slices(upload(tslj), (int)tslj.size(), max_t, stream_);
             launch_gammatone(upload(rg), (int)rg.size(), rmax_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, stream_);
-            for (auto& c : copies)
-                check(hipMemcpy2DAsync(c.dst, sizeof(double) * c.nf, c.src, sizeof(double) * c.src_stride,
-                                       sizeof(double) * c.nf, nb, hipMemcpyDeviceToDevice, stream_),
-                      "copy frames");
+            launch_frame_copies(upload(copies), (int)copies.size(), nb, rmax_nf, stream_);
             auto* rscratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * std::max<size_t>(1, rpj.size()));
             launch_prep(upload(rpj), (int)rpj.size(), nb, rmax_nf, rscratch, stream_);
             double* d_rstats = (double*)arena_.alloc(sizeof(double) * stat_len * std::max<size_t>(1, rjs.size()));
@@ -859,6 +886,22 @@ long long Engine::best_lag(const double* a, long long la, const double* b, long 
     readback(p, d_lag, sizeof(long long));
     sync();
     return *p;
+}
+
+void Engine::fft_probe(double* z, long long n, bool dit) {
+    arena_.reset();
+    pinned_.reset();
+    if (n < 2) return;
+    double2* d = (double2*)arena_.alloc(sizeof(double2) * n);
+    check(hipMemcpyAsync(d, z, sizeof(double2) * n, hipMemcpyHostToDevice, stream_), "upload");
+    std::vector<double2*> rows{d};
+    double2** d_rows = upload(rows);
+    if (dit) launch_fft_dit(d_rows, 1, n, twiddles(n), stream_);
+    else launch_fft_dif(d_rows, 1, n, twiddles(n), stream_);
+    double* p = (double*)pinned_.alloc(sizeof(double2) * n);
+    readback(p, d, sizeof(double2) * n);
+    sync();
+    memcpy(z, p, sizeof(double2) * n);
 }
 
 void Engine::math_probe(int op, const double* a, const double* b, double* out, long long n) {

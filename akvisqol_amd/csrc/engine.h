@@ -108,6 +108,7 @@ class Engine {
     void spectrogram(uint32_t sample_rate, const double* host_sig, long long len, std::vector<double>& out, int& nf);
     long long best_lag(const double* a, long long la, const double* b, long long lb);
     void math_probe(int op, const double* a, const double* b, double* out, long long n);
+    void fft_probe(double* z, long long n, bool dit);
     const RateParams& rate(uint32_t sample_rate);
 
     // Device time of the full-signal gammatone launches, measured with HIP

@@ -95,6 +95,15 @@ struct VadJob {
 };
 void launch_vad(const VadJob* d_jobs, int njobs, int max_chunks, unsigned long long* d_scratch, hipStream_t stream);
 
+// Spectrogram frames on the full-signal grid reused by a realigned patch
+// (realign.zig:262-281): dst[b * nf + f] = src[b * src_stride + f].
+struct FrameCopyJob {
+    double* dst;
+    const double* src;
+    int src_stride, nf;
+};
+void launch_frame_copies(const FrameCopyJob* d_jobs, int njobs, int nb, int max_nf, hipStream_t stream);
+
 // ---------------------------------------------------------------- spectrogram prep
 struct PrepJob {
     double* ref;

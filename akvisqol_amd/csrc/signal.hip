@@ -23,8 +23,13 @@ __device__ inline double sum_term(const SumJob& j, long long i, double mean) {
     }
 }
 
-// One block per job.  serial: lane 0 walks the signal in order (the
-// reference's rounding sequence).  tree: fixed-shape block reduction.
+constexpr int kSerialTile = 2048;  // terms per LDS tile (16 KiB)
+
+// One block per job.  serial: the terms are added in sample order by one
+// lane (the reference's rounding sequence).  Each term (x, x·x or
+// ±(x - mean)) rounds on its own, so all lanes compute terms for the next
+// tile into a second LDS buffer while lane 0 runs the add chain over the
+// current one.  tree: fixed-shape block reduction.
 __global__ __launch_bounds__(256) void k_sums(const SumJob* __restrict__ jobs, double* __restrict__ out, int serial) {
     const SumJob j = jobs[blockIdx.x];
     long long start = 0;
@@ -36,17 +41,42 @@ __global__ __launch_bounds__(256) void k_sums(const SumJob* __restrict__ jobs, d
     double mean = 0.0;
     if (j.kind == kNyquist) mean = out[j.aux] / (double)j.len;
     if (serial) {
-        if (threadIdx.x != 0) return;
+        __shared__ double tile[2][kSerialTile];
+        constexpr int per = kSerialTile / 256;
+        const long long total = j.len - start;
+        const long long ntiles = (total + kSerialTile - 1) / kSerialTile;
         double s = 0.0;
-        long long i = start;
-        if (j.kind == kNyquist) {
-            for (; i < j.len; i++) s += sum_term(j, i, mean);
-        } else if (j.kind == kSum) {
-            for (; i < j.len; i++) s += j.x[i];
-        } else {
-            for (; i < j.len; i++) s += j.x[i] * j.x[i];
+        double v[per];
+        // prologue: tile 0
+        for (int q = 0; q < per; q++) {
+            const long long i = start + (long long)q * 256 + threadIdx.x;
+            v[q] = i < j.len ? sum_term(j, i, mean) : 0.0;
         }
-        out[j.slot] = s;
+        for (long long t = 0; t < ntiles; t++) {
+            const int cur = (int)(t & 1);
+            for (int q = 0; q < per; q++) tile[cur][q * 256 + threadIdx.x] = v[q];
+            __syncthreads();
+            // issue the next tile's loads before the serial chain
+            const long long base = start + (t + 1) * kSerialTile;
+            for (int q = 0; q < per; q++) {
+                const long long i = base + (long long)q * 256 + threadIdx.x;
+                v[q] = i < j.len ? sum_term(j, i, mean) : 0.0;
+            }
+            if (threadIdx.x == 0) {
+                const long long rem = total - t * kSerialTile;
+                const int cnt = rem < kSerialTile ? (int)rem : kSerialTile;
+                const double* b = tile[cur];
+                int k = 0;
+                for (; k + 8 <= cnt; k += 8) {
+                    const double a0 = b[k], a1 = b[k + 1], a2 = b[k + 2], a3 = b[k + 3];
+                    const double a4 = b[k + 4], a5 = b[k + 5], a6 = b[k + 6], a7 = b[k + 7];
+                    s += a0; s += a1; s += a2; s += a3; s += a4; s += a5; s += a6; s += a7;
+                }
+                for (; k < cnt; k++) s += b[k];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[j.slot] = s;
         return;
     }
     __shared__ double part[256];
@@ -176,22 +206,114 @@ __global__ __launch_bounds__(256) void k_fft_block(double2* const* __restrict__ 
     for (int i = threadIdx.x; i < B; i += blockDim.x) z[i] = s[i];
 }
 
+// k consecutive radix-2 stages in one HBM round trip.  The stages with spans
+// S_hi, S_hi/2, ..., S_hi/2^(k-1) only mix elements i = blk·S_hi + r + m·q
+// (q = S_hi/2^k, m < 2^k) of one residue r, so a workgroup loads R
+// consecutive residues (R·16 B contiguous per m) of one span block into LDS,
+// runs the k stages on them and writes them back.  Every butterfly is the
+// radix-2 butterfly of the stage it belongs to, with the same twiddle, so
+// the result equals the stage-by-stage transform bit for bit.
+// A workgroup holds Bn consecutive span blocks × R residues × 2^k elements
+// (Bn > 1 only when a block has fewer than R residues, so short spans still
+// fill a workgroup with contiguous data).
+template <bool DIT>
+__global__ __launch_bounds__(256) void k_fft_pass(double2* const* __restrict__ rows, long long n, long long S_hi, int k,
+                                                  int R, int Bn, const double2* __restrict__ tw) {
+    extern __shared__ double2 sm[];
+    const int G = 1 << k;
+    const int GR = G * R;
+    const long long q = S_hi >> k;
+    const long long per_blk = q / R;
+    const long long wg = blockIdx.x;
+    const long long blk0 = (wg / per_blk) * Bn;
+    const long long r0 = (wg - (wg / per_blk) * per_blk) * R;
+    double2* z = rows[blockIdx.y] + blk0 * S_hi + r0;
+    for (int e = threadIdx.x; e < GR * Bn; e += blockDim.x) {
+        const int bb = e / GR, rem = e - bb * GR;
+        const int m = rem / R, rr = rem - m * R;
+        sm[e] = z[(long long)bb * S_hi + (long long)m * q + rr];
+    }
+    __syncthreads();
+    for (int li = 0; li < k; li++) {
+        const int l = DIT ? k - 1 - li : li;
+        const int hm = G >> (l + 1);
+        const long long stride = n / (S_hi >> l);
+        const int half = GR / 2;
+        for (int t = threadIdx.x; t < half * Bn; t += blockDim.x) {
+            const int bb = t / half, rem = t - bb * half;
+            const int rr = rem % R, bm = rem / R;
+            const int m0 = (bm / hm) * 2 * hm + (bm % hm);
+            const int i0 = bb * GR + m0 * R + rr, i1 = i0 + hm * R;
+            const long long j = (long long)(bm % hm) * q + r0 + rr;
+            const double2 w = tw[j * stride];
+            const double2 a = sm[i0], b = sm[i1];
+            if (DIT) {
+                const double2 bw = cmul_conj(b, w);
+                sm[i0] = cadd(a, bw);
+                sm[i1] = csub(a, bw);
+            } else {
+                sm[i0] = cadd(a, b);
+                sm[i1] = cmul(csub(a, b), w);
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < GR * Bn; e += blockDim.x) {
+        const int bb = e / GR, rem = e - bb * GR;
+        const int m = rem / R, rr = rem - m * R;
+        z[(long long)bb * S_hi + (long long)m * q + rr] = sm[e];
+    }
+}
+
+namespace {
+constexpr int kMaxFuse = 8;      // stages per pass: 256-element groups
+constexpr int kMaxGroupR = 4096;  // G·R complex values per workgroup (64 KiB LDS)
+
+// Stage counts per pass, as even as possible.
+void plan_passes(int L, int* ks, int& np) {
+    np = (L + kMaxFuse - 1) / kMaxFuse;
+    for (int p = 0; p < np; p++) ks[p] = L / np + (p < L % np ? 1 : 0);
+}
+
+template <bool DIT>
+void launch_pass(double2* const* d_rows, int nrows, long long n, long long S_hi, int k, const double2* d_tw,
+                 hipStream_t stream) {
+    const long long G = 1LL << k, q = S_hi >> k;
+    const long long want = kMaxGroupR / G;  // residues (times blocks) per workgroup
+    long long R = want < q ? want : q;
+    long long Bn = want / R;
+    if (Bn > n / S_hi) Bn = n / S_hi;
+    if (Bn < 1) Bn = 1;
+    const long long wgs = (n / S_hi / Bn) * (q / R);
+    hipLaunchKernelGGL(k_fft_pass<DIT>, dim3((unsigned)wgs, nrows), dim3(256), sizeof(double2) * G * R * Bn, stream, d_rows,
+                       n, S_hi, k, (int)R, (int)Bn, d_tw);
+}
+}  // namespace
+
 void launch_fft_dif(double2* const* d_rows, int nrows, long long n, const double2* d_tw, hipStream_t stream) {
     if (nrows == 0 || n < 2) return;
-    const int B = (int)(n < kFftBlock ? n : kFftBlock);
-    const unsigned gblocks = (unsigned)((n / 2 + 255) / 256);
-    for (long long S = n; S > B; S >>= 1)
-        hipLaunchKernelGGL(k_fft_stage<false>, dim3(gblocks, nrows), dim3(256), 0, stream, d_rows, n, S, d_tw);
-    hipLaunchKernelGGL(k_fft_block<false>, dim3((unsigned)(n / B), nrows), dim3(256), 0, stream, d_rows, n, B, d_tw);
+    int L = 0;
+    while ((1LL << L) < n) L++;
+    int ks[8], np;
+    plan_passes(L, ks, np);
+    long long S = n;
+    for (int p = 0; p < np; p++) {
+        launch_pass<false>(d_rows, nrows, n, S, ks[p], d_tw, stream);
+        S >>= ks[p];
+    }
 }
 
 void launch_fft_dit(double2* const* d_rows, int nrows, long long n, const double2* d_tw, hipStream_t stream) {
     if (nrows == 0 || n < 2) return;
-    const int B = (int)(n < kFftBlock ? n : kFftBlock);
-    const unsigned gblocks = (unsigned)((n / 2 + 255) / 256);
-    hipLaunchKernelGGL(k_fft_block<true>, dim3((unsigned)(n / B), nrows), dim3(256), 0, stream, d_rows, n, B, d_tw);
-    for (long long S = 2LL * B; S <= n; S <<= 1)
-        hipLaunchKernelGGL(k_fft_stage<true>, dim3(gblocks, nrows), dim3(256), 0, stream, d_rows, n, S, d_tw);
+    int L = 0;
+    while ((1LL << L) < n) L++;
+    int ks[8], np;
+    plan_passes(L, ks, np);
+    long long S = 1;
+    for (int p = np - 1; p >= 0; p--) {
+        S <<= ks[p];
+        launch_pass<true>(d_rows, nrows, n, S, ks[p], d_tw, stream);
+    }
 }
 
 // xcorr.zig:182-192: with X = Z[k], Y = conj(Z[-k]): (X+Y)·i·conj(X-Y)/4.
@@ -336,6 +458,19 @@ __global__ __launch_bounds__(256) void k_slices(const SliceJob* __restrict__ job
 void launch_slices(const SliceJob* d_jobs, int njobs, long long max_len, hipStream_t stream) {
     if (njobs == 0 || max_len == 0) return;
     hipLaunchKernelGGL(k_slices, dim3((unsigned)((max_len + 255) / 256), njobs), dim3(256), 0, stream, d_jobs);
+}
+
+__global__ __launch_bounds__(256) void k_frame_copies(const FrameCopyJob* __restrict__ jobs, int nb) {
+    const FrameCopyJob j = jobs[blockIdx.y];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * j.nf) return;
+    const int b = e / j.nf, f = e % j.nf;
+    j.dst[e] = j.src[(long long)b * j.src_stride + f];
+}
+
+void launch_frame_copies(const FrameCopyJob* d_jobs, int njobs, int nb, int max_nf, hipStream_t stream) {
+    if (njobs == 0 || max_nf == 0) return;
+    hipLaunchKernelGGL(k_frame_copies, dim3((nb * max_nf + 255) / 256, njobs), dim3(256), 0, stream, d_jobs, nb);
 }
 
 // ---------------------------------------------------------------- VAD

@@ -129,6 +129,7 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_filters": (c_int, [c_uint32, c_uint32, D, D]),
         "visqol_hip_best_lag": (c_int, [D, c_size_t, D, c_size_t, POINTER(c_int64)]),
         "visqol_hip_math_probe": (c_int, [c_int, D, D, D, c_size_t]),
+        "visqol_hip_fft": (c_int, [D, c_size_t, c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -386,6 +387,14 @@ def best_lag(a, b) -> int:
     lag = c_int64()
     _check(lib.visqol_hip_best_lag(_dptr(a), a.size, _dptr(b), b.size, ctypes.byref(lag)))
     return lag.value
+
+
+def fft(z: np.ndarray, dit: bool = False) -> np.ndarray:
+    """Device DIF forward (bit-reversed out) or DIT inverse of complex z, unscaled."""
+    z = np.asarray(z, dtype=np.complex128)
+    buf = np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).ravel())
+    _check(lib.visqol_hip_fft(_dptr(buf), z.size, int(dit)))
+    return buf[0::2] + 1j * buf[1::2]
 
 
 def math_probe(op: int, a, b=None) -> np.ndarray:

@@ -72,6 +72,11 @@ int visqol_hip_spectrogram(uint32_t mode, uint32_t sample_rate, uint32_t precisi
 int visqol_hip_filters(uint32_t mode, uint32_t sample_rate, double *bands8, double *center_freqs);
 /* FFT cross-correlation lag of the upper envelopes of a and b. */
 int visqol_hip_best_lag(const double *a, size_t la, const double *b, size_t lb, int64_t *lag);
+/* The device radix-2 transform in place on n interleaved complex values
+ * (n a power of two): dit = 0 runs the DIF forward transform (natural in,
+ * bit-reversed out), dit = 1 the DIT inverse (bit-reversed in, natural out),
+ * both unscaled. */
+int visqol_hip_fft(double *z, size_t n, int dit);
 /* Device arithmetic probes: op 0 log10(a), 1 sqrt(a), 2 a / b. */
 int visqol_hip_math_probe(int op, const double *a, const double *b, double *out, size_t n);
 

@@ -171,8 +171,8 @@ void vo_similarity_table(const double *ref_spec, size_t nf_ref, const double *de
                          double *sims) {
     size_t width = nf_deg;
     for (size_t p = 0; p < np; p++) if (hi[p] + patch > width) width = hi[p] + patch;
-    deg_plane pl;
-    plane_init(&pl, deg_spec, nb, nf_deg, width);
+    deg_plane pl = {0};
+    if (plane_init(&pl, deg_spec, nb, nf_deg, width)) return;
     ref_cache s;
     ref_cache_init(&s, nb, patch);
     double *ref_patch = (double *)malloc(sizeof(double) * nb * patch);

@@ -759,7 +759,7 @@ static int exec_op(const struct vo_lattice *m, scratch *sc, const op_t *op, val 
             break;
         }
         case K_FC: {
-            val w, bias;
+            val w, bias = {0};
             int has_bias = 0;
             if (get_in(vals, op, 0, &x) || get_in(vals, op, 1, &w)) return -1;
             if (op->nin > 2 && op->in[2] >= 0) { if (get_in(vals, op, 2, &bias)) return -1; has_bias = 1; }

@@ -83,6 +83,20 @@ def test_spectrogram_fast_close(native, vo):
     assert rel < 1e-9
 
 
+@pytest.mark.parametrize("n", [2, 64, 4096, 1 << 13, 1 << 16, 1 << 18, 1 << 20])
+def test_fft_bit_identical(native, vo, n):
+    say(f"fft DIF/DIT n={n}")
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    got = native.fft(x, dit=False)
+    ref = vo.f64(np.stack([x.real, x.imag], 1).ravel())
+    vo.lib().vo_fft_dif_forward(vo.dptr(ref), n)
+    assert np.array_equal(got, ref[0::2] + 1j * ref[1::2])
+    back = native.fft(got, dit=True)
+    vo.lib().vo_fft_dit_inverse(vo.dptr(ref), n)
+    assert np.array_equal(back, ref[0::2] + 1j * ref[1::2])
+
+
 REF14 = [2.0, 2.0, 1.0, 0.1, -3.0, 0.1, 1.0, 2.0, 2.0, 6.0, 8.0, 6.0, 2.0, 2.0]
 
 
