@@ -21,7 +21,7 @@ LIB = HERE / "libvisqol.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["gammatone.hip", "signal.hip", "nsim.hip", "engine.cpp", "host_plan.cpp", "models.cpp", "wav.cpp",
+SOURCES = ["gammatone.hip", "signal.hip", "xcorr.hip", "nsim.hip", "engine.cpp", "host_plan.cpp", "models.cpp", "wav.cpp",
            "capi.cpp"]
 
 FLAGS = [

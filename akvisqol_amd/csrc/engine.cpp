@@ -173,24 +173,36 @@ const double2* Engine::twiddles(long long n) {
     return d;
 }
 
-// Envelope packing, FFT rows grouped by size, cross terms, inverse, lag scan.
-void Engine::lag_search(std::vector<LagJob>& jobs, LagJob* d_jobs, long long* d_lags) {
-    if (jobs.empty()) return;
-    long long max_n = 0;
-    std::map<long long, std::vector<double2*>> groups;
-    for (auto& j : jobs) {
-        max_n = std::max(max_n, j.n);
-        groups[j.n].push_back(j.z);
+// Lag searches grouped by transform size; pos[i] is job i's slot in the
+// returned device array of lags.
+long long* Engine::lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos) {
+    pos.assign(jobs.size(), 0);
+    long long* d_lags = (long long*)arena_.alloc(sizeof(long long) * std::max<size_t>(1, jobs.size()));
+    if (jobs.empty()) return d_lags;
+    std::vector<int> idx(jobs.size());
+    for (size_t i = 0; i < idx.size(); i++) idx[i] = (int)i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return jobs[a].n < jobs[b].n; });
+    std::vector<LagJob> sorted;
+    sorted.reserve(jobs.size());
+    for (size_t k = 0; k < idx.size(); k++) {
+        pos[idx[k]] = (int)k;
+        sorted.push_back(jobs[idx[k]]);
     }
-    for (auto& g : groups) {
-        const double2* tw = twiddles(g.first);
-        double2** rows = upload(g.second);
-        launch_fft_dif(rows, (int)g.second.size(), g.first, tw, stream_);
-        launch_cross_terms(rows, (int)g.second.size(), g.first, stream_);
-        launch_fft_dit(rows, (int)g.second.size(), g.first, tw, stream_);
+    LagJob* d_jobs = upload(sorted);
+    size_t parts = 0;
+    for (const LagJob& j : sorted) parts += 2 * (size_t)lag_partials_per_job(j.n);
+    auto* d_parts = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * parts);
+    size_t start = 0, poff = 0;
+    while (start < sorted.size()) {
+        size_t end = start;
+        while (end < sorted.size() && sorted[end].n == sorted[start].n) end++;
+        const long long n = sorted[start].n;
+        launch_lag_search(d_jobs + start, (int)(end - start), n, twiddles(n), d_sums, d_parts + poff, d_lags + start,
+                          stream_);
+        poff += 2 * (size_t)lag_partials_per_job(n) * (end - start);
+        start = end;
     }
-    auto* scratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * 2 * jobs.size());
-    launch_lag_argmax(d_jobs, (int)jobs.size(), max_n, scratch, d_lags, stream_);
+    return d_lags;
 }
 
 namespace {
@@ -284,19 +296,14 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     }
     launch_sums(upload(j1), (int)j1.size(), d_sums, serial, stream_);
     launch_sums(upload(j2), (int)j2.size(), d_sums, serial, stream_);
-    long long* d_lags = (long long*)arena_.alloc(sizeof(long long) * (lj.size() + 1));
-    long long* d_zero = d_lags + lj.size();
+    long long* d_zero = (long long*)arena_.alloc(sizeof(long long));
     check(hipMemsetAsync(d_zero, 0, sizeof(long long), stream_), "memset");
-    if (!lj.empty()) {
-        LagJob* d_lj = upload(lj);
-        long long max_n = 0;
-        for (auto& j : lj) max_n = std::max(max_n, j.n);
-        launch_env_pack(d_lj, (int)lj.size(), max_n, d_sums, stream_);
-        lag_search(lj, d_lj, d_lags);
-    }
+    std::vector<int> lag_pos;
+    long long* d_lags = lag_search(lj, d_sums, lag_pos);
     for (int i = 0; i < n; i++) {
         PairState& s = st[i];
-        j3.push_back(SumJob{s.ddeg, s.dl, kSumSqFrom, 6 * i + 5, 0, 0, s.lagjob >= 0 ? d_lags + s.lagjob : d_zero, s.rl / 2});
+        const long long* lp = s.lagjob >= 0 ? d_lags + lag_pos[s.lagjob] : d_zero;
+        j3.push_back(SumJob{s.ddeg, s.dl, kSumSqFrom, 6 * i + 5, 0, 0, lp, s.rl / 2});
     }
     launch_sums(upload(j3), (int)j3.size(), d_sums, serial, stream_);
     std::vector<VadJob> vj;
@@ -342,7 +349,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     for (int i = 0; i < n; i++) {
         PairState& s = st[i];
         PairResult& r = out[i];
-        s.lag = s.lagjob >= 0 ? h_lags[s.lagjob] : 0;
+        s.lag = s.lagjob >= 0 ? h_lags[lag_pos[s.lagjob]] : 0;
         r.lag = s.lag;
         // realign.zig:91-110 (applyLag)
         s.dl2 = s.dl;
@@ -662,17 +669,13 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             }
             launch_sums(upload(a1), (int)a1.size(), d_rs, serial, stream_);
             launch_sums(upload(a2), (int)a2.size(), d_rs, serial, stream_);
-            LagJob* d_rlj = upload(rlj);
-            long long max_n = 0;
-            for (auto& j : rlj) max_n = std::max(max_n, j.n);
-            launch_env_pack(d_rlj, (int)rlj.size(), max_n, d_rs, stream_);
-            long long* d_rlags = (long long*)arena_.alloc(sizeof(long long) * rlj.size());
-            lag_search(rlj, d_rlj, d_rlags);
+            std::vector<int> rpos;
+            long long* d_rlags = lag_search(rlj, d_rs, rpos);
             std::vector<long long> h_rl(rlj.size());
             long long* pl = (long long*)pinned_.alloc(sizeof(long long) * rlj.size());
             readback(pl, d_rlags, sizeof(long long) * rlj.size());
             sync();
-            memcpy(h_rl.data(), pl, sizeof(long long) * rlj.size());
+            for (size_t q = 0; q < rlj.size(); q++) h_rl[q] = pl[rpos[q]];
 
             // Trim geometry, frame reuse, spectrograms of what is kept.
             std::vector<GtJob> rg;
@@ -878,10 +881,8 @@ long long Engine::best_lag(const double* a, long long la, const double* b, long 
     const long long nfft = fft_size_for(std::max(la, lb));
     double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
     std::vector<LagJob> lj{LagJob{da, db, la, lb, z, nfft, 0, 1, 2, 3}};
-    LagJob* d_lj = upload(lj);
-    launch_env_pack(d_lj, 1, nfft, sums, stream_);
-    long long* d_lag = (long long*)arena_.alloc(sizeof(long long));
-    lag_search(lj, d_lj, d_lag);
+    std::vector<int> pos;
+    long long* d_lag = lag_search(lj, sums, pos);
     long long* p = (long long*)pinned_.alloc(sizeof(long long));
     readback(p, d_lag, sizeof(long long));
     sync();
@@ -894,8 +895,8 @@ void Engine::fft_probe(double* z, long long n, bool dit) {
     if (n < 2) return;
     double2* d = (double2*)arena_.alloc(sizeof(double2) * n);
     check(hipMemcpyAsync(d, z, sizeof(double2) * n, hipMemcpyHostToDevice, stream_), "upload");
-    std::vector<double2*> rows{d};
-    double2** d_rows = upload(rows);
+    std::vector<LagJob> rows{LagJob{nullptr, nullptr, 0, 0, d, n, 0, 0, 0, 0}};
+    LagJob* d_rows = upload(rows);
     if (dit) launch_fft_dit(d_rows, 1, n, twiddles(n), stream_);
     else launch_fft_dif(d_rows, 1, n, twiddles(n), stream_);
     double* p = (double*)pinned_.alloc(sizeof(double2) * n);

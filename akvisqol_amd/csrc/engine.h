@@ -143,8 +143,9 @@ class Engine {
     void check(hipError_t e, const char* what);
     // One chunk of pairs (bounded HBM footprint).
     void run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages);
-    // FFT cross-correlation lags of jobs whose packed rows are set up.
-    void lag_search(std::vector<LagJob>& jobs, LagJob* d_jobs, long long* d_lags);
+    // FFT cross-correlation lags of `jobs` (envelope sums already in
+    // d_sums); pos[i] is job i's slot in the returned device lag array.
+    long long* lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos);
     void finish(PairResult& r, const RateParams& rp);
 
     EngineOptions opts_;

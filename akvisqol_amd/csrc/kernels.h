@@ -44,24 +44,26 @@ struct SumJob {
 void launch_sums(const SumJob* d_jobs, int njobs, double* d_out, bool serial, hipStream_t stream);
 
 // ---------------------------------------------------------------- envelope + FFT xcorr
-// One lag search: z[i] = (env(a)[i], env(b)[i]) zero padded to n (a power of two).
+// One lag search: z[i] = (env(a)[i], env(b)[i]) zero padded to n (a power
+// of two), z being the job's n-element work row.
 struct LagJob {
     const double* a;
     const double* b;
     long long la, lb;
     double2* z;
     long long n;
-    int sum_a, sum_b;  // indices of the plain sums in the sums output
-    int nyq_a, nyq_b;  // indices of the nyquist sums
+    int sum_a, sum_b;  // slots of the plain sums in the sums array
+    int nyq_a, nyq_b;  // slots of the alternating (Nyquist) sums
 };
-void launch_env_pack(const LagJob* d_jobs, int njobs, long long max_n, const double* d_sums, hipStream_t stream);
-// Radix-2 DIF forward / DIT inverse over rows of the same n; tw = W_n^k, k < n/2.
-void launch_fft_dif(double2* const* d_rows, int nrows, long long n, const double2* d_tw, hipStream_t stream);
-void launch_fft_dit(double2* const* d_rows, int nrows, long long n, const double2* d_tw, hipStream_t stream);
-void launch_cross_terms(double2* const* d_rows, int nrows, long long n, hipStream_t stream);
-// lag = first argmax of |Re z| in the order -max_lag..-1, 0..max_lag.
-void launch_lag_argmax(const LagJob* d_jobs, int njobs, long long max_n, unsigned long long* d_scratch,
-                       long long* d_lags, hipStream_t stream);
+// Lag search for njobs jobs of one size n (contiguous in d_jobs): lags[j] =
+// first argmax of |Re xcorr| in the order -max_lag..-1, 0..max_lag.
+// d_partials holds 2 * njobs * lag_partials_per_job(n) words.
+long long lag_partials_per_job(long long n);
+void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, const double* d_sums,
+                       unsigned long long* d_partials, long long* d_lags, hipStream_t stream);
+// Radix-2 DIF forward / DIT inverse of each job's z (tw = W_n^k, k < n/2).
+void launch_fft_dif(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, hipStream_t stream);
+void launch_fft_dit(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, hipStream_t stream);
 
 // ---------------------------------------------------------------- signal prep
 // dst[i] = src-aligned sample i times factor: negative lag drops -lag
