@@ -471,6 +471,14 @@ int visqol_hip_profile_read(visqol_ctx* c, double* ms, double* flop, double* byt
     return VISQOL_OK;
 }
 
+int visqol_hip_profile_phases(visqol_ctx* c, double* ms, int n) {
+    if (!c) return invalid("ctx is NULL");
+    if (n > 0 && !ms) return invalid("ms is NULL");
+    const Engine::Profile p = c->engine->profile();
+    for (int i = 0; i < n && i < Engine::Profile::kPhases; i++) ms[i] = p.phase_ms[i];
+    return Engine::Profile::kPhases;
+}
+
 int visqol_hip_alloc(size_t bytes, void** p) {
     if (!p) return invalid("NULL argument");
     if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {

@@ -13,6 +13,31 @@ namespace vq {
 
 constexpr int kMaxBands = 32;
 
+// A device pointer read from a job table is a generic (flat) pointer; its
+// loads count against both the vector-memory and the LDS wait counters, so a
+// wait for an LDS read also waits for them.  Every buffer here is global
+// memory: this view makes the loads global loads.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T* as_global(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T* as_global(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+// double2 element i of a global buffer, as one 16-byte global access.
+typedef double vq_d2 __attribute__((ext_vector_type(2)));
+__device__ inline double2 load2(const double2* p, long long i) {
+    const vq_d2 v = ((const __attribute__((address_space(1))) vq_d2*)p)[i];
+    return make_double2(v.x, v.y);
+}
+__device__ inline void store2(double2* p, long long i, double2 v) {
+    vq_d2 w;
+    w.x = v.x;
+    w.y = v.y;
+    ((__attribute__((address_space(1))) vq_d2*)p)[i] = w;
+}
+
 // 2 × 8 coefficients per band, precomputed on the host exactly as
 // gammatone.zig:268-278 derives them from Filters: s1n0, s1n1, s2n1, s3n1,
 // s4n1, ca0, d1, d2.

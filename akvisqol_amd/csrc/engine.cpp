@@ -173,6 +173,21 @@ const double2* Engine::twiddles(long long n) {
     return d;
 }
 
+void Engine::lap(int phase) {
+    const auto now = std::chrono::steady_clock::now();
+    if (profiling_ && phase >= 0)
+        prof_.phase_ms[phase] += std::chrono::duration<double, std::milli>(now - lap_t_).count();
+    lap_t_ = now;
+}
+
+void Engine::sums(const std::vector<SumJob>& jobs, double* d_out, bool serial) {
+    if (jobs.empty()) return;
+    double* parts = serial ? nullptr : (double*)arena_.alloc(sizeof(double) * kSumParts * jobs.size());
+    long long max_len = 0;
+    for (const SumJob& j : jobs) max_len = std::max(max_len, j.len);
+    launch_sums(upload(jobs), (int)jobs.size(), max_len, d_out, serial, parts, stream_);
+}
+
 // Lag searches grouped by transform size; pos[i] is job i's slot in the
 // returned device array of lags.
 long long* Engine::lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos) {
@@ -257,6 +272,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     const int nb = rp.nb, W = rp.window, hop = rp.hop, S = rp.patch;
     const bool serial = !opts_.fast;
     std::vector<PairState> st(n);
+    lap(-1);
 
     // ------------------------------------------------------------ phase A
     for (int i = 0; i < n; i++) {
@@ -294,8 +310,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         }
         j1.push_back(SumJob{s.dref, s.rl, kSumSq, 6 * i + 4, 0, 0, nullptr, 0});
     }
-    launch_sums(upload(j1), (int)j1.size(), d_sums, serial, stream_);
-    launch_sums(upload(j2), (int)j2.size(), d_sums, serial, stream_);
+    sums(j1, d_sums, serial);
+    sums(j2, d_sums, serial);
     long long* d_zero = (long long*)arena_.alloc(sizeof(long long));
     check(hipMemsetAsync(d_zero, 0, sizeof(long long), stream_), "memset");
     std::vector<int> lag_pos;
@@ -305,7 +321,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         const long long* lp = s.lagjob >= 0 ? d_lags + lag_pos[s.lagjob] : d_zero;
         j3.push_back(SumJob{s.ddeg, s.dl, kSumSqFrom, 6 * i + 5, 0, 0, lp, s.rl / 2});
     }
-    launch_sums(upload(j3), (int)j3.size(), d_sums, serial, stream_);
+    sums(j3, d_sums, serial);
     std::vector<VadJob> vj;
     size_t vad_bytes = 0;
     if (opts_.mode == 1) {
@@ -344,6 +360,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         memcpy(h_sums.data(), ps, sizeof(double) * 6 * n);
         memcpy(h_active.data(), pa, vad_bytes);
     }
+    lap(0);
 
     // ------------------------------------------------------------ host plan
     for (int i = 0; i < n; i++) {
@@ -393,6 +410,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         for (int p = 0; p < s.np; p++) s.width = std::max(s.width, s.hi[p] + S);
     }
 
+    lap(1);
     // ------------------------------------------------------------ phase B
     std::vector<ScaleJob> scj;
     std::vector<GtJob> gj;
@@ -529,6 +547,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             }
         }
         sync();
+        lap(2);
         if (timed) {
             float ms = 0.f;
             check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
@@ -573,6 +592,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         double* ph = (double*)pinned_.alloc(sizeof(double) * (h.size() + 1));
         readback(ph, d_stats, sizeof(double) * h.size());
         sync();
+        lap(3);
         memcpy(h.data(), ph, sizeof(double) * h.size());
         for (int i = 0; i < n; i++)
             if (st[i].live) {
@@ -613,6 +633,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         }
     }
 
+    lap(4);
     // ------------------------------------------------------------ phase C
     if (!opts_.no_realign) {
         struct RJ {
@@ -667,14 +688,15 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                 rj.lagjob = (int)rlj.size();
                 rlj.push_back(LagJob{rj.sref, rj.sdeg, rj.rlen, rj.dlen, z, nfft, b, b + 1, b + 2, b + 3});
             }
-            launch_sums(upload(a1), (int)a1.size(), d_rs, serial, stream_);
-            launch_sums(upload(a2), (int)a2.size(), d_rs, serial, stream_);
+            sums(a1, d_rs, serial);
+            sums(a2, d_rs, serial);
             std::vector<int> rpos;
             long long* d_rlags = lag_search(rlj, d_rs, rpos);
             std::vector<long long> h_rl(rlj.size());
             long long* pl = (long long*)pinned_.alloc(sizeof(long long) * rlj.size());
             readback(pl, d_rlags, sizeof(long long) * rlj.size());
             sync();
+            lap(5);
             for (size_t q = 0; q < rlj.size(); q++) h_rl[q] = pl[rpos[q]];
 
             // Trim geometry, frame reuse, spectrograms of what is kept.
@@ -748,6 +770,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             double* ph = (double*)pinned_.alloc(sizeof(double) * (h.size() + 1));
             readback(ph, d_rstats, sizeof(double) * h.size());
             sync();
+            lap(6);
             memcpy(h.data(), ph, sizeof(double) * h.size());
             // realign.zig:398-460 (finishRealign)
             for (size_t q = 0; q < rstj.size(); q++) {
@@ -775,6 +798,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
 
     for (int i = 0; i < n; i++)
         if (st[i].live) finish(out[i], rp);
+    lap(7);
 }
 
 // visqol.zig:595-709 (buildResult) and visqol.zig:81-98 (mapping).
@@ -872,17 +896,17 @@ long long Engine::best_lag(const double* a, long long la, const double* b, long 
     double* db = (double*)arena_.alloc(sizeof(double) * std::max(1LL, lb));
     if (la) check(hipMemcpyAsync(da, a, sizeof(double) * la, hipMemcpyHostToDevice, stream_), "upload");
     if (lb) check(hipMemcpyAsync(db, b, sizeof(double) * lb, hipMemcpyHostToDevice, stream_), "upload");
-    double* sums = (double*)arena_.alloc(sizeof(double) * 4);
-    check(hipMemsetAsync(sums, 0, sizeof(double) * 4, stream_), "memset");
+    double* d_s = (double*)arena_.alloc(sizeof(double) * 4);
+    check(hipMemsetAsync(d_s, 0, sizeof(double) * 4, stream_), "memset");
     std::vector<SumJob> s1{SumJob{da, la, kSum, 0, 0, 0, nullptr, 0}, SumJob{db, lb, kSum, 1, 0, 0, nullptr, 0}};
     std::vector<SumJob> s2{SumJob{da, la, kNyquist, 2, 0, 0, nullptr, 0}, SumJob{db, lb, kNyquist, 3, 1, 0, nullptr, 0}};
-    launch_sums(upload(s1), 2, sums, !opts_.fast, stream_);
-    launch_sums(upload(s2), 2, sums, !opts_.fast, stream_);
+    sums(s1, d_s, !opts_.fast);
+    sums(s2, d_s, !opts_.fast);
     const long long nfft = fft_size_for(std::max(la, lb));
     double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
     std::vector<LagJob> lj{LagJob{da, db, la, lb, z, nfft, 0, 1, 2, 3}};
     std::vector<int> pos;
-    long long* d_lag = lag_search(lj, sums, pos);
+    long long* d_lag = lag_search(lj, d_s, pos);
     long long* p = (long long*)pinned_.alloc(sizeof(long long));
     readback(p, d_lag, sizeof(long long));
     sync();

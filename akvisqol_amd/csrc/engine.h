@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <map>
 #include <memory>
 #include <string>
@@ -118,6 +119,12 @@ class Engine {
         double gammatone_flop = 0;  // 30 per (sample, band) of every frame
         double gammatone_bytes = 0; // signal samples read + spectrogram written
         long long launches = 0;
+        // Host wall time of each stretch of a batch call: phase A up to its
+        // readback, host planning, phase B up to the DP readback, the match
+        // statistics, their host decoding, phase C's lag searches, phase C's
+        // spectrograms and statistics, and the final aggregation.
+        static constexpr int kPhases = 8;
+        double phase_ms[kPhases] = {0};
     };
     void set_profiling(bool on) { profiling_ = on; }
     Profile profile() const { return prof_; }
@@ -125,6 +132,8 @@ class Engine {
 
    private:
     bool profiling_ = false;
+    std::chrono::steady_clock::time_point lap_t_;
+    void lap(int phase);
     Profile prof_;
     hipEvent_t ev_[2] = {nullptr, nullptr};
     struct DevRate {
@@ -143,6 +152,8 @@ class Engine {
     void check(hipError_t e, const char* what);
     // One chunk of pairs (bounded HBM footprint).
     void run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages);
+    // Reductions of `jobs` into d_out (serial: the reference's order).
+    void sums(const std::vector<SumJob>& jobs, double* d_out, bool serial);
     // FFT cross-correlation lags of `jobs` (envelope sums already in
     // d_sums); pos[i] is job i's slot in the returned device lag array.
     long long* lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos);

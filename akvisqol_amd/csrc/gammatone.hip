@@ -5,9 +5,10 @@
 // consecutive frames of ONE band, so the band's eight coefficients are
 // wave-uniform (scalar registers) and the four cascaded DF2T biquads run in
 // vector registers, in the reference's exact operation order.  A block
-// holds `bands_per_block` waves over the same 64 frames; the frames'
-// Hann-windowed samples are staged chunk by chunk in LDS (coalesced 512 B
-// rows from HBM, x·hann computed once per block and shared by its bands).
+// holds four band waves over the same 64 frames; the frames' Hann-windowed
+// samples are staged chunk by chunk in LDS (coalesced 512 B rows from HBM,
+// x·hann computed once per block and shared by its bands), the next chunk's
+// loads in flight while the current chunk's recurrences run.
 //
 // Exact variant: mul and add rounded separately (-ffp-contract=off), the
 // output bit-identical to the CPU recurrence.  Fast variant: the same
@@ -20,18 +21,20 @@ namespace vq {
 
 constexpr int kFramesPerWave = 64;
 constexpr int kChunk = 64;
+constexpr int kBandsPerBlock = 4;
+constexpr int kThreadsGt = 64 * kBandsPerBlock;
+constexpr int kStage = kFramesPerWave * kChunk / kThreadsGt;  // tile elements staged per thread
 
 template <bool FAST>
-__global__ __launch_bounds__(512) void k_gammatone(const GtJob* __restrict__ jobs, const BandCoef* __restrict__ coef,
-                                                   const double* __restrict__ hann, int nb, int window, int hop,
-                                                   int bands_per_block) {
+__global__ __launch_bounds__(kThreadsGt) void k_gammatone(const GtJob* __restrict__ jobs, const BandCoef* __restrict__ coef,
+                                                          const double* __restrict__ hann, int nb, int window, int hop) {
     const GtJob job = jobs[blockIdx.z];
     const int f0 = blockIdx.x * kFramesPerWave;
     if (f0 >= job.nf) return;  // whole block idle: uniform exit
     __shared__ double tile[kFramesPerWave][kChunk + 1];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int band = blockIdx.y * bands_per_block + wave;
+    const int band = blockIdx.y * kBandsPerBlock + wave;
     const bool active_band = band < nb;
     const int frame = f0 + lane;
     const bool active = active_band && frame < job.nf;
@@ -41,18 +44,38 @@ __global__ __launch_bounds__(512) void k_gammatone(const GtJob* __restrict__ job
     else k = BandCoef{0, 0, 0, 0, 0, 0, 0, 0};
     double s1c0 = 0, s1c1 = 0, s2c0 = 0, s2c1 = 0, s3c0 = 0, s3c1 = 0, s4c0 = 0, s4c1 = 0, sum_sq = 0;
 
-    const double* sig = job.sig;
-    const int nthreads = blockDim.x;
+    // Staging: thread t stages samples i0 + lane of frames f0 + (t >> 6) +
+    // 4u (u < kStage), so every wave reads 512 B rows and the Hann factor is
+    // one value per thread and chunk.  The next chunk's loads are issued
+    // (unconditionally, from clamped in-range addresses) before the current
+    // chunk's recurrence runs; masking and x·hann happen at the LDS write.
+    const auto* sig = as_global(job.sig);
+    const int fr0 = threadIdx.x >> 6;
+    double v[kStage];
+    double h = 0.0;
+    auto load_chunk = [&](int i0) {
+        const int i = i0 + lane;
+        const int ic = i < window ? i : 0;
+        h = hann[ic];
+#pragma unroll
+        for (int u = 0; u < kStage; u++) {
+            const int fr = fr0 + u * kBandsPerBlock;
+            const int fc = f0 + fr < job.nf ? f0 + fr : f0;
+            v[u] = sig[(size_t)fc * hop + ic];
+        }
+    };
+    load_chunk(0);
     for (int i0 = 0; i0 < window; i0 += kChunk) {
         const int clen = min(kChunk, window - i0);
         __syncthreads();
-        for (int e = threadIdx.x; e < kFramesPerWave * kChunk; e += nthreads) {
-            const int fr = e / kChunk, i = e % kChunk;
-            double v = 0.0;
-            if (f0 + fr < job.nf && i < clen) v = sig[(size_t)(f0 + fr) * hop + i0 + i] * hann[i0 + i];
-            tile[fr][i] = v;
+        const bool in = i0 + lane < window;
+#pragma unroll
+        for (int u = 0; u < kStage; u++) {
+            const int fr = fr0 + u * kBandsPerBlock;
+            tile[fr][lane] = (in && f0 + fr < job.nf) ? v[u] * h : 0.0;
         }
         __syncthreads();
+        if (i0 + kChunk < window) load_chunk(i0 + kChunk);
         if (active) {
             for (int i = 0; i < clen; i++) {
                 const double s = tile[lane][i];
@@ -97,13 +120,12 @@ __global__ __launch_bounds__(512) void k_gammatone(const GtJob* __restrict__ job
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const double* d_hann, int nb,
                       int window, int hop, bool fast, hipStream_t stream) {
     if (njobs == 0 || max_nf == 0) return;
-    const int bpb = nb % 4 == 0 ? 4 : (nb % 3 == 0 ? 3 : 4);
-    dim3 grid((max_nf + kFramesPerWave - 1) / kFramesPerWave, (nb + bpb - 1) / bpb, njobs);
-    dim3 block(64 * bpb);
+    dim3 grid((max_nf + kFramesPerWave - 1) / kFramesPerWave, (nb + kBandsPerBlock - 1) / kBandsPerBlock, njobs);
+    dim3 block(kThreadsGt);
     if (fast)
-        hipLaunchKernelGGL(k_gammatone<true>, grid, block, 0, stream, d_jobs, d_coef, d_hann, nb, window, hop, bpb);
+        hipLaunchKernelGGL(k_gammatone<true>, grid, block, 0, stream, d_jobs, d_coef, d_hann, nb, window, hop);
     else
-        hipLaunchKernelGGL(k_gammatone<false>, grid, block, 0, stream, d_jobs, d_coef, d_hann, nb, window, hop, bpb);
+        hipLaunchKernelGGL(k_gammatone<false>, grid, block, 0, stream, d_jobs, d_coef, d_hann, nb, window, hop);
 }
 
 }  // namespace vq

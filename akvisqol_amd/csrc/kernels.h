@@ -40,8 +40,12 @@ struct SumJob {
     long long half;        // kSumSqFrom: ref_len / 2
 };
 // out[job.slot] = reduction of each job.  `serial` keeps the reference's
-// left-to-right order (bit-exact); otherwise a deterministic tree order.
-void launch_sums(const SumJob* d_jobs, int njobs, double* d_out, bool serial, hipStream_t stream);
+// left-to-right order (bit-exact); otherwise a deterministic tree order over
+// up to kSumParts shares per job (max_len: longest job), d_partials holding
+// njobs * kSumParts doubles.
+constexpr int kSumParts = 64;
+void launch_sums(const SumJob* d_jobs, int njobs, long long max_len, double* d_out, bool serial, double* d_partials,
+                 hipStream_t stream);
 
 // ---------------------------------------------------------------- envelope + FFT xcorr
 // One lag search: z[i] = (env(a)[i], env(b)[i]) zero padded to n (a power

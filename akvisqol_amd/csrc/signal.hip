@@ -24,12 +24,14 @@ __device__ inline double sum_term(const SumJob& j, long long i, double mean) {
 
 constexpr int kSerialTile = 2048;  // terms per LDS tile (16 KiB)
 
-// One block per job.  serial: the terms are added in sample order by one
+// serial: one block per job; the terms are added in sample order by one
 // lane (the reference's rounding sequence).  Each term (x, x·x or
 // ±(x - mean)) rounds on its own, so all lanes compute terms for the next
 // tile into a second LDS buffer while lane 0 runs the add chain over the
-// current one.  tree: fixed-shape block reduction.
-__global__ __launch_bounds__(256) void k_sums(const SumJob* __restrict__ jobs, double* __restrict__ out, int serial) {
+// current one.  tree: kSumParts blocks per job, each a fixed-shape block
+// reduction of one contiguous share, then k_sums_finish adds the shares.
+__global__ __launch_bounds__(256) void k_sums(const SumJob* __restrict__ jobs, double* __restrict__ out,
+                                              double* __restrict__ partials, int serial, int parts) {
     const SumJob j = jobs[blockIdx.x];
     long long start = 0;
     if (j.kind == kSumSqFrom) {
@@ -78,21 +80,43 @@ __global__ __launch_bounds__(256) void k_sums(const SumJob* __restrict__ jobs, d
         if (threadIdx.x == 0) out[j.slot] = s;
         return;
     }
+    // tree: this block's share of [start, len) into partials[job][blockIdx.y]
+    const long long total = j.len - start;
+    const long long per = (total + parts - 1) / parts;
+    const long long lo = start + per * blockIdx.y;
+    const long long hi = lo + per < j.len ? lo + per : j.len;
     __shared__ double part[256];
     double s = 0.0;
-    for (long long i = start + threadIdx.x; i < j.len; i += blockDim.x) s += sum_term(j, i, mean);
+    for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += sum_term(j, i, mean);
     part[threadIdx.x] = s;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[j.slot] = part[0];
+    if (threadIdx.x == 0) partials[(long long)blockIdx.x * kSumParts + blockIdx.y] = part[0];
 }
 
-void launch_sums(const SumJob* d_jobs, int njobs, double* d_out, bool serial, hipStream_t stream) {
+// out[slot] = fixed-order sum of a job's partials (one wave per job).
+__global__ __launch_bounds__(64) void k_sums_finish(const SumJob* __restrict__ jobs, const double* __restrict__ partials,
+                                                    double* __restrict__ out, int parts) {
+    double v = (int)threadIdx.x < parts ? partials[(long long)blockIdx.x * kSumParts + threadIdx.x] : 0.0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (threadIdx.x == 0) out[jobs[blockIdx.x].slot] = v;
+}
+
+void launch_sums(const SumJob* d_jobs, int njobs, long long max_len, double* d_out, bool serial, double* d_partials,
+                 hipStream_t stream) {
     if (njobs == 0) return;
-    hipLaunchKernelGGL(k_sums, dim3(njobs), dim3(256), 0, stream, d_jobs, d_out, serial ? 1 : 0);
+    if (serial) {
+        hipLaunchKernelGGL(k_sums, dim3(njobs, 1), dim3(256), 0, stream, d_jobs, d_out, d_partials, 1, 1);
+        return;
+    }
+    // about 8 K terms per block
+    long long p = (max_len + 8191) / 8192;
+    const int parts = p < 1 ? 1 : (p > kSumParts ? kSumParts : (int)p);
+    hipLaunchKernelGGL(k_sums, dim3(njobs, parts), dim3(256), 0, stream, d_jobs, d_out, d_partials, 0, parts);
+    hipLaunchKernelGGL(k_sums_finish, dim3(njobs), dim3(64), 0, stream, d_jobs, (const double*)d_partials, d_out, parts);
 }
 
 // ---------------------------------------------------------------- signal prep

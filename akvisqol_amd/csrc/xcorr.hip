@@ -39,7 +39,8 @@ __device__ inline long long env_fft_size(long long l) {
 
 // envelope.zig:23-53 closed form for sample i of a signal with the given
 // plain sum and alternating (Nyquist) sum.
-__device__ inline double env_at(const double* x, long long i, long long len, double sum, double nyq) {
+template <typename P>
+__device__ inline double env_at(P x, long long i, long long len, double sum, double nyq) {
     const double mean = sum / (double)len;
     const double nq = nyq / (double)env_fft_size(len);
     const double term = (i & 1) ? -nq : nq;
@@ -48,8 +49,8 @@ __device__ inline double env_at(const double* x, long long i, long long len, dou
 
 __device__ inline double2 packed_env(const LagJob& j, const double* sums, long long i) {
     double re = 0.0, im = 0.0;
-    if (i < j.la) re = env_at(j.a, i, j.la, sums[j.sum_a], sums[j.nyq_a]);
-    if (i < j.lb) im = env_at(j.b, i, j.lb, sums[j.sum_b], sums[j.nyq_b]);
+    if (i < j.la) re = env_at(as_global(j.a), i, j.la, sums[j.sum_a], sums[j.nyq_a]);
+    if (i < j.lb) im = env_at(as_global(j.b), i, j.lb, sums[j.sum_b], sums[j.nyq_b]);
     return make_double2(re, im);
 }
 
@@ -67,43 +68,108 @@ __device__ inline long long scan_order(long long i, long long n, long long max_l
     return -1;
 }
 
-template <bool DIT, int SRC, bool ARGMAX>
-__global__ __launch_bounds__(kThreads) void k_fft_pass(const LagJob* __restrict__ jobs, long long n, long long S_hi, int k,
-                                                       int R, int Bn, const double2* __restrict__ tw,
-                                                       const double* __restrict__ sums,
-                                                       unsigned long long* __restrict__ partials) {
-    __shared__ double2 sm[kTile];
-    const LagJob job = jobs[blockIdx.y];
-    const int G = 1 << k;
-    const int GR = G * R;
-    const long long q = S_hi >> k;
-    const long long per_blk = q / R;
-    const long long wg = blockIdx.x;
-    const long long blk0 = (wg / per_blk) * Bn;
-    const long long r0 = (wg - (wg / per_blk) * per_blk) * R;
-    const long long base = blk0 * S_hi + r0;
-    double2* z = job.z;
-    for (int e = threadIdx.x; e < GR * Bn; e += kThreads) {
-        const int bb = e / GR, rem = e - bb * GR;
-        const int m = rem / R, rr = rem - m * R;
-        const long long i = base + (long long)bb * S_hi + (long long)m * q + rr;
-        sm[e] = SRC == kSrcEnvelope ? packed_env(job, sums, i) : z[i];
+// First maximum of |Re| in scan order over a tile in LDS; writes the
+// (max bits, scan position) pair of this workgroup to part[0..1].
+template <typename GIDX>
+__device__ inline void tile_argmax(const double2* sm, int count, GIDX gidx, long long n, const LagJob& job,
+                                   unsigned long long* part) {
+    const long long big = job.la > job.lb ? job.la : job.lb;
+    const long long max_lag = big - 1;
+    unsigned long long best_v = 0, best_o = ~0ULL;
+    for (int e = threadIdx.x; e < count; e += kThreads) {
+        const long long o = scan_order(gidx(e), n, max_lag);
+        if (o < 0) continue;
+        const unsigned long long v = f64_bits(fabs(sm[e].x));
+        if (v > best_v || (v == best_v && (unsigned long long)o < best_o)) {
+            best_v = v;
+            best_o = (unsigned long long)o;
+        }
     }
-    // Butterfly u of this thread at level li: (i0, i1) in LDS and its twiddle.
-    const int half = GR / 2;
+    __shared__ unsigned long long rv[kThreads], ro[kThreads];
+    rv[threadIdx.x] = best_v;
+    ro[threadIdx.x] = best_o;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const unsigned long long v2 = rv[threadIdx.x + s], o2 = ro[threadIdx.x + s];
+            if (v2 > rv[threadIdx.x] || (v2 == rv[threadIdx.x] && o2 < ro[threadIdx.x])) {
+                rv[threadIdx.x] = v2;
+                ro[threadIdx.x] = o2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[0] = rv[0];
+        part[1] = ro[0];
+    }
+}
+
+// DIF butterfly (fft.zig:45-90): (a, b) -> (a + b, (a - b)·w).
+__device__ inline void bf_dif(double2& a, double2& b, double2 w) {
+    const double2 t = csub(a, b);
+    a = cadd(a, b);
+    b = cmul(t, w);
+}
+// DIT butterfly with the conjugate twiddle (fft.zig:92-166).
+__device__ inline void bf_dit(double2& a, double2& b, double2 w) {
+    const double2 bw = cmul_conj(b, w);
+    b = csub(a, bw);
+    a = cadd(a, bw);
+}
+
+// Every size is a power of two: lgn = log2 n, lgS = log2 S_hi, k stages,
+// R = 2^lgR residues, Bn = 2^lgBn span blocks per workgroup.
+template <bool DIT, int SRC, bool ARGMAX>
+__global__ __launch_bounds__(kThreads) void k_fft_pass(const LagJob* __restrict__ jobs, int lgn, int lgS, int k, int lgR,
+                                                       int lgBn, const double2* __restrict__ tw,
+                                                       const double* __restrict__ sums,
+                                                       unsigned long long* __restrict__ partials, int njobs) {
+    __shared__ double2 sm[kTile];
+    // Job index fastest: workgroups in flight together share residues, so
+    // their twiddle reads hit the same cache lines.
+    const int jb = (int)(blockIdx.x % (unsigned)njobs);
+    const long long wg = blockIdx.x / (unsigned)njobs;
+    const long long nwg = gridDim.x / (unsigned)njobs;
+    const LagJob job = jobs[jb];
+    const long long n = 1LL << lgn;
+    const int lgq = lgS - k;                 // q = S_hi / 2^k
+    const int lgGR = k + lgR;                // elements per span block in the tile
+    const int lgPer = lgq - lgR;             // workgroups per span-block group
+    const long long blk0 = (wg >> lgPer) << lgBn;
+    const long long r0 = (wg & ((1LL << lgPer) - 1)) << lgR;
+    const long long base = (blk0 << lgS) + r0;
+    const int count = 1 << (lgGR + lgBn);   // tile elements
+    double2* z = job.z;
+    // tile element e = (bb, m, rr) <-> z[base + bb·S_hi + m·q + rr]
+    auto gidx = [&](int e) -> long long {
+        const int bb = e >> lgGR;
+        const int m = (e >> lgR) & ((1 << k) - 1);
+        const int rr = e & ((1 << lgR) - 1);
+        return base + ((long long)bb << lgS) + ((long long)m << lgq) + rr;
+    };
+    for (int e = threadIdx.x; e < count; e += kThreads) {
+        const long long i = gidx(e);
+        sm[e] = SRC == kSrcEnvelope ? packed_env(job, sums, i) : load2(z, i);
+    }
+    // Butterfly u of this thread at level li: (i0, i1) in LDS and its twiddle
+    // index.  Stage l has distance hm·q (hm = 2^(k-1-l)) and twiddle
+    // W_{S_hi/2^l}^j = tw[j · n/(S_hi/2^l)].
+    const int lgHalf = lgGR - 1;
     auto butterfly = [&](int li, int u, int& i0, int& i1) -> long long {
         const int l = DIT ? k - 1 - li : li;
-        const int hm = G >> (l + 1);
+        const int lgHm = k - 1 - l;
         const int t = threadIdx.x + u * kThreads;
-        const int bb = t / half, rem = t - bb * half;
-        const int rr = rem % R, bm = rem / R;
-        const int m0 = (bm / hm) * 2 * hm + (bm % hm);
-        i0 = bb * GR + m0 * R + rr;
-        i1 = i0 + hm * R;
-        const long long j = (long long)(bm % hm) * q + r0 + rr;
-        return j * (n / (S_hi >> l));
+        const int bb = t >> lgHalf, rem = t & ((1 << lgHalf) - 1);
+        const int rr = rem & ((1 << lgR) - 1), bm = rem >> lgR;
+        const int lo = bm & ((1 << lgHm) - 1);
+        const int m0 = ((bm >> lgHm) << (lgHm + 1)) | lo;
+        i0 = (bb << lgGR) + (m0 << lgR) + rr;
+        i1 = i0 + (1 << (lgHm + lgR));
+        const long long j = ((long long)lo << lgq) + r0 + rr;
+        return j << (lgn - lgS + l);
     };
-    const int nbf = half * Bn;  // == kTile / 2 whenever the tile is full
+    const int nbf = count >> 1;  // == kTile / 2 whenever the tile is full
     double2 w[kBfPerThread];
     {
         int a, b;
@@ -137,47 +203,180 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const LagJob* __restrict_
             for (int u = 0; u < kBfPerThread; u++) w[u] = wn[u];
     }
     if (!ARGMAX) {
-        for (int e = threadIdx.x; e < GR * Bn; e += kThreads) {
-            const int bb = e / GR, rem = e - bb * GR;
-            const int m = rem / R, rr = rem - m * R;
-            z[base + (long long)bb * S_hi + (long long)m * q + rr] = sm[e];
-        }
+        for (int e = threadIdx.x; e < count; e += kThreads) store2(z, gidx(e), sm[e]);
         return;
     }
-    // First maximum of |Re| in scan order within this workgroup.
-    const long long big = job.la > job.lb ? job.la : job.lb;
-    const long long max_lag = big - 1;
-    unsigned long long best_v = 0, best_o = ~0ULL;
-    for (int e = threadIdx.x; e < GR * Bn; e += kThreads) {
-        const int bb = e / GR, rem = e - bb * GR;
-        const int m = rem / R, rr = rem - m * R;
-        const long long i = base + (long long)bb * S_hi + (long long)m * q + rr;
-        const long long o = scan_order(i, n, max_lag);
-        if (o < 0) continue;
-        const unsigned long long v = f64_bits(fabs(sm[e].x));
-        if (v > best_v || (v == best_v && (unsigned long long)o < best_o)) {
-            best_v = v;
-            best_o = (unsigned long long)o;
+    tile_argmax(sm, count, gidx, n, job, partials + 2 * (jb * nwg + wg));
+}
+
+// Full-tile pass (n >= kTile, so every tile holds kTile elements) with K
+// stages, fully unrolled.  Stages are taken two at a time: a thread holds
+// the four elements m0, m0+h, m0+2h, m0+3h of one residue (h the smaller of
+// the two stage distances, in units of q) and runs both stages'
+// butterflies on them in registers with those stages' twiddles — the same
+// radix-2 operations as the stage-by-stage transform.  An odd K ends with
+// one radix-2 stage.  Twiddles of the next step load while the current one
+// computes.
+template <int K, bool DIT>
+struct TileStep {
+    // processing index li -> stage l (distance 2^(K-1-l) in units of q)
+    static constexpr int stage(int li) { return DIT ? K - 1 - li : li; }
+};
+
+template <int K, bool DIT, int SRC, bool ARGMAX>
+__global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict__ jobs, int lgn, int lgS, int lgR, int lgBn,
+                                                       const double2* __restrict__ tw, const double* __restrict__ sums,
+                                                       unsigned long long* __restrict__ partials, int njobs) {
+    __shared__ double2 sm[kTile];
+    const int jb = (int)(blockIdx.x % (unsigned)njobs);
+    const long long wg = blockIdx.x / (unsigned)njobs;
+    const long long nwg = gridDim.x / (unsigned)njobs;
+    const LagJob job = jobs[jb];
+    const long long n = 1LL << lgn;
+    const int lgq = lgS - K;
+    const int lgGR = K + lgR;
+    const int lgPer = lgq - lgR;
+    const long long blk0 = (wg >> lgPer) << lgBn;
+    const long long r0 = (wg & ((1LL << lgPer) - 1)) << lgR;
+    const long long base = (blk0 << lgS) + r0;
+    const int sh = lgn - lgS;  // twiddle stride of stage l is 2^(sh + l)
+    double2* z = job.z;
+    auto gidx = [&](int e) -> long long {
+        const int bb = e >> lgGR;
+        const int m = (e >> lgR) & ((1 << K) - 1);
+        const int rr = e & ((1 << lgR) - 1);
+        return base + ((long long)bb << lgS) + ((long long)m << lgq) + rr;
+    };
+    constexpr int kPer = kTile / kThreads;
+    {
+        double2 v[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const long long i = gidx(threadIdx.x + u * kThreads);
+            v[u] = SRC == kSrcEnvelope ? packed_env(job, sums, i) : load2(z, i);
         }
+#pragma unroll
+        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
     }
-    __shared__ unsigned long long rv[kThreads], ro[kThreads];
-    rv[threadIdx.x] = best_v;
-    ro[threadIdx.x] = best_o;
+
+    // Radix-4 group g (two per thread): residue rr, quarter index mq, block bb.
+    // lgH = log2 of the smaller distance.  Element x_c sits at m0 + c·h.
+    auto quad = [&](int g, int lgH, int& i0, int& lo, int& rr) {
+        rr = g & ((1 << lgR) - 1);
+        const int mq = (g >> lgR) & ((1 << (K - 2)) - 1);
+        const int bb = g >> (lgR + K - 2);
+        lo = mq & ((1 << lgH) - 1);
+        const int m0 = ((mq >> lgH) << (lgH + 2)) | lo;
+        i0 = (bb << lgGR) + (m0 << lgR) + rr;
+    };
+    // Twiddle of stage l at position p (< distance) for residue r0 + rr.
+    auto twi = [&](int l, int p, int rr) -> double2 {
+        const long long j = ((long long)p << lgq) + r0 + rr;
+        return tw[j << (sh + l)];
+    };
+    constexpr int kSteps = (K + 1) / 2;
+    // w[u][0..1]: the larger-distance stage at lo and lo + h; w[u][2]: the
+    // smaller-distance stage at lo.  Radix-2 step: w[u][0] for butterfly u.
+    double2 w[4][3];
+    auto load_tw = [&](int st, double2 (&dst)[4][3]) {
+        const int li = 2 * st;
+        if (li + 1 < K) {
+            const int lgH = DIT ? li : K - 2 - li;  // smaller distance 2^lgH
+            const int l_big = K - 2 - lgH, l_small = K - 1 - lgH;
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                int i0, lo, rr;
+                quad(threadIdx.x + u * kThreads, lgH, i0, lo, rr);
+                dst[u][0] = twi(l_big, lo, rr);
+                dst[u][1] = twi(l_big, lo + (1 << lgH), rr);
+                dst[u][2] = twi(l_small, lo, rr);
+            }
+        } else {
+            const int l = TileStep<K, DIT>::stage(li);
+            const int lgHm = K - 1 - l;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int t = threadIdx.x + u * kThreads;
+                const int rem = t & ((1 << (lgGR - 1)) - 1);
+                const int rr = rem & ((1 << lgR) - 1), bm = rem >> lgR;
+                dst[u][0] = twi(l, bm & ((1 << lgHm) - 1), rr);
+            }
+        }
+    };
+    load_tw(0, w);
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            const unsigned long long v2 = rv[threadIdx.x + s], o2 = ro[threadIdx.x + s];
-            if (v2 > rv[threadIdx.x] || (v2 == rv[threadIdx.x] && o2 < ro[threadIdx.x])) {
-                rv[threadIdx.x] = v2;
-                ro[threadIdx.x] = o2;
+#pragma unroll
+    for (int st = 0; st < kSteps; st++) {
+        const int li = 2 * st;
+        double2 wn[4][3];
+        if (li + 1 < K) {
+            const int lgH = DIT ? li : K - 2 - li;
+            double2 x[2][4];
+            int ib[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                int lo, rr;
+                quad(threadIdx.x + u * kThreads, lgH, ib[u], lo, rr);
+#pragma unroll
+                for (int c = 0; c < 4; c++) x[u][c] = sm[ib[u] + ((c << lgH) << lgR)];
+            }
+            if (st + 1 < kSteps) load_tw(st + 1, wn);
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                if (DIT) {  // smaller distance first, then the larger
+                    bf_dit(x[u][0], x[u][1], w[u][2]);
+                    bf_dit(x[u][2], x[u][3], w[u][2]);
+                    bf_dit(x[u][0], x[u][2], w[u][0]);
+                    bf_dit(x[u][1], x[u][3], w[u][1]);
+                } else {    // larger distance first, then the smaller
+                    bf_dif(x[u][0], x[u][2], w[u][0]);
+                    bf_dif(x[u][1], x[u][3], w[u][1]);
+                    bf_dif(x[u][0], x[u][1], w[u][2]);
+                    bf_dif(x[u][2], x[u][3], w[u][2]);
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++) sm[ib[u] + ((c << lgH) << lgR)] = x[u][c];
+            }
+        } else {
+            const int l = TileStep<K, DIT>::stage(li);
+            const int lgHm = K - 1 - l;
+            double2 x[4][2];
+            int ia[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int t = threadIdx.x + u * kThreads;
+                const int bb = t >> (lgGR - 1), rem = t & ((1 << (lgGR - 1)) - 1);
+                const int rr = rem & ((1 << lgR) - 1), bm = rem >> lgR;
+                const int m0 = ((bm >> lgHm) << (lgHm + 1)) | (bm & ((1 << lgHm) - 1));
+                ia[u] = (bb << lgGR) + (m0 << lgR) + rr;
+                x[u][0] = sm[ia[u]];
+                x[u][1] = sm[ia[u] + ((1 << lgHm) << lgR)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (DIT) bf_dit(x[u][0], x[u][1], w[u][0]);
+                else bf_dif(x[u][0], x[u][1], w[u][0]);
+                sm[ia[u]] = x[u][0];
+                sm[ia[u] + ((1 << lgHm) << lgR)] = x[u][1];
             }
         }
         __syncthreads();
+        if (st + 1 < kSteps) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) w[u][c] = wn[u][c];
+        }
     }
-    if (threadIdx.x == 0) {
-        partials[2 * ((long long)blockIdx.y * gridDim.x + blockIdx.x)] = rv[0];
-        partials[2 * ((long long)blockIdx.y * gridDim.x + blockIdx.x) + 1] = ro[0];
+    if (!ARGMAX) {
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int e = threadIdx.x + u * kThreads;
+            store2(z, gidx(e), sm[e]);
+        }
+        return;
     }
+    tile_argmax(sm, kTile, gidx, n, job, partials + 2 * (jb * nwg + wg));
 }
 
 // xcorr.zig:211-229 cross terms, in place in bit-reversed order.
@@ -194,16 +393,16 @@ __global__ __launch_bounds__(256) void k_cross_terms(const LagJob* __restrict__ 
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (i < 2) {
-        const double2 v = z[i];
-        z[i] = cross_term(v, v);
+        const double2 v = load2(z, i);
+        store2(z, i, cross_term(v, v));
         return;
     }
     const long long h = 1LL << (63 - __builtin_clzll((unsigned long long)i));
     const long long p = 3 * h - 1 - i;
     if (i < p) {
-        const double2 za = z[i], zb = z[p];
-        z[i] = cross_term(za, zb);
-        z[p] = cross_term(zb, za);
+        const double2 za = load2(z, i), zb = load2(z, p);
+        store2(z, i, cross_term(za, zb));
+        store2(z, p, cross_term(zb, za));
     }
 }
 
@@ -241,36 +440,58 @@ __global__ __launch_bounds__(256) void k_lag_reduce(const LagJob* __restrict__ j
 namespace {
 constexpr int kMaxFuse = 8;
 
-// Stage counts per pass, as even as possible.
+// Stage counts per pass, as even as possible, the smaller counts first: the
+// pass over the largest spans then reads the longest contiguous runs.
 int plan_passes(long long n, int* ks) {
     int L = 0;
     while ((1LL << L) < n) L++;
     const int np = (L + kMaxFuse - 1) / kMaxFuse;
-    for (int p = 0; p < np; p++) ks[p] = L / np + (p < L % np ? 1 : 0);
+    for (int p = 0; p < np; p++) ks[p] = L / np + (p >= np - L % np ? 1 : 0);
     return np;
 }
 
+int log2i(long long v) {
+    int l = 0;
+    while ((1LL << l) < v) l++;
+    return l;
+}
+
 struct PassShape {
-    int R, Bn;
+    int lgR, lgBn;
     long long wgs;
 };
 
+// R residues (<= q) and Bn span blocks (<= n / S_hi) filling one tile.
 PassShape pass_shape(long long n, long long S_hi, int k) {
-    const long long G = 1LL << k, q = S_hi >> k;
-    const long long want = kTile / G;
-    long long R = want < q ? want : q;
-    long long Bn = want / R;
-    if (Bn > n / S_hi) Bn = n / S_hi;
-    if (Bn < 1) Bn = 1;
-    return PassShape{(int)R, (int)Bn, (n / S_hi / Bn) * (q / R)};
+    const int lgn = log2i(n), lgS = log2i(S_hi), lgq = lgS - k;
+    const int lgWant = log2i(kTile) - k;
+    const int lgR = lgWant < lgq ? lgWant : lgq;
+    int lgBn = lgWant - lgR;
+    if (lgBn > lgn - lgS) lgBn = lgn - lgS;
+    return PassShape{lgR, lgBn, 1LL << ((lgn - lgS - lgBn) + (lgq - lgR))};
+}
+
+template <int K, bool DIT, int SRC, bool ARGMAX>
+void tile_pass(const PassShape& s, const LagJob* d_jobs, int njobs, long long n, long long S_hi, const double2* tw,
+               const double* sums, unsigned long long* partials, hipStream_t stream) {
+    hipLaunchKernelGGL((k_fft_tile<K, DIT, SRC, ARGMAX>), dim3((unsigned)(s.wgs * njobs)), dim3(kThreads), 0, stream,
+                       d_jobs, log2i(n), log2i(S_hi), s.lgR, s.lgBn, tw, sums, partials, njobs);
 }
 
 template <bool DIT, int SRC, bool ARGMAX>
 void pass(const LagJob* d_jobs, int njobs, long long n, long long S_hi, int k, const double2* tw, const double* sums,
           unsigned long long* partials, hipStream_t stream) {
     const PassShape s = pass_shape(n, S_hi, k);
-    hipLaunchKernelGGL((k_fft_pass<DIT, SRC, ARGMAX>), dim3((unsigned)s.wgs, njobs), dim3(kThreads), 0, stream, d_jobs, n,
-                       S_hi, k, s.R, s.Bn, tw, sums, partials);
+    if (n >= kTile && k >= 5 && k <= 8) {  // every tile full
+        switch (k) {
+            case 5: tile_pass<5, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 6: tile_pass<6, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 7: tile_pass<7, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            default: tile_pass<8, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+        }
+    }
+    hipLaunchKernelGGL((k_fft_pass<DIT, SRC, ARGMAX>), dim3((unsigned)(s.wgs * njobs)), dim3(kThreads), 0, stream, d_jobs,
+                       log2i(n), log2i(S_hi), k, s.lgR, s.lgBn, tw, sums, partials, njobs);
 }
 }  // namespace
 

@@ -122,6 +122,7 @@ def _load() -> ctypes.CDLL:
                                                     c_uint32, POINTER(POINTER(Result)), POINTER(c_int32)]),
         "visqol_hip_profile": (c_int, [c_void_p, c_int, c_int]),
         "visqol_hip_profile_read": (c_int, [c_void_p, D, D, D, POINTER(c_int64)]),
+        "visqol_hip_profile_phases": (c_int, [c_void_p, D, c_int]),
         "visqol_hip_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
         "visqol_hip_free": (c_int, [c_void_p]),
         "visqol_hip_upload": (c_int, [c_void_p, c_void_p, c_size_t]),
@@ -294,6 +295,16 @@ class Context:
         _check(lib.visqol_hip_profile_read(self._h, ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by),
                                            ctypes.byref(n)))
         return dict(ms=ms.value, flop=fl.value, bytes=by.value, launches=n.value)
+
+    PHASES = ("phase_a", "plan", "phase_b", "match_stats", "match_decode", "realign_lags", "realign_stats", "aggregate")
+
+    def profile_phases(self) -> dict:
+        """Host wall time (ms) of each stretch of the batch calls since the last reset."""
+        buf = (c_double * len(self.PHASES))()
+        rc = lib.visqol_hip_profile_phases(self._h, buf, len(self.PHASES))
+        if rc < 0:
+            _check(rc)
+        return {k: buf[i] for i, k in enumerate(self.PHASES)}
 
     def measure_batch(self, refs, degs, sample_rate: int, num_threads: int = 0):
         """(first error code, [result dict or None], [status]) as _native.py:215-225."""

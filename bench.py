@@ -78,6 +78,8 @@ def main() -> int:
             raise SystemExit(f"step failed: {rc}")
         print(f"bench: step {len(times)} {times[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
     prof = ctx.profile_read()
+    phases = {k: round(v / args.steps, 3) for k, v in ctx.profile_phases().items()}
+    print(f"bench: host wall ms per step by stretch {phases}", file=sys.stderr, flush=True)
     ctx.profile(False, True)
     step_s = float(np.mean(times))
     value = args.batch / step_s

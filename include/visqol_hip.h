@@ -55,6 +55,12 @@ int visqol_hip_measure_batch_device(visqol_ctx *ctx, size_t n, const double *con
  * the launch count since the last reset. */
 int visqol_hip_profile(visqol_ctx *ctx, int enable, int reset);
 int visqol_hip_profile_read(visqol_ctx *ctx, double *ms, double *flop, double *bytes, int64_t *launches);
+/* Host wall time (ms) of the stretches of batch calls since the last reset,
+ * writing min(n, 8) values: phase A to its readback, host planning, phase B
+ * to the DP readback, match statistics, their decoding, realignment lag
+ * searches, realignment spectrograms and statistics, final aggregation.
+ * Returns the number of stretches available (8). */
+int visqol_hip_profile_phases(visqol_ctx *ctx, double *ms, int n);
 
 /* Device-resident buffers for the batch entry point above. */
 int visqol_hip_alloc(size_t bytes, void **dev_ptr);
