@@ -471,6 +471,15 @@ int visqol_hip_profile_read(visqol_ctx* c, double* ms, double* flop, double* byt
     return VISQOL_OK;
 }
 
+int visqol_hip_profile_sim(visqol_ctx* c, double* ms, double* cells, int64_t* launches) {
+    if (!c) return invalid("ctx is NULL");
+    const Engine::Profile p = c->engine->profile();
+    if (ms) *ms = p.sim_ms;
+    if (cells) *cells = p.sim_cells;
+    if (launches) *launches = p.sim_launches;
+    return VISQOL_OK;
+}
+
 int visqol_hip_profile_phases(visqol_ctx* c, double* ms, int n) {
     if (!c) return invalid("ctx is NULL");
     if (n > 0 && !ms) return invalid("ms is NULL");
@@ -541,6 +550,24 @@ int visqol_hip_best_lag(const double* a, size_t la, const double* b, size_t lb, 
     std::lock_guard<std::mutex> g(g_stage_mu);
     try {
         *lag = stage_engine(0, false)->best_lag(a, (long long)la, b, (long long)lb);
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return VISQOL_OK;
+}
+
+int visqol_hip_xcorr(const double* a, size_t la, const double* b, size_t lb, double* out, size_t cap, size_t* n_out) {
+    if ((!a && la) || (!b && lb) || (!out && cap) || !n_out) return invalid("invalid argument");
+    if (std::max(la, lb) < 2) return invalid("xcorr needs a signal of at least 2 samples");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    try {
+        std::vector<double> v;
+        stage_engine(0, false)->xcorr(a, (long long)la, b, (long long)lb, v);
+        *n_out = v.size();
+        memcpy(out, v.data(), sizeof(double) * std::min(cap, v.size()));
     } catch (const BackendError& e) {
         set_error("%s", e.msg.c_str());
         return e.code;

@@ -434,7 +434,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         np_off[i] = np_total;
         np_total += (size_t)s.np;
         plane_off[i] = plane_total;
-        plane_total += (size_t)5 * nb * s.width;
+        plane_total += (size_t)kPlanes * nb * s.width;
         degs_off[i] = degs_total;
         degs_total += (size_t)s.dl2;
     }
@@ -501,10 +501,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         launch_scale(upload(scj), (int)scj.size(), max_len, stream_);
         GtJob* d_gj = upload(gj);
         if (profiling_) {
-            if (!ev_[0]) {
-                check(hipEventCreate(&ev_[0]), "hipEventCreate");
-                check(hipEventCreate(&ev_[1]), "hipEventCreate");
-            }
+            if (!ev_[0])
+                for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
             check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
         }
         launch_gammatone(d_gj, (int)gj.size(), max_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, stream_);
@@ -521,9 +519,19 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             check(hipMemcpyAsync(d_raw, d_spec, sizeof(double) * spec_total, hipMemcpyDeviceToDevice, stream_), "copy");
         auto* prep_scratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * pj.size());
         launch_prep(upload(pj), (int)pj.size(), nb, max_nf, prep_scratch, stream_);
-        launch_planes(upload(plj), (int)plj.size(), nb, max_width, stream_);
-        double* terms = (double*)arena_.alloc(sizeof(double) * 3 * (size_t)nb * S * max_np * sj.size());
-        launch_sim_table(upload(sj), (int)sj.size(), nb, S, max_np, max_win, terms, stream_);
+        launch_planes(upload(plj), (int)plj.size(), nb, S, max_width, opts_.fast, stream_);
+        double* terms = (double*)arena_.alloc(sizeof(double) * 4 * (size_t)nb * S * max_np * sj.size());
+        const SimJob* d_sj = upload(sj);
+        if (timed) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
+        launch_sim_table(d_sj, (int)sj.size(), nb, S, max_np, max_win, terms, opts_.fast, stream_);
+        if (timed) {
+            check(hipEventRecord(ev_[3], stream_), "hipEventRecord");
+            for (int i = 0; i < n; i++)
+                if (st[i].live)
+                    for (int p = 0; p < st[i].np; p++)
+                        prof_.sim_cells += (double)(st[i].hi[p] - st[i].lo[p] + 1) * nb * S;
+            prof_.sim_launches++;
+        }
         launch_dp(upload(dj), (int)dj.size(), stream_);
     }
     // Readback of the DP choice (and stage outputs on request).
@@ -552,6 +560,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             float ms = 0.f;
             check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
             prof_.gammatone_ms += ms;
+            check(hipEventElapsedTime(&ms, ev_[2], ev_[3]), "hipEventElapsedTime");
+            prof_.sim_ms += ms;
         }
         for (int i = 0; i < n; i++) {
             if (!st[i].live) continue;
@@ -911,6 +921,33 @@ long long Engine::best_lag(const double* a, long long la, const double* b, long 
     readback(p, d_lag, sizeof(long long));
     sync();
     return *p;
+}
+
+void Engine::xcorr(const double* a, long long la, const double* b, long long lb, std::vector<double>& out) {
+    arena_.reset();
+    pinned_.reset();
+    out.clear();
+    if (std::max(la, lb) == 0) return;
+    double* da = (double*)arena_.alloc(sizeof(double) * std::max(1LL, la));
+    double* db = (double*)arena_.alloc(sizeof(double) * std::max(1LL, lb));
+    if (la) check(hipMemcpyAsync(da, a, sizeof(double) * la, hipMemcpyHostToDevice, stream_), "upload");
+    if (lb) check(hipMemcpyAsync(db, b, sizeof(double) * lb, hipMemcpyHostToDevice, stream_), "upload");
+    double* d_s = (double*)arena_.alloc(sizeof(double) * 4);
+    check(hipMemsetAsync(d_s, 0, sizeof(double) * 4, stream_), "memset");
+    std::vector<SumJob> s1{SumJob{da, la, kSum, 0, 0, 0, nullptr, 0}, SumJob{db, lb, kSum, 1, 0, 0, nullptr, 0}};
+    std::vector<SumJob> s2{SumJob{da, la, kNyquist, 2, 0, 0, nullptr, 0}, SumJob{db, lb, kNyquist, 3, 1, 0, nullptr, 0}};
+    sums(s1, d_s, !opts_.fast);
+    sums(s2, d_s, !opts_.fast);
+    const long long nfft = fft_size_for(std::max(la, lb));
+    double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
+    std::vector<LagJob> lj{LagJob{da, db, la, lb, z, nfft, 0, 1, 2, 3}};
+    LagJob* d_lj = upload(lj);
+    launch_lag_search(d_lj, 1, nfft, twiddles(nfft), d_s, nullptr, nullptr, stream_, true);
+    double* p = (double*)pinned_.alloc(sizeof(double2) * nfft);
+    readback(p, z, sizeof(double2) * nfft);
+    sync();
+    out.resize(nfft);
+    for (long long i = 0; i < nfft; i++) out[i] = p[2 * i];
 }
 
 void Engine::fft_probe(double* z, long long n, bool dit) {

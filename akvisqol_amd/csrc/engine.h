@@ -108,6 +108,9 @@ class Engine {
     // Stage access for checks.
     void spectrogram(uint32_t sample_rate, const double* host_sig, long long len, std::vector<double>& out, int& nf);
     long long best_lag(const double* a, long long la, const double* b, long long lb);
+    // Re of the circular cross-correlation of the two signals' envelopes
+    // (the values best_lag maximises), fft_size_for(max(la, lb)) of them.
+    void xcorr(const double* a, long long la, const double* b, long long lb, std::vector<double>& out);
     void math_probe(int op, const double* a, const double* b, double* out, long long n);
     void fft_probe(double* z, long long n, bool dit);
     const RateParams& rate(uint32_t sample_rate);
@@ -119,6 +122,12 @@ class Engine {
         double gammatone_flop = 0;  // 30 per (sample, band) of every frame
         double gammatone_bytes = 0; // signal samples read + spectrogram written
         long long launches = 0;
+        // Similarity-table launches (reference-term prep + table), same way:
+        // device time and cells evaluated (patch offsets × bands × patch
+        // frames).
+        double sim_ms = 0;
+        double sim_cells = 0;
+        long long sim_launches = 0;
         // Host wall time of each stretch of a batch call: phase A up to its
         // readback, host planning, phase B up to the DP readback, the match
         // statistics, their host decoding, phase C's lag searches, phase C's
@@ -135,7 +144,7 @@ class Engine {
     std::chrono::steady_clock::time_point lap_t_;
     void lap(int phase);
     Profile prof_;
-    hipEvent_t ev_[2] = {nullptr, nullptr};
+    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
     struct DevRate {
         RateParams p;
         BandCoef* coef = nullptr;

@@ -61,10 +61,11 @@ struct LagJob {
 };
 // Lag search for njobs jobs of one size n (contiguous in d_jobs): lags[j] =
 // first argmax of |Re xcorr| in the order -max_lag..-1, 0..max_lag.
-// d_partials holds 2 * njobs * lag_partials_per_job(n) words.
+// d_partials holds 2 * njobs * lag_partials_per_job(n) words.  keep_xcorr:
+// leave the circular cross-correlation in each job's z instead (no lags).
 long long lag_partials_per_job(long long n);
 void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, const double* d_sums,
-                       unsigned long long* d_partials, long long* d_lags, hipStream_t stream);
+                       unsigned long long* d_partials, long long* d_lags, hipStream_t stream, bool keep_xcorr = false);
 // Radix-2 DIF forward / DIT inverse of each job's z (tw = W_n^k, k < n/2).
 void launch_fft_dif(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, hipStream_t stream);
 void launch_fft_dit(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, hipStream_t stream);
@@ -121,12 +122,27 @@ void launch_prep(const PrepJob* d_jobs, int njobs, int nb, int max_nf, unsigned 
                  hipStream_t stream);
 
 // ---------------------------------------------------------------- NSIM
+// Planes of [nb][width] doubles per degraded spectrogram: the zero-padded
+// values and their squares, then the 3x3 window means and variances for the
+// interior column set and for a patch's first and last columns (variances
+// as signed square roots in the fast mode).
+enum : int {
+    kPlanePad = 0,
+    kPlaneSq,
+    kPlaneMu,
+    kPlaneMuFirst,
+    kPlaneMuLast,
+    kPlaneVar,
+    kPlaneVarFirst,
+    kPlaneVarLast,
+    kPlanes
+};
 struct PlaneJob {
     const double* spec;  // [nb][nf]
-    double* plane;       // 5 planes of [nb][width]: pad, sq, mu, dms, sds
+    double* plane;       // kPlanes planes of [nb][width]
     int nf, width;
 };
-void launch_planes(const PlaneJob* d_jobs, int njobs, int nb, int max_width, hipStream_t stream);
+void launch_planes(const PlaneJob* d_jobs, int njobs, int nb, int patch, int max_width, bool fast, hipStream_t stream);
 
 struct SimJob {
     const double* ref_spec;  // [nb][nf_ref], comparison-ready dB
@@ -138,9 +154,10 @@ struct SimJob {
     int np, nf_ref, nf_deg, width;
 };
 // sims[p][off] for off in [lo[p], hi[p]] (accel.zig:126-146); d_terms holds
-// 3 * nb * patch * max_np * njobs doubles of ref-patch terms.
+// 4 * nb * patch * max_np * njobs doubles of ref-patch terms.  `fast`: fused
+// multiply-adds and one division per cell instead of the reference's order.
 void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int max_np, int max_window, double* d_terms,
-                      hipStream_t stream);
+                      bool fast, hipStream_t stream);
 
 struct DpJob {
     const double* sims;

@@ -7,6 +7,8 @@
 // accel.zig:126-146 (table over the search windows).
 #include <float.h>
 
+#include <type_traits>
+
 #include "dev_common.h"
 #include "kernels.h"
 
@@ -91,8 +93,8 @@ void launch_prep(const PrepJob* d_jobs, int njobs, int nb, int max_nf, unsigned 
 }
 
 // ---------------------------------------------------------------- deg plane
-// nsim.zig:236-263: the deg spectrogram zero padded to `width` columns, its
-// square, and their replicate-padded 3x3 convolutions.
+// nsim.zig:236-263: the deg spectrogram zero padded to `width` columns and
+// its square (planes kPlanePad, kPlaneSq).
 __global__ __launch_bounds__(256) void k_plane_fill(const PlaneJob* __restrict__ jobs, int nb) {
     const PlaneJob j = jobs[blockIdx.y];
     const long long n = (long long)nb * j.width;
@@ -104,7 +106,17 @@ __global__ __launch_bounds__(256) void k_plane_fill(const PlaneJob* __restrict__
     j.plane[n + e] = v * v;
 }
 
-__global__ __launch_bounds__(256) void k_plane_conv(const PlaneJob* __restrict__ jobs, int nb) {
+// Signed square root: the sign of v with the root of |v|, so that the
+// product of two of them is ±sqrt(|product|) with the product's sign.
+__device__ inline double signed_sqrt(double v) { return v < 0.0 ? -sqrt(-v) : sqrt(v); }
+
+// Means and variances of the 3x3 window at every column u, for the three
+// column sets a patch cell can see: interior (u-1, u, u+1), a patch's first
+// column (u, u, u+1) and its last (u-1, u, u) — the patch's own replicate
+// padding (nsim.zig:380-395); rows replicate at the spectrogram edges.
+// Variance planes hold conv(x²) - mean², or its signed square root (fast).
+template <bool FAST>
+__global__ __launch_bounds__(256) void k_plane_conv(const PlaneJob* __restrict__ jobs, int nb, int S) {
     const PlaneJob j = jobs[blockIdx.y];
     const long long n = (long long)nb * j.width;
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -112,29 +124,34 @@ __global__ __launch_bounds__(256) void k_plane_conv(const PlaneJob* __restrict__
     const int w = j.width;
     const int r = (int)(e / w), c = (int)(e % w);
     const int rm = r == 0 ? 0 : r - 1, rp = r + 1 >= nb ? nb - 1 : r + 1;
-    int cm, cp;
-    if (w == 1) { cm = 0; cp = 0; }
-    else { cm = c == 0 ? 0 : c - 1; cp = c + 1 >= w ? w - 1 : c + 1; }
+    const int cl = c == 0 ? 0 : c - 1, cr = c + 1 >= w ? w - 1 : c + 1;
     const double* pad = j.plane;
     const double* sq = j.plane + n;
-    const double mu = tap9([&](int rr, int cc) { return pad[(long long)rr * w + cc]; }, rm, r, rp, cm, c, cp);
-    const double csq = tap9([&](int rr, int cc) { return sq[(long long)rr * w + cc]; }, rm, r, rp, cm, c, cp);
-    const double dms = mu * mu;
-    j.plane[2 * n + e] = mu;
-    j.plane[3 * n + e] = dms;
-    j.plane[4 * n + e] = csq - dms;
+    auto put = [&](int plane_mu, int plane_v, int cm, int cp) {
+        const double mu = tap9([&](int rr, int cc) { return pad[(long long)rr * w + cc]; }, rm, r, rp, cm, c, cp);
+        const double csq = tap9([&](int rr, int cc) { return sq[(long long)rr * w + cc]; }, rm, r, rp, cm, c, cp);
+        const double v = csq - mu * mu;
+        j.plane[plane_mu * n + e] = mu;
+        j.plane[plane_v * n + e] = FAST ? signed_sqrt(v) : v;
+    };
+    put(kPlaneMu, kPlaneVar, cl, cr);
+    put(kPlaneMuFirst, kPlaneVarFirst, c, S > 1 ? cr : c);
+    put(kPlaneMuLast, kPlaneVarLast, S > 1 ? cl : c, c);
 }
 
-void launch_planes(const PlaneJob* d_jobs, int njobs, int nb, int max_width, hipStream_t stream) {
+void launch_planes(const PlaneJob* d_jobs, int njobs, int nb, int patch, int max_width, bool fast, hipStream_t stream) {
     if (njobs == 0) return;
     const dim3 g((unsigned)(((long long)nb * max_width + 255) / 256), njobs);
     hipLaunchKernelGGL(k_plane_fill, g, dim3(256), 0, stream, d_jobs, nb);
-    hipLaunchKernelGGL(k_plane_conv, g, dim3(256), 0, stream, d_jobs, nb);
+    if (fast) hipLaunchKernelGGL(k_plane_conv<true>, g, dim3(256), 0, stream, d_jobs, nb, patch);
+    else hipLaunchKernelGGL(k_plane_conv<false>, g, dim3(256), 0, stream, d_jobs, nb, patch);
 }
 
 // ---------------------------------------------------------------- similarity table
-// Ref-patch terms (nsim.zig:318-329): mu_ref, mu_ref², conv(ref²) - mu_ref²,
-// with the patch's own replicate padding; terms[(p*3 + t)*nb*S + r*S + c].
+// Ref-patch terms (nsim.zig:318-329): mu_ref, mu_ref², conv(ref²) - mu_ref²
+// (fast: its signed square root), with the patch's own replicate padding,
+// and the patch's values; terms[(p*4 + t)*nb*S + r*S + c].
+template <bool FAST>
 __global__ __launch_bounds__(256) void k_ref_terms(const SimJob* __restrict__ jobs, int nb, int S, double* __restrict__ terms,
                                                    int max_np) {
     const SimJob j = jobs[blockIdx.z];
@@ -150,60 +167,168 @@ __global__ __launch_bounds__(256) void k_ref_terms(const SimJob* __restrict__ jo
     const double mu = tap9([&](int rr, int cc) { return ref[rr * st + cc]; }, rm, r, rp, cm, c, cp);
     const double crs = tap9([&](int rr, int cc) { const double v = ref[rr * st + cc]; return v * v; }, rm, r, rp, cm, c, cp);
     const double rms = mu * mu;
-    double* t = terms + ((long long)blockIdx.z * max_np + p) * 3 * nb * S;
+    double* t = terms + ((long long)blockIdx.z * max_np + p) * 4 * nb * S;
     t[e] = mu;
     t[nb * S + e] = rms;
-    t[2 * nb * S + e] = crs - rms;
+    t[2 * nb * S + e] = FAST ? signed_sqrt(crs - rms) : crs - rms;
+    t[3 * nb * S + e] = ref[r * st + c];
 }
 
-// measureSimilarityAt (nsim.zig:364-424) for one (patch, offset) per lane:
-// cells in row-major order, summed serially as the reference sums them.
+// One similarity cell in the throughput mode: simCell's value (nsim.zig:
+// 293-308) from the signed roots of the two variances (their product is
+// ±sqrt(|srs·sds|)), with fused multiply-adds and one reciprocal refined by
+// two Newton steps in place of the two divisions.
+__device__ inline double sim_cell_fast(double mr, double rms, double ssr, double md, double ssd, double crd) {
+    const double c1 = nsim_c1(), c3 = nsim_c3();
+    const double mrd = mr * md;
+    const double num = __builtin_fma(2.0, mrd, c1) * ((crd - mrd) + c3);
+    const double t = ssr * ssd;
+    const double den = (t < 0.0 ? 0.0 : t) + c3;
+    const double d = __builtin_fma(md, md, rms + c1) * den;
+    double q = __builtin_amdgcn_rcp(d);
+    double err = __builtin_fma(-d, q, 1.0);
+    q = __builtin_fma(q, err, q);
+    err = __builtin_fma(-d, q, 1.0);
+    q = __builtin_fma(q, err, q);
+    return num * q;
+}
+
+constexpr int kSimThreads = 256;
+
+// measureSimilarityAt (nsim.zig:364-424) for one (patch, offset) per lane,
+// lanes on consecutive offsets.  The patch's terms and values sit in LDS
+// (read by broadcast).  Along a row the 3x3 taps slide one column per cell:
+// the products X = ref·deg of rows rm, r, rp at columns c-1, c, c+1
+// (patch-clamped) stay in registers, so each cell loads three degraded
+// values plus its window's mean and variance (from the plane that matches
+// its column: interior, a patch's first or its last).  Exact: the
+// reference's tap order and row-major serial sum, every product rounded on
+// its own.  FAST: the same cells with fused multiply-adds.
 template <bool FAST>
-__global__ __launch_bounds__(64) void k_sim_table(const SimJob* __restrict__ jobs, int nb, int S,
-                                                  const double* __restrict__ terms, int max_np) {
+__global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restrict__ jobs, int nb, int S,
+                                                           const double* __restrict__ terms, int max_np) {
+    extern __shared__ double sh[];  // [4][nb*S]: mu_ref, mu_ref², sigma_ref² (or root), ref
     const SimJob j = jobs[blockIdx.z];
     const int p = blockIdx.y;
     if (p >= j.np) return;
     const int lo = j.lo[p], hi = j.hi[p];
-    const int off = lo + blockIdx.x * 64 + threadIdx.x;
-    if (lo + (int)blockIdx.x * 64 > hi) return;
+    const int first = lo + (int)blockIdx.x * kSimThreads;
+    if (first > hi) return;  // uniform
+    const int cells = nb * S;
+    const double* t = terms + ((long long)blockIdx.z * max_np + p) * 4 * cells;
+    for (int e = threadIdx.x; e < 4 * cells; e += kSimThreads) sh[e] = t[e];
+    __syncthreads();
+    const double* s_mr = sh;
+    const double* s_rms = sh + cells;
+    const double* s_srs = sh + 2 * cells;
+    const double* s_ref = sh + 3 * cells;
+
+    const int off = first + (int)threadIdx.x;
     const bool active = off <= hi;
+    const int o = active ? off : lo;  // idle lanes shadow a valid column
     const int w = j.width;
     const long long n = (long long)nb * w;
+    // Loads are (uniform column pointer)[per-lane 32-bit index]: plane sizes
+    // stay far below 2^29 elements.
     const double* pad = j.plane;
-    const double* sq = j.plane + n;
-    const double* mu = j.plane + 2 * n;
-    const double* dmsp = j.plane + 3 * n;
-    const double* sdsp = j.plane + 4 * n;
-    const double* ref = j.ref_spec + j.ref_idx[p];
-    const long long rst = j.nf_ref;
-    const double* t = terms + ((long long)blockIdx.z * max_np + p) * 3 * nb * S;
-    const int o = active ? off : lo;  // idle lanes shadow a valid column
+    const double* mu = j.plane + kPlaneMu * n;
+    const double* var = j.plane + kPlaneVar * n;
+    const double* mu_first = j.plane + kPlaneMuFirst * n;
+    const double* var_first = j.plane + kPlaneVarFirst * n;
+    const double* mu_last = j.plane + kPlaneMuLast * n;
+    const double* var_last = j.plane + kPlaneVarLast * n;
     double sum = 0.0;
+    enum : int { kFirst, kMid, kLast };
     for (int r = 0; r < nb; r++) {
         const int rm = r == 0 ? 0 : r - 1, rp = r + 1 >= nb ? nb - 1 : r + 1;
-        for (int c = 0; c < S; c++) {
-            const int cm = S == 1 ? 0 : (c == 0 ? 0 : c - 1);
-            const int cp = S == 1 ? 0 : (c + 1 >= S ? S - 1 : c + 1);
-            const double crd = tap9(
-                [&](int rr, int cc) { return ref[rr * rst + cc] * pad[(long long)rr * w + o + cc]; }, rm, r, rp, cm, c, cp);
-            double md, dms, sds;
-            if (c == 0 || c == S - 1) {
-                md = tap9([&](int rr, int cc) { return pad[(long long)rr * w + o + cc]; }, rm, r, rp, cm, c, cp);
-                const double cdsq = tap9([&](int rr, int cc) { return sq[(long long)rr * w + o + cc]; }, rm, r, rp, cm, c, cp);
-                dms = md * md;
-                sds = cdsq - dms;
+        const unsigned q0 = (unsigned)(rm * w + o), q1 = (unsigned)(r * w + o), q2 = (unsigned)(rp * w + o);
+        const double* f0 = s_ref + rm * S;
+        const double* f1 = s_ref + r * S;
+        const double* f2 = s_ref + rp * S;
+        // Three column slots of products; cell c uses slots (a, b, c) as
+        // columns c-1, c, c+1, then refills slot a with column c+2 (the
+        // next cell's c+1): the roles rotate.
+        double X[3][3];
+        auto fill = [&](double (&Xd)[3], int col) {
+            const auto* g = as_global(pad + col);
+            Xd[0] = f0[col] * g[q0];
+            Xd[1] = f1[col] * g[q1];
+            Xd[2] = f2[col] * g[q2];
+        };
+        fill(X[1], 0);
+        X[0][0] = X[1][0];
+        X[0][1] = X[1][1];
+        X[0][2] = X[1][2];
+        fill(X[2], S > 1 ? 1 : 0);
+        auto cell = [&](auto kind, int c, double (&Xa)[3], double (&Xb)[3], double (&Xc)[3]) {
+            constexpr int K = decltype(kind)::value;
+            const double* pm = K == kFirst ? mu_first : (K == kLast ? mu_last : mu);
+            const double* pv = K == kFirst ? var_first : (K == kLast ? var_last : var);
+            const double md = as_global(pm + c)[q1];
+            const double v = as_global(pv + c)[q1];
+            double crd;
+            if (FAST) {
+                crd = VQ_WC * Xa[0];
+                crd = __builtin_fma(VQ_WE, Xb[0], crd);
+                crd = __builtin_fma(VQ_WC, Xc[0], crd);
+                crd = __builtin_fma(VQ_WE, Xa[1], crd);
+                crd = __builtin_fma(VQ_WM, Xb[1], crd);
+                crd = __builtin_fma(VQ_WE, Xc[1], crd);
+                crd = __builtin_fma(VQ_WC, Xa[2], crd);
+                crd = __builtin_fma(VQ_WE, Xb[2], crd);
+                crd = __builtin_fma(VQ_WC, Xc[2], crd);
             } else {
-                const long long q = (long long)r * w + o + c;
-                md = mu[q];
-                dms = dmsp[q];
-                sds = sdsp[q];
+                crd = VQ_WC * Xa[0] + VQ_WE * Xb[0] + VQ_WC * Xc[0] + VQ_WE * Xa[1] + VQ_WM * Xb[1] + VQ_WE * Xc[1] +
+                      VQ_WC * Xa[2] + VQ_WE * Xb[2] + VQ_WC * Xc[2];
             }
             const int e = r * S + c;
-            sum += sim_cell(t[e], t[nb * S + e], t[2 * nb * S + e], md, dms, sds, crd);
+            if (FAST) sum += sim_cell_fast(s_mr[e], s_rms[e], s_srs[e], md, v, crd);
+            else sum += sim_cell(s_mr[e], s_rms[e], s_srs[e], md, md * md, v, crd);
+            if (K != kLast) fill(Xa, c + 2 < S ? c + 2 : S - 1);
+        };
+        using First = std::integral_constant<int, kFirst>;
+        using Mid = std::integral_constant<int, kMid>;
+        using Last = std::integral_constant<int, kLast>;
+        // S == 1: the one cell's window is (0, 0, 0), the first-column plane.
+        cell(First(), 0, X[0], X[1], X[2]);
+        if (S > 1) {
+            int c = 1;
+            for (; c + 3 <= S - 1; c += 3) {
+                cell(Mid(), c, X[1], X[2], X[0]);
+                cell(Mid(), c + 1, X[2], X[0], X[1]);
+                cell(Mid(), c + 2, X[0], X[1], X[2]);
+            }
+            const int rem = S - 1 - c;  // interior cells left: 0, 1 or 2
+            if (rem == 0) {
+                cell(Last(), c, X[1], X[2], X[0]);
+            } else if (rem == 1) {
+                cell(Mid(), c, X[1], X[2], X[0]);
+                cell(Last(), c + 1, X[2], X[0], X[1]);
+            } else {
+                cell(Mid(), c, X[1], X[2], X[0]);
+                cell(Mid(), c + 1, X[2], X[0], X[1]);
+                cell(Last(), c + 2, X[0], X[1], X[2]);
+            }
         }
     }
     if (active) j.sims[(long long)p * j.nf_deg + off] = sum / (double)(nb * S);
+}
+
+void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int max_np, int max_window, double* d_terms,
+                      bool fast, hipStream_t stream) {
+    if (njobs == 0 || max_np == 0 || max_window == 0) return;
+    const dim3 gt((nb * patch + 255) / 256, max_np, njobs);
+    const dim3 grid((max_window + kSimThreads - 1) / kSimThreads, max_np, njobs);
+    const size_t lds = sizeof(double) * 4 * nb * patch;
+    if (fast) {
+        hipLaunchKernelGGL(k_ref_terms<true>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
+        hipLaunchKernelGGL(k_sim_table<true>, grid, dim3(kSimThreads), lds, stream, d_jobs, nb, patch,
+                           (const double*)d_terms, max_np);
+    } else {
+        hipLaunchKernelGGL(k_ref_terms<false>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
+        hipLaunchKernelGGL(k_sim_table<false>, grid, dim3(kSimThreads), lds, stream, d_jobs, nb, patch,
+                           (const double*)d_terms, max_np);
+    }
 }
 
 // ---------------------------------------------------------------- DP
@@ -372,13 +497,5 @@ void launch_stats(const StatJob* d_jobs, int njobs, int nb, int max_cols, hipStr
     hipLaunchKernelGGL(k_stats, dim3(njobs), dim3(64), lds, stream, d_jobs, nb);
 }
 
-void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int max_np, int max_window, double* d_terms,
-                      hipStream_t stream) {
-    if (njobs == 0 || max_np == 0) return;
-    hipLaunchKernelGGL(k_ref_terms, dim3((nb * patch + 255) / 256, max_np, njobs), dim3(256), 0, stream, d_jobs, nb, patch,
-                       d_terms, max_np);
-    hipLaunchKernelGGL(k_sim_table<false>, dim3((max_window + 63) / 64, max_np, njobs), dim3(64), 0, stream, d_jobs, nb,
-                       patch, (const double*)d_terms, max_np);
-}
 
 }  // namespace vq

@@ -209,56 +209,25 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const LagJob* __restrict_
     tile_argmax(sm, count, gidx, n, job, partials + 2 * (jb * nwg + wg));
 }
 
-// Full-tile pass (n >= kTile, so every tile holds kTile elements) with K
-// stages, fully unrolled.  Stages are taken two at a time: a thread holds
-// the four elements m0, m0+h, m0+2h, m0+3h of one residue (h the smaller of
-// the two stage distances, in units of q) and runs both stages'
-// butterflies on them in registers with those stages' twiddles — the same
-// radix-2 operations as the stage-by-stage transform.  An odd K ends with
-// one radix-2 stage.  Twiddles of the next step load while the current one
-// computes.
 template <int K, bool DIT>
 struct TileStep {
     // processing index li -> stage l (distance 2^(K-1-l) in units of q)
     static constexpr int stage(int li) { return DIT ? K - 1 - li : li; }
 };
 
-template <int K, bool DIT, int SRC, bool ARGMAX>
-__global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict__ jobs, int lgn, int lgS, int lgR, int lgBn,
-                                                       const double2* __restrict__ tw, const double* __restrict__ sums,
-                                                       unsigned long long* __restrict__ partials, int njobs) {
-    __shared__ double2 sm[kTile];
-    const int jb = (int)(blockIdx.x % (unsigned)njobs);
-    const long long wg = blockIdx.x / (unsigned)njobs;
-    const long long nwg = gridDim.x / (unsigned)njobs;
-    const LagJob job = jobs[jb];
-    const long long n = 1LL << lgn;
-    const int lgq = lgS - K;
+// The K radix-2 stages of one full tile in LDS (element (bb, m, rr) at
+// (bb << (K + lgR)) + (m << lgR) + rr, m the index along the stages'
+// spans), fully unrolled.  Stages are taken two at a time: a thread holds
+// the four elements m0, m0+h, m0+2h, m0+3h of one residue (h the smaller of
+// the two stage distances, in units of q) and runs both stages'
+// butterflies on them in registers with those stages' twiddles — the same
+// radix-2 operations as the stage-by-stage transform.  An odd K ends with
+// one radix-2 stage.  Twiddles of the next step load while the current one
+// computes.  Twiddle of stage l at position j within its span:
+// tw[j << (sh + l)], j = p·q + r0 + rr.  Starts and ends with a barrier.
+template <int K, bool DIT>
+__device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, int sh, const double2* __restrict__ tw) {
     const int lgGR = K + lgR;
-    const int lgPer = lgq - lgR;
-    const long long blk0 = (wg >> lgPer) << lgBn;
-    const long long r0 = (wg & ((1LL << lgPer) - 1)) << lgR;
-    const long long base = (blk0 << lgS) + r0;
-    const int sh = lgn - lgS;  // twiddle stride of stage l is 2^(sh + l)
-    double2* z = job.z;
-    auto gidx = [&](int e) -> long long {
-        const int bb = e >> lgGR;
-        const int m = (e >> lgR) & ((1 << K) - 1);
-        const int rr = e & ((1 << lgR) - 1);
-        return base + ((long long)bb << lgS) + ((long long)m << lgq) + rr;
-    };
-    constexpr int kPer = kTile / kThreads;
-    {
-        double2 v[kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; u++) {
-            const long long i = gidx(threadIdx.x + u * kThreads);
-            v[u] = SRC == kSrcEnvelope ? packed_env(job, sums, i) : load2(z, i);
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
-    }
-
     // Radix-4 group g (two per thread): residue rr, quarter index mq, block bb.
     // lgH = log2 of the smaller distance.  Element x_c sits at m0 + c·h.
     auto quad = [&](int g, int lgH, int& i0, int& lo, int& rr) {
@@ -368,6 +337,49 @@ __global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict_
                 for (int c = 0; c < 3; c++) w[u][c] = wn[u][c];
         }
     }
+}
+
+// Full-tile pass (n >= kTile, so every tile holds kTile elements) with K
+// stages: load the tile, run tile_stages, store it (or, on the inverse
+// transform's last pass, keep only the tile's first maximum).
+
+template <int K, bool DIT, int SRC, bool ARGMAX>
+__global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict__ jobs, int lgn, int lgS, int lgR, int lgBn,
+                                                       const double2* __restrict__ tw, const double* __restrict__ sums,
+                                                       unsigned long long* __restrict__ partials, int njobs) {
+    __shared__ double2 sm[kTile];
+    const int jb = (int)(blockIdx.x % (unsigned)njobs);
+    const long long wg = blockIdx.x / (unsigned)njobs;
+    const long long nwg = gridDim.x / (unsigned)njobs;
+    const LagJob job = jobs[jb];
+    const long long n = 1LL << lgn;
+    const int lgq = lgS - K;
+    const int lgGR = K + lgR;
+    const int lgPer = lgq - lgR;
+    const long long blk0 = (wg >> lgPer) << lgBn;
+    const long long r0 = (wg & ((1LL << lgPer) - 1)) << lgR;
+    const long long base = (blk0 << lgS) + r0;
+    const int sh = lgn - lgS;  // twiddle stride of stage l is 2^(sh + l)
+    double2* z = job.z;
+    auto gidx = [&](int e) -> long long {
+        const int bb = e >> lgGR;
+        const int m = (e >> lgR) & ((1 << K) - 1);
+        const int rr = e & ((1 << lgR) - 1);
+        return base + ((long long)bb << lgS) + ((long long)m << lgq) + rr;
+    };
+    constexpr int kPer = kTile / kThreads;
+    {
+        double2 v[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const long long i = gidx(threadIdx.x + u * kThreads);
+            v[u] = SRC == kSrcEnvelope ? packed_env(job, sums, i) : load2(z, i);
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
+    }
+
+    tile_stages<K, DIT>(sm, lgR, lgq, r0, sh, tw);
     if (!ARGMAX) {
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
@@ -403,6 +415,67 @@ __global__ __launch_bounds__(256) void k_cross_terms(const LagJob* __restrict__ 
         const double2 za = load2(z, i), zb = load2(z, p);
         store2(z, i, cross_term(za, zb));
         store2(z, p, cross_term(zb, za));
+    }
+}
+
+// Middle of the lag search (n >= kTile): the forward transform's last pass,
+// the cross terms and the inverse transform's first pass in one kernel.
+// Both passes work on aligned contiguous blocks of 2^K elements, and the
+// cross term of bit-reversed position i pairs it with 3h-1-i (h the largest
+// power of two <= i), which maps chunk c of 1024 elements in [h, 2h) onto
+// the whole chunk 2h/1024 - 1 - c; chunks 0 and 1 pair within themselves.
+// Workgroup w holds the chunk pair (0, 1), or for w >= 1 with
+// 2^k <= w < 2^(k+1): chunks 2^(k+1) + t and 2^(k+2) - 1 - t, t = w - 2^k.
+constexpr int kHalfTile = kTile / 2;
+
+template <int K>
+__global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__ jobs, int lgn,
+                                                      const double2* __restrict__ tw, int njobs) {
+    __shared__ double2 sm[kTile];
+    const int jb = (int)(blockIdx.x % (unsigned)njobs);
+    const long long wg = blockIdx.x / (unsigned)njobs;
+    double2* z = jobs[jb].z;
+    long long ca = 0, cb = 1;
+    if (wg > 0) {
+        const int k = 63 - __builtin_clzll((unsigned long long)wg);
+        const long long t = wg - (1LL << k);
+        ca = (2LL << k) + t;
+        cb = (4LL << k) - 1 - t;
+    }
+    auto gidx = [&](int e) -> long long { return (e < kHalfTile ? ca : cb) * kHalfTile + (e & (kHalfTile - 1)); };
+    constexpr int kPer = kTile / kThreads;
+    {
+        double2 v[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) v[u] = load2(z, gidx(threadIdx.x + u * kThreads));
+#pragma unroll
+        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
+    }
+    const int sh = lgn - K;  // spans of 2^K, one residue
+    tile_stages<K, false>(sm, 0, 0, 0, sh, tw);
+    {
+        double2 v[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int e = threadIdx.x + u * kThreads;
+            const long long g = gidx(e);
+            long long pg = g;
+            if (g >= 2) {
+                const long long h = 1LL << (63 - __builtin_clzll((unsigned long long)g));
+                pg = 3 * h - 1 - g;
+            }
+            const int ep = ((pg / kHalfTile) == ca ? 0 : kHalfTile) + (int)(pg & (kHalfTile - 1));
+            v[u] = cross_term(sm[e], sm[ep]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
+    }
+    tile_stages<K, true>(sm, 0, 0, 0, sh, tw);
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const int e = threadIdx.x + u * kThreads;
+        store2(z, gidx(e), sm[e]);
     }
 }
 
@@ -503,29 +576,45 @@ long long lag_partials_per_job(long long n) {
 }
 
 void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, const double* d_sums,
-                       unsigned long long* d_partials, long long* d_lags, hipStream_t stream) {
+                       unsigned long long* d_partials, long long* d_lags, hipStream_t stream, bool keep_xcorr) {
     if (njobs == 0) return;
     if (n < 2) {
         // n == 1: one sample per side, lag 0.
+        if (keep_xcorr) return;
         hipLaunchKernelGGL(k_lag_reduce, dim3(njobs), dim3(256), 0, stream, d_jobs, (const unsigned long long*)d_partials,
                            0, d_lags);
         return;
     }
     int ks[8];
     const int np = plan_passes(n, ks);
+    // Fused middle for full tiles: the last forward pass, the cross terms and
+    // the first inverse pass in one kernel (np >= 2 whenever n >= kTile).
+    const bool fuse = n >= kTile && np >= 2 && ks[np - 1] >= 5 && ks[np - 1] <= 8;
     long long S = n;
-    for (int p = 0; p < np; p++) {
+    for (int p = 0; p < (fuse ? np - 1 : np); p++) {
         if (p == 0) pass<false, kSrcEnvelope, false>(d_jobs, njobs, n, S, ks[p], d_tw, d_sums, nullptr, stream);
         else pass<false, kSrcZ, false>(d_jobs, njobs, n, S, ks[p], d_tw, d_sums, nullptr, stream);
         S >>= ks[p];
     }
-    hipLaunchKernelGGL(k_cross_terms, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, stream, d_jobs, n);
-    S = 1;
-    for (int p = np - 1; p >= 0; p--) {
+    if (fuse) {
+        const dim3 g((unsigned)((n / kTile) * njobs));
+        const int lgn = log2i(n);
+        switch (ks[np - 1]) {
+            case 5: hipLaunchKernelGGL(k_fft_mid<5>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
+            case 6: hipLaunchKernelGGL(k_fft_mid<6>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
+            case 7: hipLaunchKernelGGL(k_fft_mid<7>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
+            default: hipLaunchKernelGGL(k_fft_mid<8>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
+        }
+    } else {
+        hipLaunchKernelGGL(k_cross_terms, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, stream, d_jobs, n);
+    }
+    S = fuse ? (1LL << ks[np - 1]) : 1;
+    for (int p = fuse ? np - 2 : np - 1; p >= 0; p--) {
         S <<= ks[p];
-        if (S == n) pass<true, kSrcZ, true>(d_jobs, njobs, n, S, ks[p], d_tw, d_sums, d_partials, stream);
+        if (S == n && !keep_xcorr) pass<true, kSrcZ, true>(d_jobs, njobs, n, S, ks[p], d_tw, d_sums, d_partials, stream);
         else pass<true, kSrcZ, false>(d_jobs, njobs, n, S, ks[p], d_tw, d_sums, nullptr, stream);
     }
+    if (keep_xcorr) return;
     const long long parts = pass_shape(n, n, ks[0]).wgs;
     hipLaunchKernelGGL(k_lag_reduce, dim3(njobs), dim3(256), 0, stream, d_jobs, (const unsigned long long*)d_partials,
                        (int)parts, d_lags);

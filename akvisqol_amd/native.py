@@ -123,6 +123,7 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_profile": (c_int, [c_void_p, c_int, c_int]),
         "visqol_hip_profile_read": (c_int, [c_void_p, D, D, D, POINTER(c_int64)]),
         "visqol_hip_profile_phases": (c_int, [c_void_p, D, c_int]),
+        "visqol_hip_profile_sim": (c_int, [c_void_p, D, D, POINTER(c_int64)]),
         "visqol_hip_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
         "visqol_hip_free": (c_int, [c_void_p]),
         "visqol_hip_upload": (c_int, [c_void_p, c_void_p, c_size_t]),
@@ -131,6 +132,7 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_best_lag": (c_int, [D, c_size_t, D, c_size_t, POINTER(c_int64)]),
         "visqol_hip_math_probe": (c_int, [c_int, D, D, D, c_size_t]),
         "visqol_hip_fft": (c_int, [D, c_size_t, c_int]),
+        "visqol_hip_xcorr": (c_int, [D, c_size_t, D, c_size_t, D, c_size_t, POINTER(c_size_t)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -296,6 +298,13 @@ class Context:
                                            ctypes.byref(n)))
         return dict(ms=ms.value, flop=fl.value, bytes=by.value, launches=n.value)
 
+    def profile_sim(self) -> dict:
+        """Similarity-table launches since the last reset: device ms, cells, launches."""
+        ms, cells = c_double(), c_double()
+        n = c_int64()
+        _check(lib.visqol_hip_profile_sim(self._h, ctypes.byref(ms), ctypes.byref(cells), ctypes.byref(n)))
+        return dict(ms=ms.value, cells=cells.value, launches=n.value)
+
     PHASES = ("phase_a", "plan", "phase_b", "match_stats", "match_decode", "realign_lags", "realign_stats", "aggregate")
 
     def profile_phases(self) -> dict:
@@ -398,6 +407,19 @@ def best_lag(a, b) -> int:
     lag = c_int64()
     _check(lib.visqol_hip_best_lag(_dptr(a), a.size, _dptr(b), b.size, ctypes.byref(lag)))
     return lag.value
+
+
+def xcorr(a, b) -> np.ndarray:
+    """Re of the circular cross-correlation of the upper envelopes (device)."""
+    a, b = _f64(a), _f64(b)
+    biggest = max(a.size, b.size)
+    n = 1
+    while n < max(1, 2 * biggest - 1):
+        n <<= 1
+    out = np.zeros(n if biggest else 0)
+    got = c_size_t()
+    _check(lib.visqol_hip_xcorr(_dptr(a), a.size, _dptr(b), b.size, _dptr(out), out.size, ctypes.byref(got)))
+    return out[: got.value]
 
 
 def fft(z: np.ndarray, dit: bool = False) -> np.ndarray:

@@ -80,6 +80,10 @@ def main() -> int:
     prof = ctx.profile_read()
     phases = {k: round(v / args.steps, 3) for k, v in ctx.profile_phases().items()}
     print(f"bench: host wall ms per step by stretch {phases}", file=sys.stderr, flush=True)
+    simp = ctx.profile_sim()
+    if simp["launches"]:
+        print(f"bench: similarity table {simp['ms'] / simp['launches']:.3f} ms/launch, "
+              f"{simp['cells'] / simp['launches']:.4g} cells/launch", file=sys.stderr, flush=True)
     ctx.profile(False, True)
     step_s = float(np.mean(times))
     value = args.batch / step_s

@@ -55,6 +55,9 @@ int visqol_hip_measure_batch_device(visqol_ctx *ctx, size_t n, const double *con
  * the launch count since the last reset. */
 int visqol_hip_profile(visqol_ctx *ctx, int enable, int reset);
 int visqol_hip_profile_read(visqol_ctx *ctx, double *ms, double *flop, double *bytes, int64_t *launches);
+/* The same for the similarity-table launches: *ms device time, *cells the
+ * cells evaluated (search offsets x bands x patch frames), *launches. */
+int visqol_hip_profile_sim(visqol_ctx *ctx, double *ms, double *cells, int64_t *launches);
 /* Host wall time (ms) of the stretches of batch calls since the last reset,
  * writing min(n, 8) values: phase A to its readback, host planning, phase B
  * to the DP readback, match statistics, their decoding, realignment lag
@@ -78,6 +81,11 @@ int visqol_hip_spectrogram(uint32_t mode, uint32_t sample_rate, uint32_t precisi
 int visqol_hip_filters(uint32_t mode, uint32_t sample_rate, double *bands8, double *center_freqs);
 /* FFT cross-correlation lag of the upper envelopes of a and b. */
 int visqol_hip_best_lag(const double *a, size_t la, const double *b, size_t lb, int64_t *lag);
+/* The values visqol_hip_best_lag maximises: Re of the circular
+ * cross-correlation of the two upper envelopes, n = the power of two
+ * >= 2·max(la, lb) - 1 (max(la, lb) >= 2).  Writes min(n, cap) values to
+ * out and stores n in *n_out. */
+int visqol_hip_xcorr(const double *a, size_t la, const double *b, size_t lb, double *out, size_t cap, size_t *n_out);
 /* The device radix-2 transform in place on n interleaved complex values
  * (n a power of two): dit = 0 runs the DIF forward transform (natural in,
  * bit-reversed out), dit = 1 the DIT inverse (bit-reversed in, natural out),

@@ -109,6 +109,9 @@ void vo_fft_dif_forward(double *z_interleaved, size_t n);
 void vo_fft_dit_inverse(double *z_interleaved, size_t n);
 void vo_fft_natural(double *z_interleaved, size_t n, int inverse);
 int64_t vo_best_lag(const double *s1, size_t l1, const double *s2, size_t l2);
+/* Re of the circular cross-correlation behind vo_best_lag: n =
+ * vo_fft_size_for(max(l1, l2)) values written to out. */
+int vo_xcorr_re(const double *s1, size_t l1, const double *s2, size_t l2, double *out);
 int64_t vo_best_lag_reference(const double *s1, size_t l1, const double *s2, size_t l2);
 int64_t vo_best_lag_direct(const double *s1, size_t l1, const double *s2, size_t l2);
 double vo_spl(const double *x, size_t n);

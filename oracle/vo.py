@@ -124,6 +124,7 @@ def lib() -> ctypes.CDLL:
             "vo_fft_dit_inverse": (None, [D, c_size_t]),
             "vo_fft_natural": (None, [D, c_size_t, c_int]),
             "vo_best_lag": (c_int64, [D, c_size_t, D, c_size_t]),
+            "vo_xcorr_re": (c_int, [D, c_size_t, D, c_size_t, D]),
             "vo_best_lag_reference": (c_int64, [D, c_size_t, D, c_size_t]),
             "vo_best_lag_direct": (c_int64, [D, c_size_t, D, c_size_t]),
             "vo_spl": (c_double, [D, c_size_t]),

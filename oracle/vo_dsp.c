@@ -457,15 +457,18 @@ static inline cpx cross_term(cpx x, cpx zy) {
     return (cpx){0.25 * (sum.re * idc.re - sum.im * idc.im), 0.25 * (sum.re * idc.im + sum.im * idc.re)};
 }
 
-/* xcorr.zig:197-251 (bestLagCached): argmax of |Σ_t s1[t+lag]·s2[t]|,
- * first maximum scanning lags -max..-1 then 0..max. */
-int64_t vo_best_lag(const double *s1, size_t l1, const double *s2, size_t l2) {
-    size_t biggest = l1 > l2 ? l1 : l2;
-    if (biggest == 0) return 0;
-    size_t n = vo_fft_size_for(biggest);
+/* xcorr.zig:197-229: the circular cross-correlation of s1 and s2 as the
+ * real part of z after the DIF forward transform of s1 + i·s2 (zero padded
+ * to n), the cross terms in bit-reversed order and the DIT inverse.
+ * Returns z (n elements, caller frees) or NULL. */
+static cpx *xcorr_z(const double *s1, size_t l1, const double *s2, size_t l2, size_t n) {
     fft_plan p;
-    if (plan_init(&p, n)) return 0;
+    if (plan_init(&p, n)) return NULL;
     cpx *z = (cpx *)calloc(n, sizeof(cpx));
+    if (!z) {
+        plan_free(&p);
+        return NULL;
+    }
     size_t m = l1 < l2 ? l1 : l2;
     for (size_t i = 0; i < m; i++) z[i] = (cpx){s1[i], s2[i]};
     for (size_t i = m; i < l1; i++) z[i] = (cpx){s1[i], 0.0};
@@ -484,6 +487,29 @@ int64_t vo_best_lag(const double *s1, size_t l1, const double *s2, size_t l2) {
         }
     }
     dit_inverse(&p, z);
+    plan_free(&p);
+    return z;
+}
+
+int vo_xcorr_re(const double *s1, size_t l1, const double *s2, size_t l2, double *out) {
+    size_t biggest = l1 > l2 ? l1 : l2;
+    if (biggest == 0) return 0;
+    size_t n = vo_fft_size_for(biggest);
+    cpx *z = xcorr_z(s1, l1, s2, l2, n);
+    if (!z) return -1;
+    for (size_t i = 0; i < n; i++) out[i] = z[i].re;
+    free(z);
+    return 0;
+}
+
+/* xcorr.zig:230-251 (bestLagCached): argmax of |Σ_t s1[t+lag]·s2[t]|,
+ * first maximum scanning lags -max..-1 then 0..max. */
+int64_t vo_best_lag(const double *s1, size_t l1, const double *s2, size_t l2) {
+    size_t biggest = l1 > l2 ? l1 : l2;
+    if (biggest == 0) return 0;
+    size_t n = vo_fft_size_for(biggest);
+    cpx *z = xcorr_z(s1, l1, s2, l2, n);
+    if (!z) return 0;
     size_t max_lag = biggest - 1;
     double best_abs = -INFINITY;
     int64_t best_lag = 0;
@@ -496,7 +522,6 @@ int64_t vo_best_lag(const double *s1, size_t l1, const double *s2, size_t l2) {
         if (a > best_abs) { best_abs = a; best_lag = (int64_t)k; }
     }
     free(z);
-    plan_free(&p);
     return best_lag;
 }
 

@@ -121,6 +121,28 @@ def test_best_lag_matches_oracle(native, vo):
         assert native.best_lag(a, b) == want
 
 
+@pytest.mark.parametrize("lg", [2, 5, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20])
+def test_xcorr_bit_identical(native, vo, lg):
+    """Every value of the device cross-correlation (envelopes built in the
+    first forward pass, fused middle pass, inverse passes) equals the
+    oracle's, for transform sizes 2^lg covering each pass plan."""
+    rng = np.random.default_rng(100 + lg)
+    la = max(2, (1 << (lg - 1)) - 3)
+    lb = max(1, la - 17)
+    say(f"xcorr n=2^{lg} la={la} lb={lb}")
+    a = rng.uniform(-1, 1, la)
+    b = np.roll(np.resize(a, lb), 11) + 0.05 * rng.standard_normal(lb)
+    ea, eb = np.zeros(la), np.zeros(lb)
+    vo.lib().vo_upper_envelope(vo.dptr(vo.f64(a)), la, vo.dptr(ea))
+    vo.lib().vo_upper_envelope(vo.dptr(vo.f64(b)), lb, vo.dptr(eb))
+    n = vo.lib().vo_fft_size_for(max(la, lb))
+    want = np.zeros(n)
+    assert vo.lib().vo_xcorr_re(vo.dptr(ea), la, vo.dptr(eb), lb, vo.dptr(want)) == 0
+    got = native.xcorr(a, b)
+    assert got.size == n
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
 # ---------------------------------------------------------------- whole pipeline
 def _compare(res, tr, ores, otr, exact=True, tol=EXACT_MOS_TOL):
     assert tr["lag"] == otr["lag"]
