@@ -165,7 +165,7 @@ Engine::DevRate& Engine::dev_rate(uint32_t sr) {
 const double2* Engine::twiddles(long long n) {
     auto it = twiddles_.find(n);
     if (it != twiddles_.end()) return it->second;
-    const std::vector<double> tw = make_twiddles(n);
+    const std::vector<double> tw = make_level_twiddles(n);
     double2* d = nullptr;
     check(hipMalloc(&d, sizeof(double) * tw.size()), "hipMalloc twiddles");
     check(hipMemcpy(d, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice), "upload twiddles");

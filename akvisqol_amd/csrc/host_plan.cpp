@@ -129,6 +129,21 @@ std::vector<double> make_twiddles(long long n) {
     return tw;
 }
 
+// Twiddles of every radix-2 span S = 2, 4, ..., n in one table: W_S^j at
+// complex index S/2 + j (j < S/2), copied from the n-point table (W_S^j =
+// W_n^{j·n/S}, the same value), so that a stage's twiddles for consecutive j
+// are consecutive in memory.  Index 0 is unused.
+std::vector<double> make_level_twiddles(long long n) {
+    const std::vector<double> base = make_twiddles(n);
+    std::vector<double> t(2 * std::max(2LL, n), 0.0);
+    for (long long S = 2; S <= n; S <<= 1)
+        for (long long j = 0; j < S / 2; j++) {
+            t[2 * (S / 2 + j)] = base[2 * (j * (n / S))];
+            t[2 * (S / 2 + j) + 1] = base[2 * (j * (n / S)) + 1];
+        }
+    return t;
+}
+
 long long fft_size_for(long long biggest) {
     const long long need = biggest == 0 ? 1 : 2 * biggest - 1;
     long long n = 1;

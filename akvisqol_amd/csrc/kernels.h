@@ -66,7 +66,9 @@ struct LagJob {
 long long lag_partials_per_job(long long n);
 void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, const double* d_sums,
                        unsigned long long* d_partials, long long* d_lags, hipStream_t stream, bool keep_xcorr = false);
-// Radix-2 DIF forward / DIT inverse of each job's z (tw = W_n^k, k < n/2).
+// Twiddle tables (d_tw) hold W_S^j for every span S = 2..n at complex index
+// S/2 + j.
+// Radix-2 DIF forward / DIT inverse of each job's z.
 void launch_fft_dif(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, hipStream_t stream);
 void launch_fft_dit(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, hipStream_t stream);
 

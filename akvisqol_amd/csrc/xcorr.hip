@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const LagJob* __restrict_
     }
     // Butterfly u of this thread at level li: (i0, i1) in LDS and its twiddle
     // index.  Stage l has distance hm·q (hm = 2^(k-1-l)) and twiddle
-    // W_{S_hi/2^l}^j = tw[j · n/(S_hi/2^l)].
+    // W_{S_l}^j (S_l = S_hi/2^l) at tw[S_l/2 + j].
     const int lgHalf = lgGR - 1;
     auto butterfly = [&](int li, int u, int& i0, int& i1) -> long long {
         const int l = DIT ? k - 1 - li : li;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const LagJob* __restrict_
         i0 = (bb << lgGR) + (m0 << lgR) + rr;
         i1 = i0 + (1 << (lgHm + lgR));
         const long long j = ((long long)lo << lgq) + r0 + rr;
-        return j << (lgn - lgS + l);
+        return (1LL << (lgS - l - 1)) + j;
     };
     const int nbf = count >> 1;  // == kTile / 2 whenever the tile is full
     double2 w[kBfPerThread];
@@ -223,10 +223,10 @@ struct TileStep {
 // butterflies on them in registers with those stages' twiddles — the same
 // radix-2 operations as the stage-by-stage transform.  An odd K ends with
 // one radix-2 stage.  Twiddles of the next step load while the current one
-// computes.  Twiddle of stage l at position j within its span:
-// tw[j << (sh + l)], j = p·q + r0 + rr.  Starts and ends with a barrier.
+// computes.  Twiddle of stage l (span S_l = 2^(lgq + K - l)) at position
+// j = p·q + r0 + rr: tw[S_l/2 + j].  Starts and ends with a barrier.
 template <int K, bool DIT>
-__device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, int sh, const double2* __restrict__ tw) {
+__device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, const double2* __restrict__ tw) {
     const int lgGR = K + lgR;
     // Radix-4 group g (two per thread): residue rr, quarter index mq, block bb.
     // lgH = log2 of the smaller distance.  Element x_c sits at m0 + c·h.
@@ -241,7 +241,7 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
     // Twiddle of stage l at position p (< distance) for residue r0 + rr.
     auto twi = [&](int l, int p, int rr) -> double2 {
         const long long j = ((long long)p << lgq) + r0 + rr;
-        return tw[j << (sh + l)];
+        return tw[(1LL << (lgq + K - l - 1)) + j];
     };
     constexpr int kSteps = (K + 1) / 2;
     // w[u][0..1]: the larger-distance stage at lo and lo + h; w[u][2]: the
@@ -359,7 +359,6 @@ __global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict_
     const long long blk0 = (wg >> lgPer) << lgBn;
     const long long r0 = (wg & ((1LL << lgPer) - 1)) << lgR;
     const long long base = (blk0 << lgS) + r0;
-    const int sh = lgn - lgS;  // twiddle stride of stage l is 2^(sh + l)
     double2* z = job.z;
     auto gidx = [&](int e) -> long long {
         const int bb = e >> lgGR;
@@ -379,7 +378,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict_
         for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
     }
 
-    tile_stages<K, DIT>(sm, lgR, lgq, r0, sh, tw);
+    tile_stages<K, DIT>(sm, lgR, lgq, r0, tw);
     if (!ARGMAX) {
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
@@ -451,8 +450,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__
 #pragma unroll
         for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
     }
-    const int sh = lgn - K;  // spans of 2^K, one residue
-    tile_stages<K, false>(sm, 0, 0, 0, sh, tw);
+    tile_stages<K, false>(sm, 0, 0, 0, tw);  // spans of 2^K, one residue
     {
         double2 v[kPer];
 #pragma unroll
@@ -471,7 +469,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__
 #pragma unroll
         for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
     }
-    tile_stages<K, true>(sm, 0, 0, 0, sh, tw);
+    tile_stages<K, true>(sm, 0, 0, 0, tw);
 #pragma unroll
     for (int u = 0; u < kPer; u++) {
         const int e = threadIdx.x + u * kThreads;
@@ -511,16 +509,31 @@ __global__ __launch_bounds__(256) void k_lag_reduce(const LagJob* __restrict__ j
 }
 
 namespace {
-constexpr int kMaxFuse = 8;
+constexpr int kMaxFuse = 8;     // stages per pass below one full tile
+constexpr int kMaxLast = 10;    // stages of the contiguous last pass (2^10 <= kTile / 2)
+constexpr int kMinStrided = 4;  // strided passes: 4..6 stages, runs of 2 KiB..512 B
+constexpr int kMaxStrided = 6;
 
-// Stage counts per pass, as even as possible, the smaller counts first: the
-// pass over the largest spans then reads the longest contiguous runs.
+// Stage counts per pass, the pass over the largest spans first.  Below one
+// full tile: as even as possible.  With full tiles the last pass works on
+// contiguous blocks and takes up to kMaxLast stages; the strided passes
+// before it take 4..6 stages (as even as possible, smaller first), so each
+// workgroup reads runs of at least 32 residues (512 B).
 int plan_passes(long long n, int* ks) {
     int L = 0;
     while ((1LL << L) < n) L++;
-    const int np = (L + kMaxFuse - 1) / kMaxFuse;
-    for (int p = 0; p < np; p++) ks[p] = L / np + (p >= np - L % np ? 1 : 0);
-    return np;
+    if (n < kTile) {
+        const int np = (L + kMaxFuse - 1) / kMaxFuse;
+        for (int p = 0; p < np; p++) ks[p] = L / np + (p >= np - L % np ? 1 : 0);
+        return np;
+    }
+    int ns = (L - kMaxLast + kMaxStrided - 1) / kMaxStrided;
+    if (ns < 1) ns = 1;
+    const int last = L - kMinStrided * ns < kMaxLast ? L - kMinStrided * ns : kMaxLast;
+    const int T = L - last;
+    for (int p = 0; p < ns; p++) ks[p] = T / ns + (p >= ns - T % ns ? 1 : 0);
+    ks[ns] = last;
+    return ns + 1;
 }
 
 int log2i(long long v) {
@@ -555,12 +568,15 @@ template <bool DIT, int SRC, bool ARGMAX>
 void pass(const LagJob* d_jobs, int njobs, long long n, long long S_hi, int k, const double2* tw, const double* sums,
           unsigned long long* partials, hipStream_t stream) {
     const PassShape s = pass_shape(n, S_hi, k);
-    if (n >= kTile && k >= 5 && k <= 8) {  // every tile full
+    if (n >= kTile && k >= kMinStrided && k <= kMaxLast) {  // every tile full
         switch (k) {
+            case 4: tile_pass<4, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
             case 5: tile_pass<5, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
             case 6: tile_pass<6, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
             case 7: tile_pass<7, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
-            default: tile_pass<8, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 8: tile_pass<8, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 9: tile_pass<9, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            default: tile_pass<10, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
         }
     }
     hipLaunchKernelGGL((k_fft_pass<DIT, SRC, ARGMAX>), dim3((unsigned)(s.wgs * njobs)), dim3(kThreads), 0, stream, d_jobs,
@@ -589,7 +605,7 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
     const int np = plan_passes(n, ks);
     // Fused middle for full tiles: the last forward pass, the cross terms and
     // the first inverse pass in one kernel (np >= 2 whenever n >= kTile).
-    const bool fuse = n >= kTile && np >= 2 && ks[np - 1] >= 5 && ks[np - 1] <= 8;
+    const bool fuse = n >= kTile && np >= 2 && ks[np - 1] >= 7 && ks[np - 1] <= kMaxLast;
     long long S = n;
     for (int p = 0; p < (fuse ? np - 1 : np); p++) {
         if (p == 0) pass<false, kSrcEnvelope, false>(d_jobs, njobs, n, S, ks[p], d_tw, d_sums, nullptr, stream);
@@ -600,10 +616,10 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
         const dim3 g((unsigned)((n / kTile) * njobs));
         const int lgn = log2i(n);
         switch (ks[np - 1]) {
-            case 5: hipLaunchKernelGGL(k_fft_mid<5>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
-            case 6: hipLaunchKernelGGL(k_fft_mid<6>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
             case 7: hipLaunchKernelGGL(k_fft_mid<7>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
-            default: hipLaunchKernelGGL(k_fft_mid<8>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
+            case 8: hipLaunchKernelGGL(k_fft_mid<8>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
+            case 9: hipLaunchKernelGGL(k_fft_mid<9>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
+            default: hipLaunchKernelGGL(k_fft_mid<10>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
         }
     } else {
         hipLaunchKernelGGL(k_cross_terms, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, stream, d_jobs, n);

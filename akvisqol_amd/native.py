@@ -362,8 +362,13 @@ class DeviceBuffer:
             pass
 
 
-def measure_batch_device(ctx: Context, refs: list[DeviceBuffer], degs: list[DeviceBuffer], sample_rate: int):
-    """visqol_hip_measure_batch_device over device-resident signals."""
+def measure_batch_device(ctx: Context, refs: list[DeviceBuffer], degs: list[DeviceBuffer], sample_rate: int,
+                         convert: bool = True):
+    """visqol_hip_measure_batch_device over device-resident signals.
+
+    convert=False reads only each result's MOS (NaN for a failed pair)
+    before freeing it, as a C caller that keeps the scores would; the
+    results list is then a float64 array."""
     n = len(refs)
     rp = (c_void_p * n)(*[r.ptr for r in refs])
     dp = (c_void_p * n)(*[d.ptr for d in degs])
@@ -373,6 +378,13 @@ def measure_batch_device(ctx: Context, refs: list[DeviceBuffer], degs: list[Devi
     st = (c_int32 * n)()
     with ctx._lock:
         rc = lib.visqol_hip_measure_batch_device(ctx._h, n, rp, rl, dp, dl, sample_rate, out, st)
+    if not convert:
+        mos = np.full(n, np.nan)
+        for i in range(n):
+            if out[i]:
+                mos[i] = out[i].contents.moslqo
+                lib.visqol_result_free(out[i])
+        return rc, mos, list(st)
     results = []
     for i in range(n):
         if out[i]:

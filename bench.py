@@ -2,7 +2,8 @@
 
 One step = one batch of `--batch` 10 s, 48 kHz audio pairs (seeds 0..B-1,
 resident in HBM before timing) scored by visqol_hip_measure_batch_device
-in the throughput mode (precision "fast": f64 with fused multiply-adds,
+(the C call, reading each result's MOS and freeing the results; the Python
+dict conversion of the full results is not timed) in the throughput mode (precision "fast": f64 with fused multiply-adds,
 |dMOS| <= 1e-6 against the f64 oracle, checked on a sample every run).
 Prints one JSON line: whole-job pairs/s, the gammatone kernel's roofline
 fraction from HIP events on the library's stream, the plain per-pair exact
@@ -72,7 +73,7 @@ def main() -> int:
     times = []
     for _ in range(args.steps):
         t = time.perf_counter()
-        rc, res, st = native.measure_batch_device(ctx, refs, degs, 48000)
+        rc, fast_mos, st = native.measure_batch_device(ctx, refs, degs, 48000, convert=False)
         times.append(time.perf_counter() - t)
         if rc != 0:
             raise SystemExit(f"step failed: {rc}")
@@ -87,7 +88,6 @@ def main() -> int:
     ctx.profile(False, True)
     step_s = float(np.mean(times))
     value = args.batch / step_s
-    fast_mos = np.array([r["moslqo"] for r in res])
 
     # Plain per-pair f64 path (exact precision, one pair per call).
     ex = native.Context(native.MODE_AUDIO, precision="exact")
