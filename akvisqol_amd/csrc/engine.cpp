@@ -182,7 +182,8 @@ void Engine::lap(int phase) {
 
 void Engine::sums(const std::vector<SumJob>& jobs, double* d_out, bool serial) {
     if (jobs.empty()) return;
-    double* parts = serial ? nullptr : (double*)arena_.alloc(sizeof(double) * kSumParts * jobs.size());
+    double* parts =
+        serial ? nullptr : (double*)arena_.alloc(sizeof(double) * kSumOutputs * kSumParts * jobs.size());
     long long max_len = 0;
     for (const SumJob& j : jobs) max_len = std::max(max_len, j.len);
     launch_sums(upload(jobs), (int)jobs.size(), max_len, d_out, serial, parts, stream_);
@@ -298,17 +299,24 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     std::vector<LagJob> lj;
     for (int i = 0; i < n; i++) {
         PairState& s = st[i];
-        if (!opts_.no_global && s.rl > 0 && s.dl > 0) {
+        const bool global = !opts_.no_global && s.rl > 0 && s.dl > 0;
+        if (global && opts_.fast) {
+            // one pass per signal: sum, Nyquist sum, and the reference energy
+            j1.push_back(SumJob{s.dref, s.rl, kMoments, 6 * i, 6 * i + 2, 6 * i + 4, nullptr, 0});
+            j1.push_back(SumJob{s.ddeg, s.dl, kMoments, 6 * i + 1, 6 * i + 3, -1, nullptr, 0});
+        } else if (global) {
             j1.push_back(SumJob{s.dref, s.rl, kSum, 6 * i, 0, 0, nullptr, 0});
             j1.push_back(SumJob{s.ddeg, s.dl, kSum, 6 * i + 1, 0, 0, nullptr, 0});
             j2.push_back(SumJob{s.dref, s.rl, kNyquist, 6 * i + 2, 6 * i, 0, nullptr, 0});
             j2.push_back(SumJob{s.ddeg, s.dl, kNyquist, 6 * i + 3, 6 * i + 1, 0, nullptr, 0});
+        }
+        if (global) {
             const long long nfft = fft_size_for(std::max(s.rl, s.dl));
             double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
             s.lagjob = (int)lj.size();
             lj.push_back(LagJob{s.dref, s.ddeg, s.rl, s.dl, z, nfft, 6 * i, 6 * i + 1, 6 * i + 2, 6 * i + 3});
         }
-        j1.push_back(SumJob{s.dref, s.rl, kSumSq, 6 * i + 4, 0, 0, nullptr, 0});
+        if (!(global && opts_.fast)) j1.push_back(SumJob{s.dref, s.rl, kSumSq, 6 * i + 4, 0, 0, nullptr, 0});
     }
     sums(j1, d_sums, serial);
     sums(j2, d_sums, serial);
@@ -689,10 +697,15 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             for (size_t k = 0; k < rjs.size(); k++) {
                 RJ& rj = rjs[k];
                 const int b = 4 * (int)k;
-                a1.push_back(SumJob{rj.sref, rj.rlen, kSum, b, 0, 0, nullptr, 0});
-                a1.push_back(SumJob{rj.sdeg, rj.dlen, kSum, b + 1, 0, 0, nullptr, 0});
-                a2.push_back(SumJob{rj.sref, rj.rlen, kNyquist, b + 2, b, 0, nullptr, 0});
-                a2.push_back(SumJob{rj.sdeg, rj.dlen, kNyquist, b + 3, b + 1, 0, nullptr, 0});
+                if (opts_.fast) {
+                    a1.push_back(SumJob{rj.sref, rj.rlen, kMoments, b, b + 2, -1, nullptr, 0});
+                    a1.push_back(SumJob{rj.sdeg, rj.dlen, kMoments, b + 1, b + 3, -1, nullptr, 0});
+                } else {
+                    a1.push_back(SumJob{rj.sref, rj.rlen, kSum, b, 0, 0, nullptr, 0});
+                    a1.push_back(SumJob{rj.sdeg, rj.dlen, kSum, b + 1, 0, 0, nullptr, 0});
+                    a2.push_back(SumJob{rj.sref, rj.rlen, kNyquist, b + 2, b, 0, nullptr, 0});
+                    a2.push_back(SumJob{rj.sdeg, rj.dlen, kNyquist, b + 3, b + 1, 0, nullptr, 0});
+                }
                 const long long nfft = fft_size_for(std::max(rj.rlen, rj.dlen));
                 double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
                 rj.lagjob = (int)rlj.size();

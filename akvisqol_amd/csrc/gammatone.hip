@@ -1,12 +1,13 @@
 // Gammatone ERB filterbank spectrogram on gfx950 (gammatone.zig:268-306,
 // 432-467; accel.zig:116-124).
 //
-// Work decomposition: one lane per (frame, band).  A wave holds 64
-// consecutive frames of ONE band, so the band's eight coefficients are
+// Work decomposition: one lane per (frame, band pair).  A wave holds 64
+// consecutive frames of two bands, so the bands' coefficients are
 // wave-uniform (scalar registers) and the four cascaded DF2T biquads run in
-// vector registers, in the reference's exact operation order.  A block
-// holds four band waves over the same 64 frames; the frames' Hann-windowed
-// samples are staged chunk by chunk in LDS (coalesced 512 B rows from HBM,
+// vector registers, in the reference's exact operation order; each lane
+// runs two bands' recurrences side by side.  A block holds four waves (eight
+// bands) over the same 64 frames; the frames' Hann-windowed
+// samples are staged chunk by chunk in LDS (coalesced rows of 32 samples,
 // x·hann computed once per block and shared by its bands), the next chunk's
 // loads in flight while the current chunk's recurrences run.
 //
@@ -20,10 +21,51 @@
 namespace vq {
 
 constexpr int kFramesPerWave = 64;
-constexpr int kChunk = 64;
-constexpr int kBandsPerBlock = 4;
-constexpr int kThreadsGt = 64 * kBandsPerBlock;
+constexpr int kChunk = 32;        // samples per staged chunk (LDS tile 64 x 33 doubles)
+constexpr int kWavesGt = 4;
+constexpr int kBandsPerWave = 2;  // two independent recurrences per lane
+constexpr int kBandsPerBlock = kWavesGt * kBandsPerWave;
+constexpr int kThreadsGt = 64 * kWavesGt;
 constexpr int kStage = kFramesPerWave * kChunk / kThreadsGt;  // tile elements staged per thread
+
+// The four cascaded DF2T biquads of one band on one frame, and the sum of
+// squares of their output.
+struct BandState {
+    double s1c0 = 0, s1c1 = 0, s2c0 = 0, s2c1 = 0, s3c0 = 0, s3c1 = 0, s4c0 = 0, s4c1 = 0, sum_sq = 0;
+};
+
+template <bool FAST>
+__device__ inline void band_step(BandState& b, const BandCoef& k, double s) {
+    if (FAST) {
+        const double f1 = __builtin_fma(k.s1n0, s, b.s1c0);
+        b.s1c0 = __builtin_fma(-k.d1, f1, __builtin_fma(k.s1n1, s, b.s1c1));
+        b.s1c1 = -k.d2 * f1;
+        const double f2 = __builtin_fma(k.ca0, f1, b.s2c0);
+        b.s2c0 = __builtin_fma(-k.d1, f2, __builtin_fma(k.s2n1, f1, b.s2c1));
+        b.s2c1 = -k.d2 * f2;
+        const double f3 = __builtin_fma(k.ca0, f2, b.s3c0);
+        b.s3c0 = __builtin_fma(-k.d1, f3, __builtin_fma(k.s3n1, f2, b.s3c1));
+        b.s3c1 = -k.d2 * f3;
+        const double f4 = __builtin_fma(k.ca0, f3, b.s4c0);
+        b.s4c0 = __builtin_fma(-k.d1, f4, __builtin_fma(k.s4n1, f3, b.s4c1));
+        b.s4c1 = -k.d2 * f4;
+        b.sum_sq = __builtin_fma(f4, f4, b.sum_sq);
+    } else {
+        const double f1 = k.s1n0 * s + b.s1c0;
+        b.s1c0 = k.s1n1 * s + b.s1c1 - k.d1 * f1;
+        b.s1c1 = -(k.d2 * f1);
+        const double f2 = k.ca0 * f1 + b.s2c0;
+        b.s2c0 = k.s2n1 * f1 + b.s2c1 - k.d1 * f2;
+        b.s2c1 = -(k.d2 * f2);
+        const double f3 = k.ca0 * f2 + b.s3c0;
+        b.s3c0 = k.s3n1 * f2 + b.s3c1 - k.d1 * f3;
+        b.s3c1 = -(k.d2 * f3);
+        const double f4 = k.ca0 * f3 + b.s4c0;
+        b.s4c0 = k.s4n1 * f3 + b.s4c1 - k.d1 * f4;
+        b.s4c1 = -(k.d2 * f4);
+        b.sum_sq += f4 * f4;
+    }
+}
 
 template <bool FAST>
 __global__ __launch_bounds__(kThreadsGt) void k_gammatone(const GtJob* __restrict__ jobs, const BandCoef* __restrict__ coef,
@@ -34,32 +76,36 @@ __global__ __launch_bounds__(kThreadsGt) void k_gammatone(const GtJob* __restric
     __shared__ double tile[kFramesPerWave][kChunk + 1];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int band = blockIdx.y * kBandsPerBlock + wave;
-    const bool active_band = band < nb;
+    const int band0 = blockIdx.y * kBandsPerBlock + wave * kBandsPerWave;
     const int frame = f0 + lane;
-    const bool active = active_band && frame < job.nf;
+    BandCoef k[kBandsPerWave];
+    BandState st[kBandsPerWave];
+#pragma unroll
+    for (int q = 0; q < kBandsPerWave; q++) {
+        if (band0 + q < nb) k[q] = coef[band0 + q];
+        else k[q] = BandCoef{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+    const bool active = band0 < nb && frame < job.nf;
 
-    BandCoef k;
-    if (active_band) k = coef[band];
-    else k = BandCoef{0, 0, 0, 0, 0, 0, 0, 0};
-    double s1c0 = 0, s1c1 = 0, s2c0 = 0, s2c1 = 0, s3c0 = 0, s3c1 = 0, s4c0 = 0, s4c1 = 0, sum_sq = 0;
-
-    // Staging: thread t stages samples i0 + lane of frames f0 + (t >> 6) +
-    // 4u (u < kStage), so every wave reads 512 B rows and the Hann factor is
-    // one value per thread and chunk.  The next chunk's loads are issued
-    // (unconditionally, from clamped in-range addresses) before the current
-    // chunk's recurrence runs; masking and x·hann happen at the LDS write.
+    // Staging: thread t stages sample i0 + (t mod kChunk) of frames
+    // f0 + t / kChunk + u·(256 / kChunk) (u < kStage), so each wave reads
+    // contiguous runs of kChunk samples and the Hann factor is one value per
+    // thread and chunk.  The next chunk's loads are issued (unconditionally,
+    // from clamped in-range addresses) before the current chunk's recurrence
+    // runs; masking and x·hann happen at the LDS write.
+    constexpr int kRowsPerPass = kThreadsGt / kChunk;
     const auto* sig = as_global(job.sig);
-    const int fr0 = threadIdx.x >> 6;
+    const int ti = threadIdx.x % kChunk;
+    const int fr0 = threadIdx.x / kChunk;
     double v[kStage];
     double h = 0.0;
     auto load_chunk = [&](int i0) {
-        const int i = i0 + lane;
+        const int i = i0 + ti;
         const int ic = i < window ? i : 0;
         h = hann[ic];
 #pragma unroll
         for (int u = 0; u < kStage; u++) {
-            const int fr = fr0 + u * kBandsPerBlock;
+            const int fr = fr0 + u * kRowsPerPass;
             const int fc = f0 + fr < job.nf ? f0 + fr : f0;
             v[u] = sig[(size_t)fc * hop + ic];
         }
@@ -68,52 +114,27 @@ __global__ __launch_bounds__(kThreadsGt) void k_gammatone(const GtJob* __restric
     for (int i0 = 0; i0 < window; i0 += kChunk) {
         const int clen = min(kChunk, window - i0);
         __syncthreads();
-        const bool in = i0 + lane < window;
+        const bool in = i0 + ti < window;
 #pragma unroll
         for (int u = 0; u < kStage; u++) {
-            const int fr = fr0 + u * kBandsPerBlock;
-            tile[fr][lane] = (in && f0 + fr < job.nf) ? v[u] * h : 0.0;
+            const int fr = fr0 + u * kRowsPerPass;
+            tile[fr][ti] = (in && f0 + fr < job.nf) ? v[u] * h : 0.0;
         }
         __syncthreads();
         if (i0 + kChunk < window) load_chunk(i0 + kChunk);
         if (active) {
             for (int i = 0; i < clen; i++) {
                 const double s = tile[lane][i];
-                if (FAST) {
-                    const double f1 = __builtin_fma(k.s1n0, s, s1c0);
-                    s1c0 = __builtin_fma(-k.d1, f1, __builtin_fma(k.s1n1, s, s1c1));
-                    s1c1 = -k.d2 * f1;
-                    const double f2 = __builtin_fma(k.ca0, f1, s2c0);
-                    s2c0 = __builtin_fma(-k.d1, f2, __builtin_fma(k.s2n1, f1, s2c1));
-                    s2c1 = -k.d2 * f2;
-                    const double f3 = __builtin_fma(k.ca0, f2, s3c0);
-                    s3c0 = __builtin_fma(-k.d1, f3, __builtin_fma(k.s3n1, f2, s3c1));
-                    s3c1 = -k.d2 * f3;
-                    const double f4 = __builtin_fma(k.ca0, f3, s4c0);
-                    s4c0 = __builtin_fma(-k.d1, f4, __builtin_fma(k.s4n1, f3, s4c1));
-                    s4c1 = -k.d2 * f4;
-                    sum_sq = __builtin_fma(f4, f4, sum_sq);
-                } else {
-                    const double f1 = k.s1n0 * s + s1c0;
-                    s1c0 = k.s1n1 * s + s1c1 - k.d1 * f1;
-                    s1c1 = -(k.d2 * f1);
-                    const double f2 = k.ca0 * f1 + s2c0;
-                    s2c0 = k.s2n1 * f1 + s2c1 - k.d1 * f2;
-                    s2c1 = -(k.d2 * f2);
-                    const double f3 = k.ca0 * f2 + s3c0;
-                    s3c0 = k.s3n1 * f2 + s3c1 - k.d1 * f3;
-                    s3c1 = -(k.d2 * f3);
-                    const double f4 = k.ca0 * f3 + s4c0;
-                    s4c0 = k.s4n1 * f3 + s4c1 - k.d1 * f4;
-                    s4c1 = -(k.d2 * f4);
-                    sum_sq += f4 * f4;
-                }
+#pragma unroll
+                for (int q = 0; q < kBandsPerWave; q++) band_step<FAST>(st[q], k[q], s);
             }
         }
     }
     if (active) {
         const double inv_n = 1.0 / (double)window;
-        job.out[(size_t)band * job.nf + frame] = sqrt(sum_sq * inv_n);
+#pragma unroll
+        for (int q = 0; q < kBandsPerWave; q++)
+            if (band0 + q < nb) job.out[(size_t)(band0 + q) * job.nf + frame] = sqrt(st[q].sum_sq * inv_n);
     }
 }
 

@@ -24,26 +24,30 @@ void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef
 // ---------------------------------------------------------------- reductions
 // Serial (reference-order) or tree reductions of one signal each.
 enum SumKind : int {
-    kSum = 0,       // Σ x
-    kSumSq = 1,     // Σ x·x
-    kNyquist = 2,   // Σ (-1)^i (x - mean), mean = out[aux] / len
-    kSumSqFrom = 3  // Σ x·x over x[start..len), start from a lag slot (see SumJob)
+    kSum = 0,        // Σ x
+    kSumSq = 1,      // Σ x·x
+    kNyquist = 2,    // Σ (-1)^i (x - mean), mean = out[aux] / len
+    kSumSqFrom = 3,  // Σ x·x over x[start..len), start from a lag slot (see SumJob)
+    kMoments = 4     // tree only, one pass: out[slot] = Σ x, out[aux] = the
+                     // Nyquist sum as Σ (-1)^i x - mean·(len mod 2), and
+                     // out[slot2] = Σ x·x when slot2 >= 0
 };
 struct SumJob {
     const double* x;
     long long len;
     int kind;
     int slot;              // result goes to out[slot]
-    int aux;               // kNyquist: slot holding the plain sum
-    int pad_;
+    int aux;               // kNyquist: slot holding the plain sum; kMoments: Nyquist output slot
+    int slot2;             // kMoments: Σ x·x output slot, or -1
     const long long* lag;  // kSumSqFrom: global lag; start = -lag if 0 < -lag <= half, else 0
     long long half;        // kSumSqFrom: ref_len / 2
 };
 // out[job.slot] = reduction of each job.  `serial` keeps the reference's
 // left-to-right order (bit-exact); otherwise a deterministic tree order over
 // up to kSumParts shares per job (max_len: longest job), d_partials holding
-// njobs * kSumParts doubles.
+// njobs * kSumOutputs * kSumParts doubles.
 constexpr int kSumParts = 64;
+constexpr int kSumOutputs = 3;
 void launch_sums(const SumJob* d_jobs, int njobs, long long max_len, double* d_out, bool serial, double* d_partials,
                  hipStream_t stream);
 

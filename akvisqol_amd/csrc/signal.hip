@@ -80,29 +80,57 @@ __global__ __launch_bounds__(256) void k_sums(const SumJob* __restrict__ jobs, d
         if (threadIdx.x == 0) out[j.slot] = s;
         return;
     }
-    // tree: this block's share of [start, len) into partials[job][blockIdx.y]
+    // tree: this block's share of [start, len) into
+    // partials[(job·kSumOutputs + output)·kSumParts + blockIdx.y]
     const long long total = j.len - start;
     const long long per = (total + parts - 1) / parts;
     const long long lo = start + per * blockIdx.y;
     const long long hi = lo + per < j.len ? lo + per : j.len;
-    __shared__ double part[256];
-    double s = 0.0;
-    for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += sum_term(j, i, mean);
-    part[threadIdx.x] = s;
+    __shared__ double part[kSumOutputs][256];
+    double s = 0.0, alt = 0.0, sq = 0.0;
+    if (j.kind == kMoments) {
+        const auto* x = as_global(j.x);
+        for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+            const double v = x[i];
+            s += v;
+            alt += (i & 1) ? -v : v;
+            sq = __builtin_fma(v, v, sq);
+        }
+    } else {
+        for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += sum_term(j, i, mean);
+    }
+    part[0][threadIdx.x] = s;
+    part[1][threadIdx.x] = alt;
+    part[2][threadIdx.x] = sq;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        if ((int)threadIdx.x < w)
+            for (int o = 0; o < kSumOutputs; o++) part[o][threadIdx.x] += part[o][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partials[(long long)blockIdx.x * kSumParts + blockIdx.y] = part[0];
+    if (threadIdx.x < kSumOutputs)
+        partials[((long long)blockIdx.x * kSumOutputs + threadIdx.x) * kSumParts + blockIdx.y] = part[threadIdx.x][0];
 }
 
-// out[slot] = fixed-order sum of a job's partials (one wave per job).
+// Fixed-order sums of a job's partials (one wave per job), written to the
+// job's output slots.
 __global__ __launch_bounds__(64) void k_sums_finish(const SumJob* __restrict__ jobs, const double* __restrict__ partials,
                                                     double* __restrict__ out, int parts) {
-    double v = (int)threadIdx.x < parts ? partials[(long long)blockIdx.x * kSumParts + threadIdx.x] : 0.0;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (threadIdx.x == 0) out[jobs[blockIdx.x].slot] = v;
+    const SumJob j = jobs[blockIdx.x];
+    double r[kSumOutputs];
+    for (int o = 0; o < kSumOutputs; o++) {
+        double v = (int)threadIdx.x < parts ? partials[((long long)blockIdx.x * kSumOutputs + o) * kSumParts + threadIdx.x]
+                                            : 0.0;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        r[o] = v;
+    }
+    if (threadIdx.x != 0) return;
+    out[j.slot] = r[0];
+    if (j.kind == kMoments) {
+        const double mean = j.len > 0 ? r[0] / (double)j.len : 0.0;
+        out[j.aux] = (j.len & 1) ? r[1] - mean : r[1];
+        if (j.slot2 >= 0) out[j.slot2] = r[2];
+    }
 }
 
 void launch_sums(const SumJob* d_jobs, int njobs, long long max_len, double* d_out, bool serial, double* d_partials,

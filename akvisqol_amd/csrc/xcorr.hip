@@ -38,20 +38,36 @@ __device__ inline long long env_fft_size(long long l) {
 }
 
 // envelope.zig:23-53 closed form for sample i of a signal with the given
-// plain sum and alternating (Nyquist) sum.
-template <typename P>
-__device__ inline double env_at(P x, long long i, long long len, double sum, double nyq) {
-    const double mean = sum / (double)len;
-    const double nq = nyq / (double)env_fft_size(len);
-    const double term = (i & 1) ? -nq : nq;
-    return fabs(2.0 * ((x[i] - mean) - term) - 0.000001) + mean;
+// plain sum and alternating (Nyquist) sum: the per-signal constants once,
+// then one value per sample.
+struct EnvSide {
+    const double* x;
+    long long len;
+    double mean, nq;
+};
+__device__ inline EnvSide env_side(const double* x, long long len, double sum, double nyq) {
+    EnvSide e{x, len, 0.0, 0.0};
+    if (len > 0) {
+        e.mean = sum / (double)len;
+        e.nq = nyq / (double)env_fft_size(len);
+    }
+    return e;
+}
+__device__ inline double env_value(const EnvSide& e, double xi, long long i) {
+    const double term = (i & 1) ? -e.nq : e.nq;
+    return fabs(2.0 * ((xi - e.mean) - term) - 0.000001) + e.mean;
+}
+// The sample a lane needs for index i, loaded unconditionally from an
+// in-range address (the value is discarded past the end).
+__device__ inline double env_load(const EnvSide& e, long long i) {
+    return e.len > 0 ? as_global(e.x)[i < e.len ? i : 0] : 0.0;
 }
 
 __device__ inline double2 packed_env(const LagJob& j, const double* sums, long long i) {
-    double re = 0.0, im = 0.0;
-    if (i < j.la) re = env_at(as_global(j.a), i, j.la, sums[j.sum_a], sums[j.nyq_a]);
-    if (i < j.lb) im = env_at(as_global(j.b), i, j.lb, sums[j.sum_b], sums[j.nyq_b]);
-    return make_double2(re, im);
+    const EnvSide a = env_side(j.a, j.la, sums[j.sum_a], sums[j.nyq_a]);
+    const EnvSide b = env_side(j.b, j.lb, sums[j.sum_b], sums[j.nyq_b]);
+    return make_double2(i < a.len ? env_value(a, env_load(a, i), i) : 0.0,
+                        i < b.len ? env_value(b, env_load(b, i), i) : 0.0);
 }
 
 constexpr int kTile = 2048;  // complex doubles per workgroup (32 KiB LDS)
@@ -367,13 +383,27 @@ __global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict_
         return base + ((long long)bb << lgS) + ((long long)m << lgq) + rr;
     };
     constexpr int kPer = kTile / kThreads;
-    {
-        double2 v[kPer];
+    if (SRC == kSrcEnvelope) {
+        // both signals' samples for the tile, loads first, envelope after
+        const EnvSide ea = env_side(job.a, job.la, sums[job.sum_a], sums[job.nyq_a]);
+        const EnvSide eb = env_side(job.b, job.lb, sums[job.sum_b], sums[job.nyq_b]);
+        double xa[kPer], xb[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
             const long long i = gidx(threadIdx.x + u * kThreads);
-            v[u] = SRC == kSrcEnvelope ? packed_env(job, sums, i) : load2(z, i);
+            xa[u] = env_load(ea, i);
+            xb[u] = env_load(eb, i);
         }
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const long long i = gidx(threadIdx.x + u * kThreads);
+            sm[threadIdx.x + u * kThreads] = make_double2(i < ea.len ? env_value(ea, xa[u], i) : 0.0,
+                                                          i < eb.len ? env_value(eb, xb[u], i) : 0.0);
+        }
+    } else {
+        double2 v[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) v[u] = load2(z, gidx(threadIdx.x + u * kThreads));
 #pragma unroll
         for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
     }
