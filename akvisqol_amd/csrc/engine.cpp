@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <map>
+#include <thread>
 
 namespace vq {
 
@@ -819,8 +820,24 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         }
     }
 
-    for (int i = 0; i < n; i++)
-        if (st[i].live) finish(out[i], rp);
+    // Aggregation and MOS mapping per pair.  The SVR and exponential maps are
+    // pure functions of the pair's result, so larger batches spread them over
+    // a few host threads; the lattice interpreter runs on this thread.
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nthreads = (opts_.mode == 0 || !opts_.lattice) && n >= 16 ? (int)std::min<unsigned>({8u, hw, (unsigned)n / 8})
+                                                                        : 1;
+    if (nthreads > 1) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthreads; t++)
+            pool.emplace_back([&, t] {
+                for (int i = t; i < n; i += nthreads)
+                    if (st[i].live) finish(out[i], rp);
+            });
+        for (auto& th : pool) th.join();
+    } else {
+        for (int i = 0; i < n; i++)
+            if (st[i].live) finish(out[i], rp);
+    }
     lap(7);
 }
 

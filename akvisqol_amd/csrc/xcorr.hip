@@ -457,10 +457,93 @@ __global__ __launch_bounds__(256) void k_cross_terms(const LagJob* __restrict__ 
 // 2^k <= w < 2^(k+1): chunks 2^(k+1) + t and 2^(k+2) - 1 - t, t = w - 2^k.
 constexpr int kHalfTile = kTile / 2;
 
+// One radix-2^B step (B = 1, 2, 3) over the eight elements a thread holds in
+// a pass over contiguous blocks (one residue): 8 >> B groups, element c of
+// group gi in x[gi·2^B + c] at block position m0 + c·2^lgH.  The stage with
+// distance d·2^lgH (d = 2^s, s < B) pairs c with c + d (bit s of c clear)
+// and takes W_{2d·2^lgH} at position lo + (c mod d)·2^lgH, i.e. table index
+// (d << lgH) + lo + ((c mod d) << lgH).  DIF runs the stages largest
+// distance first, DIT smallest first: the radix-2 butterflies of the
+// stage-by-stage transform.
+template <int B, bool DIT>
+__device__ inline void radix_step(double2 (&x)[8], const int (&lo)[8 >> B], int lgH, const double2* __restrict__ tw) {
+    constexpr int G = 1 << B;
+    constexpr int NG = 8 >> B;
+    double2 w[NG][G];  // w[gi][d + t]: stage distance d, position t (t < d)
+#pragma unroll
+    for (int gi = 0; gi < NG; gi++)
+#pragma unroll
+        for (int s = 0; s < B; s++) {
+            const int d = 1 << s;
+#pragma unroll
+            for (int t = 0; t < d; t++) w[gi][d + t] = tw[(d << lgH) + lo[gi] + (t << lgH)];
+        }
+#pragma unroll
+    for (int si = 0; si < B; si++) {
+        const int s = DIT ? si : B - 1 - si;
+        const int d = 1 << s;
+#pragma unroll
+        for (int gi = 0; gi < NG; gi++)
+#pragma unroll
+            for (int c = 0; c < G; c++) {
+                if (c & d) continue;
+                double2& a = x[gi * G + c];
+                double2& b = x[gi * G + c + d];
+                if (DIT) bf_dit(a, b, w[gi][d + (c & (d - 1))]);
+                else bf_dif(a, b, w[gi][d + (c & (d - 1))]);
+            }
+    }
+}
+
+// Tile positions (block bb of 2^K elements, position m) of the thread's
+// eight elements for a radix-2^B step with smallest distance 2^lgH: group
+// G = thread + gi·256, q = G mod 2^(K-B) its index in the block, lo = q mod
+// 2^lgH, m0 = q with B zero bits inserted at bit lgH.
+template <int K, int B>
+__device__ inline void group_layout(int lgH, int (&idx)[8], int (&lo)[8 >> B]) {
+#pragma unroll
+    for (int gi = 0; gi < (8 >> B); gi++) {
+        const int g = (int)threadIdx.x + gi * kThreads;
+        const int q = g & ((1 << (K - B)) - 1);
+        const int bb = g >> (K - B);
+        lo[gi] = q & ((1 << lgH) - 1);
+        const int m0 = ((q >> lgH) << (lgH + B)) | lo[gi];
+#pragma unroll
+        for (int c = 0; c < (1 << B); c++) idx[gi * (1 << B) + c] = (bb << K) + m0 + (c << lgH);
+    }
+}
+
+// Stage counts of the steps before the central one (K - 3 stages in steps of
+// at most three, larger first): K = 7: 2,2; 8: 3,2; 9: 3,3; 10: 3,2,2.
 template <int K>
-__global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__ jobs, int lgn,
-                                                      const double2* __restrict__ tw, int njobs) {
+struct MidPlan {
+    static constexpr int kPre = K - 3;
+    static constexpr int kSteps = (kPre + 2) / 3;
+    static constexpr int b(int i) { return kPre / kSteps + (i < kPre % kSteps ? 1 : 0); }
+};
+
+// Middle of the lag search (n >= kTile): the forward transform's last pass,
+// the cross terms and the inverse transform's first pass in one kernel.
+// Both passes work on aligned contiguous blocks of 2^K elements, and the
+// cross term of bit-reversed position i pairs it with 3h-1-i (h the largest
+// power of two <= i), which maps chunk c of 1024 elements in [h, 2h) onto
+// the whole chunk 2h/1024 - 1 - c; chunks 0 and 1 pair within themselves.
+// Workgroup w holds the chunk pair (0, 1), or for w >= 1 with
+// 2^k <= w < 2^(k+1): chunks 2^(k+1) + t and 2^(k+2) - 1 - t, t = w - 2^k.
+//
+// A thread holds eight elements throughout.  The forward stages run in
+// radix-8/4 steps (the first loading straight from memory); the central
+// step holds an aligned run of eight, runs the forward stages of distance
+// 4, 2, 1, reads the mirror run (the cross-term partners of an aligned run
+// of eight form another aligned run of eight) through LDS, forms the cross
+// terms and runs the inverse stages of distance 1, 2, 4 in registers; the
+// remaining inverse steps mirror the forward ones, the last storing straight
+// to memory.  Every butterfly and twiddle is the stage-by-stage one.
+template <int K>
+__global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__ jobs, const double2* __restrict__ tw,
+                                                      int njobs) {
     __shared__ double2 sm[kTile];
+    using P = MidPlan<K>;
     const int jb = (int)(blockIdx.x % (unsigned)njobs);
     const long long wg = blockIdx.x / (unsigned)njobs;
     double2* z = jobs[jb].z;
@@ -472,39 +555,101 @@ __global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__
         cb = (4LL << k) - 1 - t;
     }
     auto gidx = [&](int e) -> long long { return (e < kHalfTile ? ca : cb) * kHalfTile + (e & (kHalfTile - 1)); };
-    constexpr int kPer = kTile / kThreads;
-    {
-        double2 v[kPer];
+    // Tile element e sits in LDS at e ^ ((e >> 4) & 7), a permutation within
+    // each run of 16: lanes that each hold eight consecutive elements then
+    // spread over all banks instead of two (an 8-way conflict unswizzled).
+    auto swz = [](int e) { return e ^ ((e >> 4) & 7); };
+    double2 x[8];
+    int idx[8];
+    // LDS hand-over between steps: store at the current layout, barrier,
+    // load at the next.
+    auto put = [&]() {
 #pragma unroll
-        for (int u = 0; u < kPer; u++) v[u] = load2(z, gidx(threadIdx.x + u * kThreads));
+        for (int c = 0; c < 8; c++) sm[swz(idx[c])] = x[c];
+    };
+    auto get = [&]() {
 #pragma unroll
-        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
+        for (int c = 0; c < 8; c++) x[c] = sm[swz(idx[c])];
+    };
+    // forward steps before the central one
+    int top = K;
+#pragma unroll
+    for (int i = 0; i < P::kSteps; i++) {
+        const int B = P::b(i);
+        const int lgH = top - B;
+        top -= B;
+        if (B == 3) {
+            int lo[1];
+            group_layout<K, 3>(lgH, idx, lo);
+            if (i == 0) {
+#pragma unroll
+                for (int c = 0; c < 8; c++) x[c] = load2(z, gidx(idx[c]));
+            } else {
+                get();
+            }
+            radix_step<3, false>(x, lo, lgH, tw);
+        } else {
+            int lo[2];
+            group_layout<K, 2>(lgH, idx, lo);
+            if (i == 0) {
+#pragma unroll
+                for (int c = 0; c < 8; c++) x[c] = load2(z, gidx(idx[c]));
+            } else {
+                get();
+            }
+            radix_step<2, false>(x, lo, lgH, tw);
+        }
+        put();
+        __syncthreads();
     }
-    tile_stages<K, false>(sm, 0, 0, 0, tw);  // spans of 2^K, one residue
+    // central step: forward distances 4, 2, 1; cross terms; inverse 1, 2, 4
     {
-        double2 v[kPer];
+        int lo[1];
+        group_layout<K, 3>(0, idx, lo);
+        get();
+        radix_step<3, false>(x, lo, 0, tw);
+        put();
+        __syncthreads();
+        double2 mirror[8];
 #pragma unroll
-        for (int u = 0; u < kPer; u++) {
-            const int e = threadIdx.x + u * kThreads;
-            const long long g = gidx(e);
+        for (int c = 0; c < 8; c++) {
+            const long long g = gidx(idx[c]);
             long long pg = g;
             if (g >= 2) {
                 const long long h = 1LL << (63 - __builtin_clzll((unsigned long long)g));
                 pg = 3 * h - 1 - g;
             }
             const int ep = ((pg / kHalfTile) == ca ? 0 : kHalfTile) + (int)(pg & (kHalfTile - 1));
-            v[u] = cross_term(sm[e], sm[ep]);
+            mirror[c] = sm[swz(ep)];
         }
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
+        for (int c = 0; c < 8; c++) x[c] = cross_term(x[c], mirror[c]);
+        radix_step<3, true>(x, lo, 0, tw);
     }
-    tile_stages<K, true>(sm, 0, 0, 0, tw);
+    // inverse steps, mirroring the forward ones
+    int bottom = 3;
 #pragma unroll
-    for (int u = 0; u < kPer; u++) {
-        const int e = threadIdx.x + u * kThreads;
-        store2(z, gidx(e), sm[e]);
+    for (int i = P::kSteps - 1; i >= 0; i--) {
+        const int B = P::b(i);
+        const int lgH = bottom;
+        bottom += B;
+        put();
+        __syncthreads();
+        if (B == 3) {
+            int lo[1];
+            group_layout<K, 3>(lgH, idx, lo);
+            get();
+            radix_step<3, true>(x, lo, lgH, tw);
+        } else {
+            int lo[2];
+            group_layout<K, 2>(lgH, idx, lo);
+            get();
+            radix_step<2, true>(x, lo, lgH, tw);
+        }
     }
+#pragma unroll
+    for (int c = 0; c < 8; c++) store2(z, gidx(idx[c]), x[c]);
 }
 
 // Exact reduction of the per-workgroup (max, first position) partials.
@@ -644,12 +789,11 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
     }
     if (fuse) {
         const dim3 g((unsigned)((n / kTile) * njobs));
-        const int lgn = log2i(n);
         switch (ks[np - 1]) {
-            case 7: hipLaunchKernelGGL(k_fft_mid<7>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
-            case 8: hipLaunchKernelGGL(k_fft_mid<8>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
-            case 9: hipLaunchKernelGGL(k_fft_mid<9>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
-            default: hipLaunchKernelGGL(k_fft_mid<10>, g, dim3(kThreads), 0, stream, d_jobs, lgn, d_tw, njobs); break;
+            case 7: hipLaunchKernelGGL(k_fft_mid<7>, g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 8: hipLaunchKernelGGL(k_fft_mid<8>, g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 9: hipLaunchKernelGGL(k_fft_mid<9>, g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            default: hipLaunchKernelGGL(k_fft_mid<10>, g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
         }
     } else {
         hipLaunchKernelGGL(k_cross_terms, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, stream, d_jobs, n);
