@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["gammatone.hip", "signal.hip", "xcorr.hip", "nsim.hip", "engine.cpp", "host_plan.cpp", "models.cpp", "wav.cpp",
-           "capi.cpp"]
+           "capi.cpp", "server.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -14,6 +14,7 @@
 
 #include "../../include/visqol.h"
 #include "../../include/visqol_hip.h"
+#include "capi_internal.h"
 #include "engine.h"
 #include "wav.h"
 
@@ -25,8 +26,10 @@ struct visqol_ctx {
 };
 
 namespace {
-
 thread_local char g_error[512] = {0};
+}  // namespace
+
+namespace vq_capi {
 
 void set_error(const char* fmt, ...) {
     va_list ap;
@@ -79,25 +82,6 @@ visqol_result* box(const PairResult& r) {
     return out;
 }
 
-std::string library_dir() {
-    Dl_info info;
-    if (dladdr((void*)&library_dir, &info) && info.dli_fname) {
-        std::string p = info.dli_fname;
-        const size_t s = p.rfind('/');
-        if (s != std::string::npos) return p.substr(0, s);
-    }
-    return ".";
-}
-
-bool read_file(const std::string& path, std::string& out) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) return false;
-    std::ostringstream ss;
-    ss << f.rdbuf();
-    out = ss.str();
-    return true;
-}
-
 bool device_ok(std::string& why) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
@@ -116,6 +100,30 @@ bool device_ok(std::string& why) {
         why = std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950";
         return false;
     }
+    return true;
+}
+
+}  // namespace vq_capi
+
+namespace {
+using namespace vq_capi;
+
+std::string library_dir() {
+    Dl_info info;
+    if (dladdr((void*)&library_dir, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t s = p.rfind('/');
+        if (s != std::string::npos) return p.substr(0, s);
+    }
+    return ".";
+}
+
+bool read_file(const std::string& path, std::string& out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
     return true;
 }
 
@@ -169,6 +177,49 @@ Engine* stage_engine(uint32_t mode, bool fast) {
 }
 
 }  // namespace
+
+namespace vq_capi {
+
+// Engine options and models for validated options (visqol.zig:346-359 model
+// selection; capi.zig:188-203): the embedded model unless model_path names
+// one, read only in audio mode or in speech mode with the lattice.
+int build_engine(const visqol_options& o, EngineOptions& eo, std::unique_ptr<Engine>& engine) {
+    eo = EngineOptions();
+    eo.mode = (int)o.mode;
+    eo.radius = o.search_window_radius > 0 ? o.search_window_radius : 60;
+    eo.unscaled_speech = o.use_unscaled_speech_mos_mapping != 0;
+    eo.no_global = o.disable_global_alignment != 0;
+    eo.no_realign = o.disable_realignment != 0;
+    eo.lattice = o.use_lattice_model != 0;
+    eo.fast = o.precision == VISQOL_PRECISION_FAST || o.precision == VISQOL_PRECISION_BALANCED;
+    std::unique_ptr<SvrModel> svr;
+    std::unique_ptr<LatticeModel, LatticeDeleter> lat;
+    if (eo.mode == 0 || eo.lattice) {
+        std::string path = o.model_path && o.model_path[0]
+                               ? std::string(o.model_path)
+                               : library_dir() + "/models/" + (eo.mode == 0 ? "libsvm_nu_svr_model.txt" : "lattice_speech.tflite");
+        std::string text, err;
+        if (!read_file(path, text)) {
+            set_error("cannot read model '%s'", path.c_str());
+            return VISQOL_E_IO;
+        }
+        if (eo.mode == 0) svr = SvrModel::parse(text, err);
+        else lat = load_lattice(text, kBandsSpeech, err);
+        if (!svr && !lat) {
+            set_error("invalid model '%s': %s", path.c_str(), err.c_str());
+            return VISQOL_E_MODEL;
+        }
+    }
+    try {
+        engine.reset(new Engine(eo, std::move(svr), std::move(lat)));
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return VISQOL_OK;
+}
+
+}  // namespace vq_capi
 
 extern "C" {
 
@@ -224,44 +275,15 @@ int visqol_create(const visqol_options* in, visqol_ctx** out) {
         set_error("%s", why.c_str());
         return VISQOL_E_BACKEND;
     }
-    EngineOptions eo;
-    eo.mode = (int)o.mode;
-    eo.radius = o.search_window_radius > 0 ? o.search_window_radius : 60;
-    eo.unscaled_speech = o.use_unscaled_speech_mos_mapping != 0;
-    eo.no_global = o.disable_global_alignment != 0;
-    eo.no_realign = o.disable_realignment != 0;
-    eo.lattice = o.use_lattice_model != 0;
-    eo.fast = o.precision == VISQOL_PRECISION_FAST || o.precision == VISQOL_PRECISION_BALANCED;
-    std::unique_ptr<SvrModel> svr;
-    std::unique_ptr<LatticeModel, LatticeDeleter> lat;
-    if (eo.mode == 0 || eo.lattice) {
-        std::string path = o.model_path && o.model_path[0]
-                               ? std::string(o.model_path)
-                               : library_dir() + "/models/" + (eo.mode == 0 ? "libsvm_nu_svr_model.txt" : "lattice_speech.tflite");
-        std::string text, err;
-        if (!read_file(path, text)) {
-            set_error("cannot read model '%s'", path.c_str());
-            return VISQOL_E_IO;
-        }
-        if (eo.mode == 0) svr = SvrModel::parse(text, err);
-        else lat = load_lattice(text, kBandsSpeech, err);
-        if (!svr && !lat) {
-            set_error("invalid model '%s': %s", path.c_str(), err.c_str());
-            return VISQOL_E_MODEL;
-        }
-    }
     visqol_ctx* c = new (std::nothrow) visqol_ctx();
     if (!c) {
         set_error("OutOfMemory");
         return VISQOL_E_OUT_OF_MEMORY;
     }
-    c->opts = eo;
-    try {
-        c->engine.reset(new Engine(eo, std::move(svr), std::move(lat)));
-    } catch (const BackendError& e) {
-        set_error("%s", e.msg.c_str());
+    const int rc = build_engine(o, c->opts, c->engine);
+    if (rc != VISQOL_OK) {
         delete c;
-        return e.code;
+        return rc;
     }
     *out = c;
     return VISQOL_OK;

@@ -24,6 +24,13 @@ __device__ inline double sum_term(const SumJob& j, long long i, double mean) {
 
 constexpr int kSerialTile = 2048;  // terms per LDS tile (16 KiB)
 
+// Shares of a tree reduction over a signal of `len` samples: about 8 K
+// terms each, at most kSumParts.
+__host__ __device__ inline int sum_parts_for(long long len) {
+    const long long p = (len + 8191) / 8192;
+    return p < 1 ? 1 : (p > kSumParts ? kSumParts : (int)p);
+}
+
 // serial: one block per job; the terms are added in sample order by one
 // lane (the reference's rounding sequence).  Each term (x, x·x or
 // ±(x - mean)) rounds on its own, so all lanes compute terms for the next
@@ -81,9 +88,17 @@ __global__ __launch_bounds__(256) void k_sums(const SumJob* __restrict__ jobs, d
         return;
     }
     // tree: this block's share of [start, len) into
-    // partials[(job·kSumOutputs + output)·kSumParts + blockIdx.y]
+    // partials[(job·kSumOutputs + output)·kSumParts + blockIdx.y].  The
+    // share count depends on the job's own length only, so a signal's sums
+    // do not depend on the batch it is in; blocks past it write zeros.
     const long long total = j.len - start;
-    const long long per = (total + parts - 1) / parts;
+    const int jparts = sum_parts_for(j.len);
+    if ((int)blockIdx.y >= jparts) {
+        if (threadIdx.x < kSumOutputs)
+            partials[((long long)blockIdx.x * kSumOutputs + threadIdx.x) * kSumParts + blockIdx.y] = 0.0;
+        return;
+    }
+    const long long per = (total + jparts - 1) / jparts;
     const long long lo = start + per * blockIdx.y;
     const long long hi = lo + per < j.len ? lo + per : j.len;
     __shared__ double part[kSumOutputs][256];
@@ -140,9 +155,7 @@ void launch_sums(const SumJob* d_jobs, int njobs, long long max_len, double* d_o
         hipLaunchKernelGGL(k_sums, dim3(njobs, 1), dim3(256), 0, stream, d_jobs, d_out, d_partials, 1, 1);
         return;
     }
-    // about 8 K terms per block
-    long long p = (max_len + 8191) / 8192;
-    const int parts = p < 1 ? 1 : (p > kSumParts ? kSumParts : (int)p);
+    const int parts = sum_parts_for(max_len);  // the longest job's share count
     hipLaunchKernelGGL(k_sums, dim3(njobs, parts), dim3(256), 0, stream, d_jobs, d_out, d_partials, 0, parts);
     hipLaunchKernelGGL(k_sums_finish, dim3(njobs), dim3(64), 0, stream, d_jobs, (const double*)d_partials, d_out, parts);
 }

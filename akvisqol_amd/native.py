@@ -76,6 +76,15 @@ class Result(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("workers", c_uint32),
+        ("max_batch", c_uint32),
+        ("slab_bytes", ctypes.c_uint64),
+        ("max_inflight_bytes", ctypes.c_uint64),
+    ]
+
+
 class Trace(ctypes.Structure):
     _fields_ = [
         ("lag", c_int64),
@@ -133,6 +142,14 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_math_probe": (c_int, [c_int, D, D, D, c_size_t]),
         "visqol_hip_fft": (c_int, [D, c_size_t, c_int]),
         "visqol_hip_xcorr": (c_int, [D, c_size_t, D, c_size_t, D, c_size_t, POINTER(c_size_t)]),
+        "visqol_hip_server_config_default": (None, [POINTER(ServerConfig)]),
+        "visqol_hip_server_create": (c_int, [POINTER(Options), POINTER(ServerConfig), POINTER(c_void_p)]),
+        "visqol_hip_server_submit": (c_int, [c_void_p, c_uint32, D, c_size_t, D, c_size_t, c_uint32,
+                                             POINTER(ctypes.c_uint64)]),
+        "visqol_hip_server_wait": (c_int, [c_void_p, ctypes.c_uint64, POINTER(POINTER(Result)), D]),
+        "visqol_hip_server_stats": (c_int, [c_void_p, POINTER(ctypes.c_uint64), POINTER(ctypes.c_uint64),
+                                            POINTER(ctypes.c_uint64)]),
+        "visqol_hip_server_destroy": (None, [c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -393,6 +410,62 @@ def measure_batch_device(ctx: Context, refs: list[DeviceBuffer], degs: list[Devi
         else:
             results.append(None)
     return rc, results, list(st)
+
+
+class Server:
+    """visqol_hip_server: continuous batching over a stream of requests.
+
+    submit() copies the pair to the device and returns a ticket (blocking
+    while the admission budget is full); wait(ticket) returns
+    (status, result dict or None, latency_ms from submit to completion).
+    Both may be called from several threads."""
+
+    def __init__(self, precision: str | int = "fast", workers: int = 3, max_batch: int = 256,
+                 slab_bytes: int = 16 << 30, max_inflight_bytes: int = 96 << 30, **opts):
+        o = default_options()
+        o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o.backend = BACKENDS["hip"]
+        for k, v in opts.items():
+            setattr(o, k, int(v))
+        c = ServerConfig()
+        lib.visqol_hip_server_config_default(ctypes.byref(c))
+        c.workers, c.max_batch, c.slab_bytes, c.max_inflight_bytes = workers, max_batch, slab_bytes, max_inflight_bytes
+        h = c_void_p()
+        _check(lib.visqol_hip_server_create(ctypes.byref(o), ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, mode: int, ref, deg, sample_rate: int) -> int:
+        ref, deg = _f64(ref), _f64(deg)
+        t = ctypes.c_uint64()
+        _check(lib.visqol_hip_server_submit(self._h, mode, _dptr(ref), ref.size, _dptr(deg), deg.size, sample_rate,
+                                            ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, convert: bool = True):
+        out = POINTER(Result)()
+        lat = c_double()
+        rc = lib.visqol_hip_server_wait(self._h, ticket, ctypes.byref(out), ctypes.byref(lat))
+        res = None
+        if out:
+            res = result_dict(out.contents) if convert else out.contents.moslqo
+            lib.visqol_result_free(out)
+        return rc, res, lat.value
+
+    def stats(self) -> dict:
+        b, p, f = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib.visqol_hip_server_stats(self._h, ctypes.byref(b), ctypes.byref(p), ctypes.byref(f)))
+        return dict(batches=b.value, pairs=p.value, inflight_bytes=f.value)
+
+    def close(self):
+        if self._h:
+            lib.visqol_hip_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def spectrogram(signal, sample_rate: int, mode: int = MODE_AUDIO, precision: str = "exact") -> np.ndarray:

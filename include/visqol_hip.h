@@ -65,6 +65,40 @@ int visqol_hip_profile_sim(visqol_ctx *ctx, double *ms, double *cells, int64_t *
  * Returns the number of stretches available (8). */
 int visqol_hip_profile_phases(visqol_ctx *ctx, double *ms, int n);
 
+/* Continuous-batching server (no counterpart in the reference's C ABI; a
+ * serving front end for callers with a stream of requests, in the manner of
+ * the reference's batch lanes, backend.zig:182-221).
+ *
+ * visqol_hip_server_submit copies the request's signals to device memory
+ * and queues it; it blocks while admitting it would exceed
+ * max_inflight_bytes of estimated device footprint (queued plus running
+ * requests) or the slab_bytes signal slab is full.  `workers` threads, each
+ * with its own engines and HIP stream, take the queued requests of one
+ * (mode, sample rate) at a time, up to max_batch, and score them as a batch
+ * while other workers' batches run.  visqol_hip_server_wait blocks until the
+ * ticket's result is ready, returns it as visqol_measure would (the caller
+ * frees it with visqol_result_free), and gives the time from the submit
+ * call to completion.  Each ticket is waited for exactly once.  submit and
+ * wait may be called from any threads; destroy must not overlap them.
+ * Options as for visqol_create, backend HIP; mode comes with each request;
+ * model_path must be NULL (the embedded models serve both modes). */
+typedef struct visqol_hip_server visqol_hip_server;
+typedef struct {
+    uint32_t workers;            /* concurrent batches, default 3 (1..16) */
+    uint32_t max_batch;          /* pairs per batch, default 256 */
+    uint64_t slab_bytes;         /* device memory for queued signals, default 16 GiB */
+    uint64_t max_inflight_bytes; /* admission budget, default 96 GiB */
+} visqol_hip_server_config;
+void visqol_hip_server_config_default(visqol_hip_server_config *cfg);
+int visqol_hip_server_create(const visqol_options *opts, const visqol_hip_server_config *cfg,
+                             visqol_hip_server **out);
+int visqol_hip_server_submit(visqol_hip_server *s, uint32_t mode, const double *ref, size_t ref_len,
+                             const double *deg, size_t deg_len, uint32_t sample_rate, uint64_t *ticket);
+int visqol_hip_server_wait(visqol_hip_server *s, uint64_t ticket, visqol_result **out, double *latency_ms);
+/* Batches run, pairs scored, and the footprint now admitted. */
+int visqol_hip_server_stats(visqol_hip_server *s, uint64_t *batches, uint64_t *pairs, uint64_t *inflight_bytes);
+void visqol_hip_server_destroy(visqol_hip_server *s);
+
 /* Device-resident buffers for the batch entry point above. */
 int visqol_hip_alloc(size_t bytes, void **dev_ptr);
 int visqol_hip_free(void *dev_ptr);

@@ -1,0 +1,146 @@
+"""Uneven mixed batches and the continuous-batching server on the device.
+
+* Uneven batches (audio 48 kHz and speech 16 kHz pairs of different
+  lengths, with delay, time-stretch, dropouts and noise): exact mode equals
+  the f64 oracle bit for bit, with identical alignments, pair by pair.
+* Serving: each request's result from visqol_hip_server equals the same
+  pair scored alone through visqol_measure, bit for bit (fast mode), whether
+  requests share batches, overlap across workers, or are admitted one at a
+  time by a tight footprint budget.
+"""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def native():
+    from akvisqol_amd import native as n
+
+    return n
+
+
+@pytest.fixture(scope="module")
+def synth():
+    from akvisqol_amd import synth as s
+
+    return s
+
+
+@pytest.fixture(scope="module")
+def vo():
+    from oracle import vo as v
+
+    return v
+
+
+@pytest.fixture(scope="module")
+def mix(synth):
+    """Twelve pairs of the uneven mix, 1-12 s, both modes."""
+    pairs = [synth.mixed_pair(seed, max_duration=12.0) for seed in range(12)]
+    modes = {p[3] for p in pairs}
+    assert modes == {0, 1}, modes
+    return pairs
+
+
+def _same(a, b):
+    assert a["moslqo"] == b["moslqo"]
+    assert a["vnsim"] == b["vnsim"]
+    assert a["alignment_lag_s"] == b["alignment_lag_s"]
+    for k in ("fvnsim", "fvnsim10", "fstdnsim", "fvdegenergy", "center_freq_bands"):
+        assert np.array_equal(a[k], b[k]), k
+    assert len(a["patch_sims"]) == len(b["patch_sims"])
+    for p, q in zip(a["patch_sims"], b["patch_sims"]):
+        for k in ("similarity", "ref_patch_start_time", "ref_patch_end_time", "deg_patch_start_time",
+                  "deg_patch_end_time"):
+            assert p[k] == q[k], k
+        for k in ("freq_band_means", "freq_band_stddevs", "freq_band_deg_energy"):
+            assert np.array_equal(p[k], q[k]), k
+
+
+def test_uneven_mixed_batch_exact_matches_oracle(native, vo, mix):
+    """One batch per mode over pairs of uneven length: bit-identical to the
+    oracle, pair by pair."""
+    for mode, sr in ((0, 48000), (1, 16000)):
+        sel = [p for p in mix if p[3] == mode]
+        say(f"uneven exact batch mode={mode} n={len(sel)} lens={[p[0].size for p in sel]}")
+        ctx = native.Context(mode, precision="exact")
+        rc, res, st = ctx.measure_batch([p[0] for p in sel], [p[1] for p in sel], sr)
+        orc = vo.Oracle(mode=mode)
+        orc_rc, ores, ost = orc.measure_batch([p[0] for p in sel], [p[1] for p in sel], sr)
+        assert list(st) == list(ost)
+        for r, o in zip(res, ores):
+            if o is None:
+                assert r is None
+                continue
+            _same(r, o)
+
+
+def _serve(native, pairs, **cfg):
+    with native.Server(precision="fast", **cfg) as srv:
+        tickets = [None] * len(pairs)
+
+        def submitter(idx):
+            for i in idx:
+                ref, deg, sr, mode = pairs[i]
+                tickets[i] = srv.submit(mode, ref, deg, sr)
+
+        # two submitting threads interleave the requests
+        ts = [threading.Thread(target=submitter, args=(range(k, len(pairs), 2),)) for k in range(2)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        out = [srv.wait(t) for t in tickets]
+        stats = srv.stats()
+    return out, stats
+
+
+def test_server_results_equal_alone(native, mix):
+    alone = []
+    ctxs = {0: native.Context(0, precision="fast"), 1: native.Context(1, precision="fast")}
+    for ref, deg, sr, mode in mix:
+        alone.append(ctxs[mode].measure(ref, deg, sr))
+    say("serving the mix with 3 workers, batches of up to 4")
+    out, stats = _serve(native, mix, workers=3, max_batch=4)
+    say("server stats", stats)
+    assert stats["pairs"] == len(mix) and stats["inflight_bytes"] == 0
+    for (rc, res, lat), a in zip(out, alone):
+        assert rc == 0 and lat > 0
+        _same(res, a)
+
+
+def test_server_tight_admission(native, mix):
+    """A one-byte footprint budget admits one request at a time; results are
+    unchanged."""
+    ctx = native.Context(0, precision="fast")
+    sel = [p for p in mix if p[3] == 0][:4]
+    alone = [ctx.measure(p[0], p[1], p[2]) for p in sel]
+    say("serving with a one-byte admission budget")
+    out, stats = _serve(native, sel, workers=2, max_batch=8, max_inflight_bytes=1)
+    assert stats["batches"] == len(sel)  # never more than one request in flight
+    for (rc, res, lat), a in zip(out, alone):
+        assert rc == 0
+        _same(res, a)
+
+
+def test_server_errors(native, synth):
+    ref, deg = synth.audio_pair(3, duration=2.0)
+    with native.Server(precision="fast", workers=1) as srv:
+        t_bad = srv.submit(0, ref[:100], deg[:100], 48000)  # shorter than one window
+        t_ok = srv.submit(0, ref, deg, 48000)
+        rc, res, _ = srv.wait(t_bad)
+        assert rc == native.E_SIGNAL and res is None
+        rc, res, _ = srv.wait(t_ok)
+        assert rc == 0 and 1.0 <= res["moslqo"] <= 5.0
+        rc, res, _ = srv.wait(987654)  # never issued
+        assert rc == native.E_INVALID_ARGUMENT and res is None
