@@ -16,6 +16,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 from concurrent.futures import ProcessPoolExecutor
 from pathlib import Path
@@ -40,6 +41,50 @@ def make_pairs(n: int, workers: int):
         return list(ex.map(_gen, range(n), chunksize=4))
 
 
+def _gen_mixed(seed: int):
+    from akvisqol_amd import synth
+
+    return synth.mixed_pair(seed, max_duration=120.0)
+
+
+def serve_level(srv, pool, qps: float, seconds: float, seed: int, submitters: int = 4):
+    """Poisson arrivals at `qps` for `seconds`, each request a pool pair drawn
+    uniformly; latency is the server's submit-to-completion time."""
+    rng = np.random.default_rng(seed)
+    n = max(1, int(round(qps * seconds)))
+    times = np.cumsum(rng.exponential(1.0 / qps, n))
+    picks = rng.integers(0, len(pool), n)
+    tickets = [None] * n
+    submitted = [threading.Event() for _ in range(n)]
+    start = time.perf_counter() + 0.05
+
+    def submitter(k):
+        for i in range(k, n, submitters):
+            delay = start + times[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ref, deg, sr, mode = pool[picks[i]]
+            tickets[i] = srv.submit(mode, ref, deg, sr)
+            submitted[i].set()
+
+    threads = [threading.Thread(target=submitter, args=(k,)) for k in range(submitters)]
+    for t in threads:
+        t.start()
+    lat = np.zeros(n)
+    mos = np.full(n, np.nan)
+    for i in range(n):
+        submitted[i].wait()
+        rc, m, lat[i] = srv.wait(tickets[i], convert=False)
+        if rc == 0:
+            mos[i] = m
+    end = time.perf_counter()
+    for t in threads:
+        t.join()
+    return dict(offered_qps=qps, seconds=seconds, requests=n, pairs_per_s=round(n / (end - start), 2),
+                p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
+                failed=int(np.sum(np.isnan(mos)))), picks, mos
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +94,9 @@ def main() -> int:
     ap.add_argument("--exact-pairs", type=int, default=8, help="pairs timed on the per-pair exact f64 path")
     ap.add_argument("--cpu-sample", type=int, default=0, help="oracle sample size (0: 2 per host thread, <= batch)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--qps", default="200,500,1000", help="serving loads (requests/s); empty: no serving run")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--pool", type=int, default=64, help="distinct pairs of the 1-120 s mix drawn by requests")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (pairs would shard "
@@ -97,6 +145,47 @@ def main() -> int:
     exact_mos = [ex.measure(pairs[i][0], pairs[i][1], 48000)["moslqo"] for i in range(ne)]
     exact_pps = ne / (time.perf_counter() - t)
 
+    serving = None
+    qps_levels = [float(q) for q in args.qps.split(",") if q.strip()]
+    if qps_levels:
+        # Serving: Poisson arrivals of pairs from the uneven mix (70% 48 kHz
+        # audio, 30% 16 kHz speech, 1-120 s log-uniform, noise, delay,
+        # stretch, dropouts), continuous batching on one device.
+        t = time.perf_counter()
+        with ProcessPoolExecutor(max_workers=workers) as ex:
+            pool = list(ex.map(_gen_mixed, range(args.pool)))
+        secs = float(sum(p[0].size / p[2] for p in pool))
+        print(f"bench: serving pool of {len(pool)} pairs ({secs / len(pool):.1f} s mean) in "
+              f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+        levels = []
+        served_mos = {}
+        with native.Server(precision="fast") as srv:
+            serve_level(srv, pool, 100.0, 1.0, seed=99)  # warm the workers' engines
+            for k, q in enumerate(qps_levels):
+                rep, picks, mos = serve_level(srv, pool, q, args.serve_seconds, seed=1000 + k)
+                levels.append(rep)
+                for pi, m in zip(picks, mos):
+                    served_mos.setdefault(int(pi), m)
+                print(f"bench: serving {rep}", file=sys.stderr, flush=True)
+            stats = srv.stats()
+        # the plain f64 per-pair path: service time per request, one at a time
+        exc = {0: native.Context(0, precision="exact"), 1: native.Context(1, precision="exact")}
+        svc = []
+        for ref, deg, sr, mode in pool[:8]:
+            t = time.perf_counter()
+            exc[mode].measure(ref, deg, sr)
+            svc.append((time.perf_counter() - t) * 1e3)
+        serving = {"workload": f"Poisson arrivals, requests drawn from {len(pool)} pairs of the uneven mix "
+                               "(70% 48 kHz audio / 30% 16 kHz speech, durations log-uniform 1-120 s, "
+                               f"mean {secs / len(pool):.1f} s)",
+                   "precision": "fast", "levels": levels,
+                   "batches": stats["batches"], "pairs": stats["pairs"],
+                   "exact_per_pair": {"p50_ms": round(float(np.percentile(svc, 50)), 2),
+                                      "p99_ms": round(float(np.percentile(svc, 99)), 2),
+                                      "capacity_pairs_per_s": round(1e3 * len(svc) / float(np.sum(svc)), 2),
+                                      "note": "service time of one request on an idle device (first 8 pool pairs); "
+                                              "offered loads above capacity queue without bound"}}
+
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -129,6 +218,8 @@ def main() -> int:
                             "flop_per_launch": prof["flop"] / prof["launches"],
                             "algorithmic_bytes_per_launch": prof["bytes"] / prof["launches"]}
 
+    if serving is not None:
+        line["serving"] = serving
     if not args.no_cpu:
         from oracle import vo
 
@@ -145,6 +236,21 @@ def main() -> int:
         line["cpu_baseline"] = {"value": round(ns / cpu_s, 3), "unit": "pairs/s", "cores": threads, "kind": "port",
                                 "sample": f"first {ns} pairs of the batch, C f64 oracle, one pair per thread"}
         line["max_abs_dmos_vs_oracle"] = dmos
+        if serving is not None and served_mos:
+            # served results are batch-independent: check 4 served pool pairs
+            # at length quantiles 0, 25, 50 and 75 %
+            by_len = sorted((k for k in served_mos if not np.isnan(served_mos[k])), key=lambda k: pool[k][0].size)
+            keys = sorted({by_len[int(f * (len(by_len) - 1))] for f in (0.0, 0.25, 0.5, 0.75)}) if by_len else []
+            sd = []
+            for mode in (0, 1):
+                sel = [k for k in keys if pool[k][3] == mode]
+                if not sel:
+                    continue
+                rc2, ores2, _ = vo.Oracle(mode=mode).measure_batch([pool[k][0] for k in sel], [pool[k][1] for k in sel],
+                                                                     pool[sel[0]][2], num_threads=threads)
+                sd += [abs(o["moslqo"] - served_mos[k]) for k, o in zip(sel, ores2) if o is not None]
+            serving["max_abs_dmos_vs_oracle"] = float(max(sd)) if sd else None
+            serving["oracle_checked_pairs"] = len(sd)
         line["max_abs_dmos_exact_vs_oracle"] = emos
         line["dmos_ok"] = dmos <= 1e-6
     print(json.dumps(line), flush=True)
