@@ -41,6 +41,17 @@ def make_pairs(n: int, workers: int):
         return list(ex.map(_gen, range(n), chunksize=4))
 
 
+def _traffic_ratio(path: Path):
+    """`traffic_per_algorithmic_byte = X` from a counter summary, or None."""
+    try:
+        for ln in path.read_text().splitlines():
+            if ln.startswith("traffic_per_algorithmic_byte"):
+                return float(ln.split("=")[1])
+    except OSError:
+        return None
+    return None
+
+
 def _gen_mixed(seed: int):
     from akvisqol_amd import synth
 
@@ -97,7 +108,19 @@ def main() -> int:
     ap.add_argument("--qps", default="200,500,1000", help="serving loads (requests/s); empty: no serving run")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--pool", type=int, default=64, help="distinct pairs of the 1-120 s mix drawn by requests")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer three times (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        src = torch.ones(args.copy_probe << 17, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"bench: copied {args.copy_probe} MiB three times", file=sys.stderr, flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (pairs would shard "
               "across GPUs with no exchange between them)", file=sys.stderr)
@@ -212,11 +235,21 @@ def main() -> int:
     if prof["launches"] and prof["ms"] > 0:
         ms_per_launch = prof["ms"] / prof["launches"]
         achieved = prof["flop"] / (prof["ms"] * 1e-3) / 1e12
+        alg_bytes = prof["bytes"] / prof["launches"]
+        # HBM bytes per algorithmic byte of this kernel on this workload, from
+        # the committed FETCH_SIZE / WRITE_SIZE counter runs (copy-calibrated)
+        traffic = None
+        ratio = _traffic_ratio(ROOT / "profiles" / "k_gammatone_fast_1024x10s48k_hbm_traffic.txt")
+        if ratio is not None and args.batch == 1024:
+            traffic = round(ratio * alg_bytes)
         line["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
                             "kernel": "k_gammatone<fast>", "ms_per_launch": round(ms_per_launch, 3),
                             "flop_per_launch": prof["flop"] / prof["launches"],
-                            "algorithmic_bytes_per_launch": prof["bytes"] / prof["launches"]}
+                            "algorithmic_bytes_per_launch": alg_bytes,
+                            # 30 flop per (sample, band) issue as 13 FMA + 4 mul: the FP64
+                            # instruction-issue ceiling is peak / 2 lane-ops per second
+                            "issue_frac": round(achieved * (17.0 / 30.0) / (FP64_PEAK_TFLOPS / 2.0), 4)}
 
     if serving is not None:
         line["serving"] = serving
