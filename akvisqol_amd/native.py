@@ -332,8 +332,10 @@ class Context:
             _check(rc)
         return {k: buf[i] for i, k in enumerate(self.PHASES)}
 
-    def measure_batch(self, refs, degs, sample_rate: int, num_threads: int = 0):
-        """(first error code, [result dict or None], [status]) as _native.py:215-225."""
+    def measure_batch(self, refs, degs, sample_rate: int, num_threads: int = 0, convert: bool = True):
+        """(first error code, [result dict or None], [status]) as _native.py:215-225.
+
+        convert=False returns only the MOS values (NaN for a failed pair)."""
         refs = [_f64(r) for r in refs]
         degs = [_f64(d) for d in degs]
         n = len(refs)
@@ -346,6 +348,13 @@ class Context:
         st = (c_int32 * n)()
         with self._lock:
             rc = lib.visqol_measure_batch(self._h, n, rp, rl, dp, dl, sample_rate, 0, out, st)
+        if not convert:
+            mos = np.full(n, np.nan)
+            for i in range(n):
+                if out[i]:
+                    mos[i] = out[i].contents.moslqo
+                    lib.visqol_result_free(out[i])
+            return rc, mos, list(st)
         results = []
         for i in range(n):
             if out[i]:

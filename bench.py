@@ -160,6 +160,12 @@ def main() -> int:
     step_s = float(np.mean(times))
     value = args.batch / step_s
 
+    # The same batch handed over as host arrays (visqol_measure_batch): the
+    # PCIe-inclusive rate of the plain C entry point, one step.
+    t = time.perf_counter()
+    rc_h, _, _ = ctx.measure_batch([p[0] for p in pairs], [p[1] for p in pairs], 48000, convert=False)
+    host_pps = args.batch / (time.perf_counter() - t) if rc_h == 0 else None
+
     # Plain per-pair f64 path (exact precision, one pair per call).
     ex = native.Context(native.MODE_AUDIO, precision="exact")
     ne = min(args.exact_pairs, args.batch)
@@ -228,6 +234,7 @@ def main() -> int:
                    "batch": args.batch, "pair_seconds": 10.0, "sample_rate": 48000, "bands": 32,
                    "precision": "fast"},
         "audio_seconds_per_s": round(value * 10.0, 2),
+        "host_buffers_pairs_per_s": round(host_pps, 3) if host_pps else None,
         "exact_per_pair_pairs_per_s": round(exact_pps, 3),
         "speedup_vs_exact_per_pair": round(value / exact_pps, 2),
         "max_abs_dmos_fast_vs_exact_gpu": float(np.max(np.abs(fast_mos[:ne] - np.array(exact_mos)))),
