@@ -25,6 +25,10 @@ struct visqol_ctx {
     std::unique_ptr<Engine> engine;
 };
 
+struct visqol_hip_prepared {
+    std::unique_ptr<PreparedSet> set;
+};
+
 namespace {
 thread_local char g_error[512] = {0};
 }  // namespace
@@ -475,6 +479,66 @@ int visqol_hip_measure_batch_device(visqol_ctx* c, size_t n, const double* const
     }
     return finish_batch(res, out, status);
 }
+
+int visqol_hip_prepare(visqol_ctx* c, size_t n, const double* const* refs, const size_t* ref_lens,
+                       uint32_t sample_rate, visqol_hip_prepared** out) {
+    if (!c || !out) return invalid("NULL argument");
+    *out = nullptr;
+    if (n == 0 || !refs || !ref_lens) return invalid("no references");
+    if (sample_rate == 0) return invalid("InvalidInput");
+    for (size_t i = 0; i < n; i++)
+        if (!refs[i] && ref_lens[i]) return invalid("NULL reference with nonzero length");
+    auto* p = new (std::nothrow) visqol_hip_prepared();
+    if (!p) {
+        set_error("OutOfMemory");
+        return VISQOL_E_OUT_OF_MEMORY;
+    }
+    std::vector<long long> lens(n);
+    for (size_t i = 0; i < n; i++) lens[i] = (long long)ref_lens[i];
+    try {
+        p->set = c->engine->prepare(refs, lens.data(), (int)n, sample_rate);
+    } catch (const BackendError& e) {
+        delete p;
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        delete p;
+        set_error("OutOfMemory");
+        return VISQOL_E_OUT_OF_MEMORY;
+    }
+    *out = p;
+    return VISQOL_OK;
+}
+
+int visqol_hip_measure_prepared(visqol_ctx* c, const visqol_hip_prepared* prep, size_t n, const size_t* ref_index,
+                                const double* const* degs, const size_t* deg_lens, visqol_result** out,
+                                int32_t* status) {
+    if (!c || !prep || !prep->set) return invalid("NULL argument");
+    if (n == 0) return VISQOL_OK;
+    if (!out || !degs || !deg_lens) return invalid("NULL argument");
+    const PreparedSet& ps = *prep->set;
+    if (ps.mode != c->opts.mode || ps.fast != c->opts.fast)
+        return invalid("prepared references were made by a context of another mode or precision");
+    std::vector<PairInput> pairs(n);
+    for (size_t i = 0; i < n; i++) {
+        const size_t k = ref_index ? ref_index[i] : i;
+        if (k >= ps.refs.size()) return invalid("reference index out of range");
+        if (!degs[i] && deg_lens[i]) return invalid("NULL signal with nonzero length");
+        pairs[i] = PairInput{nullptr, 0, degs[i], (long long)deg_lens[i], false, &ps.refs[k]};
+    }
+    std::vector<PairResult> res;
+    const int rc = run_pairs(c, pairs.data(), n, ps.sample_rate, res);
+    if (rc != VISQOL_OK) {
+        for (size_t i = 0; i < n; i++) {
+            out[i] = nullptr;
+            if (status) status[i] = rc;
+        }
+        return rc;
+    }
+    return finish_batch(res, out, status);
+}
+
+void visqol_hip_prepared_free(visqol_hip_prepared* p) { delete p; }
 
 int visqol_hip_profile(visqol_ctx* c, int enable, int reset) {
     if (!c) return invalid("ctx is NULL");

@@ -226,6 +226,7 @@ namespace {
 
 struct PairState {
     bool live = true;
+    const PreparedRef* prep = nullptr;
     const double* dref = nullptr;
     const double* ddeg = nullptr;
     long long rl = 0, dl = 0;
@@ -280,35 +281,58 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     for (int i = 0; i < n; i++) {
         out[i] = PairResult();
         PairState& s = st[i];
-        s.rl = pairs[i].ref_len;
+        s.prep = pairs[i].prep;
+        s.rl = s.prep ? s.prep->len : pairs[i].ref_len;
         s.dl = pairs[i].deg_len;
-        if (pairs[i].on_device) {
+        if (s.prep) {
+            s.dref = s.prep->d_signal;
+        } else if (pairs[i].on_device) {
             s.dref = pairs[i].ref;
-            s.ddeg = pairs[i].deg;
         } else {
             double* r = (double*)arena_.alloc(sizeof(double) * std::max(1LL, s.rl));
-            double* d = (double*)arena_.alloc(sizeof(double) * std::max(1LL, s.dl));
             if (s.rl) check(hipMemcpyAsync(r, pairs[i].ref, sizeof(double) * s.rl, hipMemcpyHostToDevice, stream_), "upload");
-            if (s.dl) check(hipMemcpyAsync(d, pairs[i].deg, sizeof(double) * s.dl, hipMemcpyHostToDevice, stream_), "upload");
             s.dref = r;
+        }
+        if (pairs[i].on_device) {
+            s.ddeg = pairs[i].deg;
+        } else {
+            double* d = (double*)arena_.alloc(sizeof(double) * std::max(1LL, s.dl));
+            if (s.dl) check(hipMemcpyAsync(d, pairs[i].deg, sizeof(double) * s.dl, hipMemcpyHostToDevice, stream_), "upload");
             s.ddeg = d;
         }
     }
-    double* d_sums = (double*)arena_.alloc(sizeof(double) * 6 * n);
-    check(hipMemsetAsync(d_sums, 0, sizeof(double) * 6 * n, stream_), "memset");
+    // Slots per pair: Σref, Σdeg, Nyquist ref, Nyquist deg, ref energy,
+    // aligned deg energy; a prepared reference's three come precomputed.
+    bool any_prep = false;
+    for (int i = 0; i < n; i++) any_prep = any_prep || st[i].prep;
+    double* d_sums = nullptr;
+    if (any_prep) {
+        std::vector<double> init(6 * n, 0.0);
+        for (int i = 0; i < n; i++)
+            if (st[i].prep) {
+                init[6 * i] = st[i].prep->sum;
+                init[6 * i + 2] = st[i].prep->nyq;
+                init[6 * i + 4] = st[i].prep->energy;
+            }
+        d_sums = upload(init);
+    } else {
+        d_sums = (double*)arena_.alloc(sizeof(double) * 6 * n);
+        check(hipMemsetAsync(d_sums, 0, sizeof(double) * 6 * n, stream_), "memset");
+    }
     std::vector<SumJob> j1, j2, j3;
     std::vector<LagJob> lj;
     for (int i = 0; i < n; i++) {
         PairState& s = st[i];
         const bool global = !opts_.no_global && s.rl > 0 && s.dl > 0;
+        const bool pr = s.prep != nullptr;  // reference slots already filled
         if (global && opts_.fast) {
             // one pass per signal: sum, Nyquist sum, and the reference energy
-            j1.push_back(SumJob{s.dref, s.rl, kMoments, 6 * i, 6 * i + 2, 6 * i + 4, nullptr, 0});
+            if (!pr) j1.push_back(SumJob{s.dref, s.rl, kMoments, 6 * i, 6 * i + 2, 6 * i + 4, nullptr, 0});
             j1.push_back(SumJob{s.ddeg, s.dl, kMoments, 6 * i + 1, 6 * i + 3, -1, nullptr, 0});
         } else if (global) {
-            j1.push_back(SumJob{s.dref, s.rl, kSum, 6 * i, 0, 0, nullptr, 0});
+            if (!pr) j1.push_back(SumJob{s.dref, s.rl, kSum, 6 * i, 0, 0, nullptr, 0});
             j1.push_back(SumJob{s.ddeg, s.dl, kSum, 6 * i + 1, 0, 0, nullptr, 0});
-            j2.push_back(SumJob{s.dref, s.rl, kNyquist, 6 * i + 2, 6 * i, 0, nullptr, 0});
+            if (!pr) j2.push_back(SumJob{s.dref, s.rl, kNyquist, 6 * i + 2, 6 * i, 0, nullptr, 0});
             j2.push_back(SumJob{s.ddeg, s.dl, kNyquist, 6 * i + 3, 6 * i + 1, 0, nullptr, 0});
         }
         if (global) {
@@ -317,7 +341,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             s.lagjob = (int)lj.size();
             lj.push_back(LagJob{s.dref, s.ddeg, s.rl, s.dl, z, nfft, 6 * i, 6 * i + 1, 6 * i + 2, 6 * i + 3});
         }
-        if (!(global && opts_.fast)) j1.push_back(SumJob{s.dref, s.rl, kSumSq, 6 * i + 4, 0, 0, nullptr, 0});
+        if (!(global && opts_.fast) && !pr) j1.push_back(SumJob{s.dref, s.rl, kSumSq, 6 * i + 4, 0, 0, nullptr, 0});
     }
     sums(j1, d_sums, serial);
     sums(j2, d_sums, serial);
@@ -336,7 +360,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     if (opts_.mode == 1) {
         for (int i = 0; i < n; i++) {
             PairState& s = st[i];
-            if (s.rl <= W) continue;
+            if (s.rl <= W || s.prep) continue;
             s.vp = vad_plan(num_frames(s.rl, W, hop), S, W);
             if (s.vp.n_chunks <= 0) continue;
             s.vadjob = (int)vj.size();
@@ -406,7 +430,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             if (!ref_patch_indices(s.nf_ref, S, s.idx)) { fail(r, s, -5, "ReferenceSpectrogramTooSmall"); continue; }
         } else {
             s.idx.clear();
-            if (s.vadjob >= 0) vad_patches(s.vp, h_active.data() + s.vad_off, S, s.idx);
+            if (s.prep) s.idx = s.prep->vad_idx;
+            else if (s.vadjob >= 0) vad_patches(s.vp, h_active.data() + s.vad_off, S, s.idx);
         }
         if (s.idx.empty()) { fail(r, s, -5, "NoPatches"); continue; }
         if (!plan_search(s.idx, s.nf_ref, s.nf_deg, S, opts_.radius, s.np, s.lo, s.hi)) {
@@ -423,6 +448,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     // ------------------------------------------------------------ phase B
     std::vector<ScaleJob> scj;
     std::vector<GtJob> gj;
+    std::vector<FrameCopyJob> fcj;  // prepared references' raw spectrograms
     std::vector<PrepJob> pj;
     std::vector<PlaneJob> plj;
     std::vector<SimJob> sj;
@@ -481,7 +507,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         max_len = std::max(max_len, s.dl2);
         s.spec_ref = d_spec + spec_off[i];
         s.spec_deg = s.spec_ref + (size_t)nb * s.nf_ref;
-        gj.push_back(GtJob{s.dref, s.spec_ref, s.nf_ref, 0});
+        if (s.prep) fcj.push_back(FrameCopyJob{s.spec_ref, s.prep->d_spec, s.nf_ref, s.nf_ref});
+        else gj.push_back(GtJob{s.dref, s.spec_ref, s.nf_ref, 0});
         gj.push_back(GtJob{s.degs, s.spec_deg, s.nf_deg, 0});
         max_nf = std::max(max_nf, std::max(s.nf_ref, s.nf_deg));
         if (d_raw) {
@@ -523,6 +550,11 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                 prof_.gammatone_bytes += 8.0 * ((double)(g.nf - 1) * hop + W) + 8.0 * nb * g.nf;
             }
             prof_.launches++;
+        }
+        if (!fcj.empty()) {
+            int fmax = 0;
+            for (const auto& f : fcj) fmax = std::max(fmax, f.nf);
+            launch_frame_copies(upload(fcj), (int)fcj.size(), nb, fmax, stream_);
         }
         if (d_raw && spec_total)
             check(hipMemcpyAsync(d_raw, d_spec, sizeof(double) * spec_total, hipMemcpyDeviceToDevice, stream_), "copy");
@@ -907,6 +939,112 @@ void Engine::finish(PairResult& r, const RateParams& rp) {
     }
     if (r.vnsim < 0.15) mos = 1.0;
     r.moslqo = mos;
+}
+
+// ---------------------------------------------------------------- prepared references
+PreparedSet::~PreparedSet() {
+    if (d_block) (void)hipFree(d_block);
+}
+
+// Per reference, the values an unprepared pair computes for it, by the same
+// kernels with the same per-signal work split: Σx, the Nyquist sum and the
+// energy (exact: serial; fast: the one-pass moments when global alignment
+// runs, else the Σx² tree), the raw spectrogram, and in speech mode the VAD
+// patch indices.
+std::unique_ptr<PreparedSet> Engine::prepare(const double* const* refs, const long long* lens, int n,
+                                             uint32_t sr) {
+    arena_.reset();
+    pinned_.reset();
+    DevRate& dr = dev_rate(sr);
+    const RateParams& rp = dr.p;
+    const int nb = rp.nb, W = rp.window, hop = rp.hop, S = rp.patch;
+    auto set = std::make_unique<PreparedSet>();
+    set->sample_rate = sr;
+    set->mode = opts_.mode;
+    set->fast = opts_.fast;
+    set->refs.resize(n);
+    std::vector<size_t> sig_off(n), spec_off(n);
+    size_t total = 0;
+    for (int i = 0; i < n; i++) {
+        PreparedRef& r = set->refs[i];
+        r.len = lens[i];
+        r.nf = r.len > W ? num_frames(r.len, W, hop) : 0;
+        sig_off[i] = total;
+        total += ((size_t)std::max(1LL, r.len) * sizeof(double) + 255) & ~(size_t)255;
+        spec_off[i] = total;
+        total += ((size_t)nb * r.nf * sizeof(double) + 255) & ~(size_t)255;
+    }
+    if (hipMalloc(&set->d_block, std::max<size_t>(total, 256)) != hipSuccess) {
+        (void)hipGetLastError();
+        throw BackendError{-2, "device out of memory"};
+    }
+    char* base = (char*)set->d_block;
+    std::vector<SumJob> s1, s2;
+    std::vector<GtJob> gj;
+    std::vector<VadJob> vj;
+    std::vector<int> vad_of(n, -1);
+    std::vector<VadPlan> vps(n);
+    std::vector<size_t> vad_off(n, 0);
+    size_t vad_bytes = 0;
+    int max_nf = 0, max_chunks = 0;
+    for (int i = 0; i < n; i++) {
+        PreparedRef& r = set->refs[i];
+        double* sig = (double*)(base + sig_off[i]);
+        r.d_signal = sig;
+        if (r.len)
+            check(hipMemcpyAsync(sig, refs[i], sizeof(double) * r.len, hipMemcpyHostToDevice, stream_), "upload");
+        if (r.len > 0) {
+            if (opts_.fast && !opts_.no_global) {
+                s1.push_back(SumJob{sig, r.len, kMoments, 3 * i, 3 * i + 1, 3 * i + 2, nullptr, 0});
+            } else {
+                s1.push_back(SumJob{sig, r.len, kSum, 3 * i, 0, 0, nullptr, 0});
+                s2.push_back(SumJob{sig, r.len, kNyquist, 3 * i + 1, 3 * i, 0, nullptr, 0});
+                s1.push_back(SumJob{sig, r.len, kSumSq, 3 * i + 2, 0, 0, nullptr, 0});
+            }
+        }
+        if (r.nf > 0) {
+            r.d_spec = (const double*)(base + spec_off[i]);
+            gj.push_back(GtJob{sig, (double*)(base + spec_off[i]), r.nf, 0});
+            max_nf = std::max(max_nf, r.nf);
+            if (opts_.mode == 1) {
+                vps[i] = vad_plan(r.nf, S, W);
+                if (vps[i].n_chunks > 0) {
+                    vad_of[i] = (int)vj.size();
+                    vad_off[i] = vad_bytes;
+                    vad_bytes += (size_t)vps[i].n_chunks;
+                    max_chunks = std::max(max_chunks, vps[i].n_chunks);
+                    vj.push_back(VadJob{sig, r.len, nullptr, vps[i].n_chunks, vps[i].frame_len, vps[i].first, 0});
+                }
+            }
+        }
+    }
+    double* d_out = (double*)arena_.alloc(sizeof(double) * 3 * std::max(1, n));
+    check(hipMemsetAsync(d_out, 0, sizeof(double) * 3 * std::max(1, n), stream_), "memset");
+    sums(s1, d_out, !opts_.fast);
+    sums(s2, d_out, !opts_.fast);
+    if (!gj.empty())
+        launch_gammatone(upload(gj), (int)gj.size(), max_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, stream_);
+    unsigned char* d_active = vad_bytes ? (unsigned char*)arena_.alloc(vad_bytes) : nullptr;
+    if (!vj.empty()) {
+        for (int i = 0; i < n; i++)
+            if (vad_of[i] >= 0) vj[vad_of[i]].active = d_active + vad_off[i];
+        auto* scratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * vj.size());
+        launch_vad(upload(vj), (int)vj.size(), max_chunks, scratch, stream_);
+    }
+    double* ps = (double*)pinned_.alloc(sizeof(double) * 3 * std::max(1, n));
+    unsigned char* pa = (unsigned char*)pinned_.alloc(vad_bytes + 1);
+    readback(ps, d_out, sizeof(double) * 3 * n);
+    readback(pa, d_active, vad_bytes);
+    sync();
+    for (int i = 0; i < n; i++) {
+        PreparedRef& r = set->refs[i];
+        r.sum = ps[3 * i];
+        r.nyq = ps[3 * i + 1];
+        r.energy = ps[3 * i + 2];
+        if (opts_.mode == 1 && vad_of[i] >= 0) vad_patches(vps[i], pa + vad_off[i], S, r.vad_idx);
+        r.vad_done = true;
+    }
+    return set;
 }
 
 // ---------------------------------------------------------------- stage access

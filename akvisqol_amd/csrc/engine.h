@@ -27,12 +27,34 @@ struct EngineOptions {
     bool fast = false;
 };
 
+// A reference prepared for scoring many degraded signals against it
+// (backend.zig:84-99, visqol.zig:194-244): what depends on the reference
+// alone, computed once by the same kernels as an unprepared pair.
+struct PreparedRef {
+    const double* d_signal = nullptr;  // device samples
+    long long len = 0;
+    const double* d_spec = nullptr;    // raw spectrogram [bands][nf] (before dB prep), or null if too short
+    int nf = 0;
+    double sum = 0, nyq = 0, energy = 0;  // Σx, Nyquist sum, Σx² (the pair path's slots 0, 2, 4)
+    bool vad_done = false;               // speech: patch indices from the VAD
+    std::vector<int> vad_idx;
+};
+struct PreparedSet {
+    uint32_t sample_rate = 0;
+    int mode = 0;
+    bool fast = false;
+    void* d_block = nullptr;  // one device allocation for every reference's samples and spectrogram
+    std::vector<PreparedRef> refs;
+    ~PreparedSet();
+};
+
 struct PairInput {
     const double* ref;
     long long ref_len;
     const double* deg;
     long long deg_len;
-    bool on_device;  // pointers are device memory
+    bool on_device;                      // pointers are device memory
+    const PreparedRef* prep = nullptr;   // ref/ref_len are then the prepared reference's
 };
 
 struct Match {
@@ -104,6 +126,10 @@ class Engine {
     // BackendError for device failures that poison the whole batch.
     void run(const PairInput* pairs, int n, uint32_t sample_rate, PairResult* out, bool want_stages);
     const EngineOptions& options() const { return opts_; }
+
+    // References of one sample rate prepared for run() (PairInput::prep);
+    // throws BackendError on device failure.
+    std::unique_ptr<PreparedSet> prepare(const double* const* refs, const long long* lens, int n, uint32_t sample_rate);
 
     // Stage access for checks.
     void spectrogram(uint32_t sample_rate, const double* host_sig, long long len, std::vector<double>& out, int& nf);

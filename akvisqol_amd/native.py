@@ -142,6 +142,10 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_math_probe": (c_int, [c_int, D, D, D, c_size_t]),
         "visqol_hip_fft": (c_int, [D, c_size_t, c_int]),
         "visqol_hip_xcorr": (c_int, [D, c_size_t, D, c_size_t, D, c_size_t, POINTER(c_size_t)]),
+        "visqol_hip_prepare": (c_int, [c_void_p, c_size_t, POINTER(D), S, c_uint32, POINTER(c_void_p)]),
+        "visqol_hip_measure_prepared": (c_int, [c_void_p, c_void_p, c_size_t, S, POINTER(D), S,
+                                                POINTER(POINTER(Result)), POINTER(c_int32)]),
+        "visqol_hip_prepared_free": (None, [c_void_p]),
         "visqol_hip_server_config_default": (None, [POINTER(ServerConfig)]),
         "visqol_hip_server_create": (c_int, [POINTER(Options), POINTER(ServerConfig), POINTER(c_void_p)]),
         "visqol_hip_server_submit": (c_int, [c_void_p, c_uint32, D, c_size_t, D, c_size_t, c_uint32,
@@ -322,6 +326,40 @@ class Context:
         _check(lib.visqol_hip_profile_sim(self._h, ctypes.byref(ms), ctypes.byref(cells), ctypes.byref(n)))
         return dict(ms=ms.value, cells=cells.value, launches=n.value)
 
+    def prepare(self, refs, sample_rate: int) -> "Prepared":
+        """visqol_hip_prepare: references kept on the device for measure_prepared."""
+        refs = [_f64(r) for r in refs]
+        n = len(refs)
+        D = POINTER(c_double)
+        rp = (D * n)(*[_dptr(r) for r in refs])
+        rl = (c_size_t * n)(*[r.size for r in refs])
+        h = c_void_p()
+        with self._lock:
+            _check(lib.visqol_hip_prepare(self._h, n, rp, rl, sample_rate, ctypes.byref(h)))
+        return Prepared(h, n)
+
+    def measure_prepared(self, prep: "Prepared", degs, ref_index=None, convert: bool = True):
+        """(first error code, [result dict or None], [status]): degs[i] against
+        reference ref_index[i] (default i)."""
+        degs = [_f64(d) for d in degs]
+        n = len(degs)
+        D = POINTER(c_double)
+        dp = (D * n)(*[_dptr(d) for d in degs])
+        dl = (c_size_t * n)(*[d.size for d in degs])
+        ri = None if ref_index is None else (c_size_t * n)(*[int(k) for k in ref_index])
+        out = (POINTER(Result) * n)()
+        st = (c_int32 * n)()
+        with self._lock:
+            rc = lib.visqol_hip_measure_prepared(self._h, prep._h, n, ri, dp, dl, out, st)
+        res = []
+        for i in range(n):
+            if out[i]:
+                res.append(result_dict(out[i].contents) if convert else out[i].contents.moslqo)
+                lib.visqol_result_free(out[i])
+            else:
+                res.append(None)
+        return rc, res, list(st)
+
     PHASES = ("phase_a", "plan", "phase_b", "match_stats", "match_decode", "realign_lags", "realign_stats", "aggregate")
 
     def profile_phases(self) -> dict:
@@ -419,6 +457,25 @@ def measure_batch_device(ctx: Context, refs: list[DeviceBuffer], degs: list[Devi
         else:
             results.append(None)
     return rc, results, list(st)
+
+
+class Prepared:
+    """visqol_hip_prepared: freed when collected or by free()."""
+
+    def __init__(self, h, n: int):
+        self._h = h
+        self.count = n
+
+    def free(self):
+        if self._h:
+            lib.visqol_hip_prepared_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Server:

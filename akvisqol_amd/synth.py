@@ -10,6 +10,8 @@ from a recorded PRNG (numpy PCG64) and a seed:
 * ``speech_pair``: 16 kHz harmonic bursts (f0 100-250 Hz, 12 harmonics,
   150-400 ms voiced, 50-300 ms silences); degraded by a 4th-order
   Butterworth low-pass at 4 kHz plus white noise.
+* ``plain_audio_pair``: the per-pair f64 check set: seeds 0-63, durations
+  uniform 5-15 s, SNR uniform 10-40 dB, delay uniform 0-2000 samples.
 * ``mixed_pair``: one pair of the uneven serving mix: 70 % audio / 30 %
   speech, log-uniform duration, noise 5-40 dB SNR, delay 0-4800 samples,
   0-3 % linear-resampling time stretch, 0-5 zeroed 20 ms dropouts.
@@ -105,6 +107,15 @@ def speech_pair(seed: int, duration: float = 8.0, snr_db: float = 20.0, delay: i
     deg = _butter_lowpass(_delay(ref, delay), 4000.0, sample_rate)
     deg = _add_noise(rng, deg, snr_db)
     return ref, deg
+
+
+def plain_audio_pair(seed: int):
+    """(ref, deg) of the per-pair f64 check set, parameters drawn from the seed."""
+    rng = np.random.default_rng(20_000_000 + seed)
+    dur = float(rng.uniform(5.0, 15.0))
+    snr = float(rng.uniform(10.0, 40.0))
+    delay = int(rng.integers(0, 2001))
+    return audio_pair(seed, duration=dur, snr_db=snr, delay=delay)
 
 
 def _stretch(x: np.ndarray, factor: float) -> np.ndarray:

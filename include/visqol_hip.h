@@ -65,6 +65,25 @@ int visqol_hip_profile_sim(visqol_ctx *ctx, double *ms, double *cells, int64_t *
  * Returns the number of stretches available (8). */
 int visqol_hip_profile_phases(visqol_ctx *ctx, double *ms, int n);
 
+/* References prepared once for scoring many degraded signals against them
+ * (backend.zig:84-99 prepare / measurePrepared; visqol.zig:194-244
+ * PreparedRef): each reference's samples and raw spectrogram stay in device
+ * memory with its sums, energy and (speech) VAD patch indices, computed by
+ * the same kernels as an unprepared pair, so scores are identical to
+ * visqol_measure_batch on the same pairs while the reference's spectrogram
+ * is not recomputed.  visqol_hip_measure_prepared scores degs[i] (host
+ * arrays) against reference ref_index[i] (ref_index NULL: reference i) at
+ * the prepared sample rate, with visqol_measure_batch's per-pair status and
+ * return contract.  The context must have the preparing context's mode and
+ * precision. */
+typedef struct visqol_hip_prepared visqol_hip_prepared;
+int visqol_hip_prepare(visqol_ctx *ctx, size_t n, const double *const *refs, const size_t *ref_lens,
+                       uint32_t sample_rate, visqol_hip_prepared **out);
+int visqol_hip_measure_prepared(visqol_ctx *ctx, const visqol_hip_prepared *prep, size_t n,
+                                const size_t *ref_index, const double *const *degs, const size_t *deg_lens,
+                                visqol_result **out, int32_t *status);
+void visqol_hip_prepared_free(visqol_hip_prepared *prep);
+
 /* Continuous-batching server (no counterpart in the reference's C ABI; a
  * serving front end for callers with a stream of requests, in the manner of
  * the reference's batch lanes, backend.zig:182-221).

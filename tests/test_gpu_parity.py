@@ -244,3 +244,37 @@ def test_fast_mode_close(native, vo, synth):
     ores, otr = vo.Oracle(mode=0).measure(ref, deg, 48000)
     say("fast mos", res["moslqo"], "oracle", ores["moslqo"], "diff", res["moslqo"] - ores["moslqo"])
     _compare(res, tr, ores, otr, exact=False, tol=FAST_MOS_TOL)
+
+
+# ---------------------------------------------------------------- per-pair f64 check set
+def _bitsame(r, o):
+    assert r["moslqo"] == o["moslqo"] and r["vnsim"] == o["vnsim"]
+    assert r["alignment_lag_s"] == o["alignment_lag_s"]
+    for k in ("fvnsim", "fvnsim10", "fstdnsim", "fvdegenergy"):
+        assert np.array_equal(r[k], o[k]), k
+    assert len(r["patch_sims"]) == len(o["patch_sims"])
+    for p, q in zip(r["patch_sims"], o["patch_sims"]):
+        for k in ("similarity", "ref_patch_start_time", "ref_patch_end_time", "deg_patch_start_time",
+                  "deg_patch_end_time"):
+            assert p[k] == q[k], k
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_plain_f64_loop_64_pairs(native, vo, synth, mode):
+    """The per-pair check set: 64 audio pairs (5-15 s, SNR 10-40 dB, delay
+    0-2000 samples) or 64 speech pairs (8 s, lattice, VAD), scored one at a
+    time on the exact path: each bit-identical to the oracle, so every
+    alignment (global lag, patch times, realignment) is identical."""
+    if mode == 0:
+        pairs = [synth.plain_audio_pair(s) for s in range(64)]
+        sr = 48000
+    else:
+        pairs = [synth.speech_pair(s) for s in range(64)]
+        sr = 16000
+    say(f"per-pair f64 loop mode={mode}: 64 pairs")
+    ctx = native.Context(mode, precision="exact")
+    got = [ctx.measure(r, d, sr) for r, d in pairs]
+    rc, want, st = vo.Oracle(mode=mode).measure_batch([p[0] for p in pairs], [p[1] for p in pairs], sr)
+    assert rc == 0
+    for g, w in zip(got, want):
+        _bitsame(g, w)

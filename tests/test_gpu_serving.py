@@ -144,3 +144,43 @@ def test_server_errors(native, synth):
         assert rc == 0 and 1.0 <= res["moslqo"] <= 5.0
         rc, res, _ = srv.wait(987654)  # never issued
         assert rc == native.E_INVALID_ARGUMENT and res is None
+
+
+@pytest.mark.parametrize("mode,precision", [(0, "fast"), (0, "exact"), (1, "fast"), (1, "exact")])
+def test_prepared_references_equal_batch(native, synth, mode, precision):
+    """Prepared references (spectrogram, sums, VAD kept on the device) give
+    the scores of the plain batch on the same pairs, several degraded
+    signals per reference."""
+    if mode == 0:
+        sr = 48000
+        refs, pairs = [], []
+        for k, seed in enumerate((11, 12, 13)):
+            dur = 3.0 + k
+            for j, (snr, delay) in enumerate(((15.0, 120), (30.0, 0))):
+                ref, deg = synth.audio_pair(seed, duration=dur, snr_db=snr, delay=delay)
+                if j == 0:
+                    refs.append(ref)
+                pairs.append((k, deg))
+    else:
+        sr = 16000
+        refs, pairs = [], []
+        for k, seed in enumerate((21, 22)):
+            for j, (snr, delay) in enumerate(((12.0, 40), (25.0, 0))):
+                ref, deg = synth.speech_pair(seed, duration=6.0, snr_db=snr, delay=delay)
+                if j == 0:
+                    refs.append(ref)
+                pairs.append((k, deg))
+    ctx = native.Context(mode, precision=precision)
+    say(f"prepared refs mode={mode} precision={precision}: {len(refs)} refs, {len(pairs)} degs")
+    prep = ctx.prepare(refs, sr)
+    rc, got, st = ctx.measure_prepared(prep, [d for _, d in pairs], ref_index=[k for k, _ in pairs])
+    rc2, want, st2 = ctx.measure_batch([refs[k] for k, _ in pairs], [d for _, d in pairs], sr)
+    assert rc == rc2 == 0 and list(st) == list(st2)
+    for g, w in zip(got, want):
+        _same(g, w)
+    other = native.Context(mode, precision="exact" if precision == "fast" else "fast")
+    rc, res, _ = other.measure_prepared(prep, [pairs[0][1]])
+    assert rc == native.E_INVALID_ARGUMENT
+    rc, res, _ = ctx.measure_prepared(prep, [pairs[0][1]], ref_index=[len(refs)])
+    assert rc == native.E_INVALID_ARGUMENT
+    prep.free()
