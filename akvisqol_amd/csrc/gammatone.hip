@@ -23,7 +23,7 @@ namespace vq {
 constexpr int kFramesPerWave = 64;
 constexpr int kChunk = 32;        // samples per staged chunk (LDS tile 64 x 33 doubles)
 constexpr int kWavesGt = 4;
-constexpr int kBandsPerWave = 2;  // two independent recurrences per lane
+constexpr int kBandsPerWave = 4;  // independent recurrences per lane
 constexpr int kBandsPerBlock = kWavesGt * kBandsPerWave;
 constexpr int kThreadsGt = 64 * kWavesGt;
 constexpr int kStage = kFramesPerWave * kChunk / kThreadsGt;  // tile elements staged per thread
