@@ -689,11 +689,15 @@ constexpr int kMaxLast = 10;    // stages of the contiguous last pass (2^10 <= k
 constexpr int kMinStrided = 4;  // strided passes: 4..6 stages, runs of 2 KiB..512 B
 constexpr int kMaxStrided = 6;
 
+constexpr int kMinLast = 7;
+
 // Stage counts per pass, the pass over the largest spans first.  Below one
-// full tile: as even as possible.  With full tiles the last pass works on
-// contiguous blocks and takes up to kMaxLast stages; the strided passes
-// before it take 4..6 stages (as even as possible, smaller first), so each
-// workgroup reads runs of at least 32 residues (512 B).
+// full tile: as even as possible.  With full tiles: the fewest passes such
+// that the strided passes take 4..6 stages each (each workgroup then reads
+// runs of at least 32 residues, 512 B) and the contiguous last pass 7..10;
+// within that, the strided passes take as many stages as they can (the
+// fused last pass costs more per stage), split as evenly as possible,
+// smaller first.
 int plan_passes(long long n, int* ks) {
     int L = 0;
     while ((1LL << L) < n) L++;
@@ -704,7 +708,9 @@ int plan_passes(long long n, int* ks) {
     }
     int ns = (L - kMaxLast + kMaxStrided - 1) / kMaxStrided;
     if (ns < 1) ns = 1;
-    const int last = L - kMinStrided * ns < kMaxLast ? L - kMinStrided * ns : kMaxLast;
+    int last = L - kMaxStrided * ns;
+    if (last < kMinLast) last = kMinLast;
+    if (L - last < kMinStrided * ns) last = L - kMinStrided * ns;
     const int T = L - last;
     for (int p = 0; p < ns; p++) ks[p] = T / ns + (p >= ns - T % ns ? 1 : 0);
     ks[ns] = last;
