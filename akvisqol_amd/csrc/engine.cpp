@@ -852,12 +852,13 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         }
     }
 
-    // Aggregation and MOS mapping per pair.  The SVR and exponential maps are
-    // pure functions of the pair's result, so larger batches spread them over
-    // a few host threads; the lattice interpreter runs on this thread.
+    // Aggregation and MOS mapping per pair.  Every map is a pure function of
+    // the pair's result (the lattice interpreter only reads the model's
+    // constant tensors and writes its own per-call values), so larger
+    // batches spread them over a few host threads.
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nthreads = (opts_.mode == 0 || !opts_.lattice) && n >= 16 ? (int)std::min<unsigned>({8u, hw, (unsigned)n / 8})
-                                                                        : 1;
+    const int per = opts_.mode == 1 && opts_.lattice ? 1 : 8;  // pairs per thread worth a thread
+    const int nthreads = n >= 2 * per ? (int)std::min<unsigned>({8u, hw, (unsigned)(n / per)}) : 1;
     if (nthreads > 1) {
         std::vector<std::thread> pool;
         for (int t = 0; t < nthreads; t++)

@@ -152,6 +152,24 @@ struct Fb {
     }
 };
 
+// Sequential fused dot product acc = fma(x[k], w[k], acc), k = 0..n-1: the
+// same single-rounding operations whether fmaf is the FMA instruction or the
+// C library's routine; the clone for FMA hardware keeps them inline.
+__attribute__((target("fma"))) float dot_fma_hw(const float* x, const float* w, size_t n) {
+    float acc = 0.0f;
+    for (size_t k = 0; k < n; k++) acc = __builtin_fmaf(x[k], w[k], acc);
+    return acc;
+}
+float dot_fma_lib(const float* x, const float* w, size_t n) {
+    float acc = 0.0f;
+    for (size_t k = 0; k < n; k++) acc = fmaf(x[k], w[k], acc);
+    return acc;
+}
+float dot_fma(const float* x, const float* w, size_t n) {
+    static const bool hw = __builtin_cpu_supports("fma");
+    return hw ? dot_fma_hw(x, w, n) : dot_fma_lib(x, w, n);
+}
+
 enum class Op { Add, Sub, Mul, Min, Max, Equal, Relu, Relu6, Abs, Cast, Concat, FullyConnected, BatchMatMul, Reshape,
                 ExpandDims, Split, Shape, Fill, Unknown };
 enum class Dt { None, F32, I32, Bool };
@@ -274,14 +292,38 @@ class LatticeModel {
         return -1;
     }
 
+    // release[i]: tensors whose last reader is node i (never the output);
+    // dropping them there keeps a call's working set small and its heap
+    // blocks reused instead of returned to the system and faulted back in.
+    std::vector<std::vector<int>> release;
+
+    void plan_release() {
+        std::vector<int> last(consts.size(), -1);
+        for (size_t i = 0; i < nodes.size(); i++)
+            for (int t : nodes[i].in)
+                if (t >= 0 && (size_t)t < last.size()) last[t] = (int)i;
+        release.assign(nodes.size(), {});
+        for (size_t t = 0; t < last.size(); t++)
+            if (last[t] >= 0 && (int)t != output) release[last[t]].push_back((int)t);
+    }
+
     void run(std::vector<Val>& v) const {
-        for (const Node& n : nodes) exec(n, v);
+        for (size_t i = 0; i < nodes.size(); i++) {
+            exec(nodes[i], v);
+            for (int t : release[i]) v[t] = Val{};
+        }
     }
 
    private:
-    static const Val& in(const std::vector<Val>& v, const Node& n, size_t k) {
-        if (k >= n.in.size() || n.in[k] < 0 || !v[n.in[k]].set) throw GraphError{};
-        return v[n.in[k]];
+    // Tensor k of node n: the value computed in this call, else the model's
+    // constant (the per-call table starts empty; constants are never copied).
+    const Val& in(const std::vector<Val>& v, const Node& n, size_t k) const {
+        if (k >= n.in.size() || n.in[k] < 0) throw GraphError{};
+        const Val& a = v[n.in[k]];
+        if (a.set) return a;
+        const Val& c = consts[n.in[k]];
+        if (!c.set) throw GraphError{};
+        return c;
     }
     static Val floats(std::vector<size_t> shape, std::vector<float> data) {
         Val r;
@@ -305,21 +347,38 @@ class LatticeModel {
                 const Val &x = in(v, n, 0), &y = in(v, n, 1);
                 if (x.dt != Dt::F32 || y.dt != Dt::F32) throw GraphError{};
                 auto shape = bshape(x.shape, y.shape);
-                auto ix = bindex(x.shape, shape), iy = bindex(y.shape, shape);
-                std::vector<float> o(ix.size());
-                for (size_t e = 0; e < o.size(); e++) {
-                    const float p = (*x.f)[ix[e]], q = (*y.f)[iy[e]];
-                    float z;
-                    switch (n.op) {
-                        case Op::Add: z = p + q; break;
-                        case Op::Sub: z = p - q; break;
-                        case Op::Mul: z = p * q; break;
-                        case Op::Min: z = fminf(p, q); break;
-                        default: z = fmaxf(p, q); break;
+                size_t total = 1;
+                for (size_t d : shape) total *= d;
+                std::vector<float> o(total);
+                auto apply = [&](auto px, auto qy) {
+                    for (size_t e = 0; e < total; e++) {
+                        const float p = px(e), q = qy(e);
+                        float z;
+                        switch (n.op) {
+                            case Op::Add: z = p + q; break;
+                            case Op::Sub: z = p - q; break;
+                            case Op::Mul: z = p * q; break;
+                            case Op::Min: z = fminf(p, q); break;
+                            default: z = fmaxf(p, q); break;
+                        }
+                        o[e] = act_apply(n.act, z);
                     }
-                    o[e] = act_apply(n.act, z);
+                };
+                const float* xf = x.f->data();
+                const float* yf = y.f->data();
+                const size_t xc = x.f->size(), yc = y.f->size();
+                // same element order without index maps when an operand is
+                // the whole output or a single value
+                auto dense = [&](const float* f) { return [f](size_t e) { return f[e]; }; };
+                auto scalar = [&](const float* f) { return [f](size_t) { return f[0]; }; };
+                if (xc == total && yc == total && x.shape == y.shape) apply(dense(xf), dense(yf));
+                else if (xc == total && x.shape == shape && yc == 1) apply(dense(xf), scalar(yf));
+                else if (yc == total && y.shape == shape && xc == 1) apply(scalar(xf), dense(yf));
+                else {
+                    auto ix = bindex(x.shape, shape), iy = bindex(y.shape, shape);
+                    apply([&](size_t e) { return xf[ix[e]]; }, [&](size_t e) { return yf[iy[e]]; });
                 }
-                r = floats(shape, std::move(o));
+                r = floats(std::move(shape), std::move(o));
                 break;
             }
             case Op::Equal: {
@@ -417,8 +476,7 @@ class LatticeModel {
                     for (size_t oo = 0; oo < nout; oo++) {
                         const float* xr = x.f->data() + rr * nin;
                         const float* wr = w.f->data() + oo * nin;
-                        float acc = 0.0f;
-                        for (size_t k = 0; k < nin; k++) acc = fmaf(xr[k], wr[k], acc);
+                        float acc = dot_fma(xr, wr, nin);
                         if (bias) acc += (*bias->f)[oo];
                         o[rr * nout + oo] = act_apply(n.act, acc);
                     }
@@ -448,6 +506,14 @@ class LatticeModel {
                 for (size_t bi = 0; bi < ix.size(); bi++) {
                     const float* xm = x.f->data() + ix[bi] * m * kk;
                     const float* ym = y.f->data() + iy[bi] * kk * nn;
+                    if (kk == 1) {
+                        // outer product: fma(p, q, +0) rounds p*q once and turns
+                        // a -0 product into +0, which is exactly p*q + 0.0f
+                        float* ob_ = o.data() + bi * m * nn;
+                        for (size_t a = 0; a < m; a++)
+                            for (size_t c = 0; c < nn; c++) ob_[a * nn + c] = xm[a] * ym[c] + 0.0f;
+                        continue;
+                    }
                     for (size_t a = 0; a < m; a++)
                         for (size_t c = 0; c < nn; c++) {
                             float acc = 0.0f;
@@ -558,7 +624,7 @@ class LatticeModel {
             }
             default: throw GraphError{};
         }
-        v[n.out[0]] = r;
+        v[n.out[0]] = std::move(r);
     }
 };
 
@@ -737,12 +803,13 @@ std::unique_ptr<LatticeModel, LatticeDeleter> load_lattice(const std::string& by
     m->tau = m->find(m->inputs, "tau");
     m->output = m->find(m->outputs, "predictions");
     if (m->tau < 0 || m->output < 0) { err = "MissingInput"; return nullptr; }
+    m->plan_release();
     return m;
 }
 
 bool lattice_predict(const LatticeModel& m, const double* f0, const double* f1, const double* f2, const double* f3,
                      double& out) {
-    std::vector<Val> v = m.consts;
+    std::vector<Val> v(m.consts.size());  // per-call values; constants are read in place
     const double* feats[4] = {f0, f1, f2, f3};
     for (int f = 0; f < 4; f++)
         for (int b = 0; b < m.nb; b++) {

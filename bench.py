@@ -58,6 +58,12 @@ def _gen_mixed(seed: int):
     return synth.mixed_pair(seed, max_duration=120.0)
 
 
+def _gen_mixed60(seed: int):
+    from akvisqol_amd import synth
+
+    return synth.mixed_pair(50_000 + seed, max_duration=60.0)
+
+
 def serve_level(srv, pool, qps: float, seconds: float, seed: int, submitters: int = 4):
     """Poisson arrivals at `qps` for `seconds`, each request a pool pair drawn
     uniformly; latency is the server's submit-to-completion time."""
@@ -107,6 +113,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--qps", default="200,500,1000", help="serving loads (requests/s); empty: no serving run")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--uneven", type=int, default=512, help="pairs of the 1-60 s mix scored as batches (0: skip)")
     ap.add_argument("--pool", type=int, default=64, help="distinct pairs of the 1-120 s mix drawn by requests")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer three times (calibrates HBM byte counters) and exit")
@@ -173,6 +180,44 @@ def main() -> int:
     t = time.perf_counter()
     exact_mos = [ex.measure(pairs[i][0], pairs[i][1], 48000)["moslqo"] for i in range(ne)]
     exact_pps = ne / (time.perf_counter() - t)
+
+    # Uneven batches: pairs of the 1-60 s mix (70% 48 kHz audio, 30% 16 kHz
+    # speech, per-pair noise, delay, stretch, dropouts), device-resident,
+    # scored as one batch per mode.
+    uneven = None
+    if args.uneven:
+        t = time.perf_counter()
+        with ProcessPoolExecutor(max_workers=workers) as ex:
+            mix = list(ex.map(_gen_mixed60, range(args.uneven)))
+        print(f"bench: generated {len(mix)} uneven pairs in {time.perf_counter() - t:.1f}s", file=sys.stderr,
+              flush=True)
+        groups = {}
+        for ref, deg, sr, mode in mix:
+            g = groups.setdefault((mode, sr), ([], []))
+            g[0].append(native.DeviceBuffer(ref))
+            g[1].append(native.DeviceBuffer(deg))
+        uctx = {m: native.Context(m, precision="fast") for m in (0, 1)}
+        for (mode, sr), (rb, db) in groups.items():  # warm the engines
+            native.measure_batch_device(uctx[mode], rb, db, sr, convert=False)
+        ts = []
+        failed = 0
+        for _ in range(2):
+            t = time.perf_counter()
+            failed = 0
+            for (mode, sr), (rb, db) in groups.items():
+                _, umos, _ = native.measure_batch_device(uctx[mode], rb, db, sr, convert=False)
+                failed += int(np.sum(np.isnan(umos)))
+            ts.append(time.perf_counter() - t)
+        secs = float(sum(p[0].size / p[2] for p in mix))
+        step = float(np.mean(ts))
+        uneven = {"workload": f"{len(mix)} pairs of the uneven mix (durations log-uniform 1-60 s, "
+                              f"{sum(1 for p in mix if p[3] == 1)} speech), one device-resident batch per mode",
+                  "pairs_per_s": round(len(mix) / step, 2), "audio_seconds_per_s": round(secs / step, 1),
+                  "ms_per_batch": round(step * 1e3, 2), "failed": failed}
+        for g in groups.values():
+            for b in g[0] + g[1]:
+                b.free()
+        print(f"bench: uneven {uneven}", file=sys.stderr, flush=True)
 
     serving = None
     qps_levels = [float(q) for q in args.qps.split(",") if q.strip()]
@@ -258,6 +303,8 @@ def main() -> int:
                             # instruction-issue ceiling is peak / 2 lane-ops per second
                             "issue_frac": round(achieved * (17.0 / 30.0) / (FP64_PEAK_TFLOPS / 2.0), 4)}
 
+    if uneven is not None:
+        line["uneven_batch"] = uneven
     if serving is not None:
         line["serving"] = serving
     if not args.no_cpu:

@@ -121,7 +121,7 @@ def test_best_lag_matches_oracle(native, vo):
         assert native.best_lag(a, b) == want
 
 
-@pytest.mark.parametrize("lg", [2, 5, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20])
+@pytest.mark.parametrize("lg", [2, 5, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 24])
 def test_xcorr_bit_identical(native, vo, lg):
     """Every value of the device cross-correlation (envelopes built in the
     first forward pass, fused middle pass, inverse passes) equals the
