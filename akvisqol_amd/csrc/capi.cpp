@@ -353,6 +353,26 @@ int visqol_measure_files(visqol_ctx* c, const char* ref_path, const char* deg_pa
     return visqol_measure(c, ra.samples.data(), ra.samples.size(), da.samples.data(), da.samples.size(), ra.sample_rate, out);
 }
 
+int visqol_hip_decode_wav(const uint8_t* data, size_t len, double* samples, size_t cap, size_t* n_samples,
+                          uint32_t* sample_rate) {
+    if (!data && len) return invalid("data is NULL");
+    if (!n_samples) return invalid("n_samples is NULL");
+    *n_samples = 0;
+    WavAudio a;
+    std::string err;
+    if (!decode_wav_mono(data, len, a, err)) {
+        set_error("%s", err.c_str());
+        return wav_error_code(err);
+    }
+    *n_samples = a.samples.size();
+    if (sample_rate) *sample_rate = a.sample_rate;
+    if (samples) {
+        if (cap < a.samples.size()) return invalid("samples holds fewer than the decoded frames");
+        memcpy(samples, a.samples.data(), a.samples.size() * sizeof(double));
+    }
+    return VISQOL_OK;
+}
+
 int visqol_measure_files_batch(visqol_ctx* c, size_t n, const char* const* ref_paths, const char* const* deg_paths,
                                uint32_t num_threads, visqol_result** out, int32_t* status) {
     (void)num_threads;
@@ -572,6 +592,30 @@ int visqol_hip_profile_phases(visqol_ctx* c, double* ms, int n) {
     const Engine::Profile p = c->engine->profile();
     for (int i = 0; i < n && i < Engine::Profile::kPhases; i++) ms[i] = p.phase_ms[i];
     return Engine::Profile::kPhases;
+}
+
+int visqol_hip_gammatone_shape(int shape, int* waves, int* bands_per_lane, int* chunk) {
+    if (shape < 0 || shape >= gammatone_shape_count()) return invalid("no such spectrogram block shape");
+    const GtShape s = gammatone_shape(shape);
+    if (waves) *waves = s.waves;
+    if (bands_per_lane) *bands_per_lane = s.bands_per_lane;
+    if (chunk) *chunk = s.chunk;
+    return gammatone_shape_count();
+}
+
+int visqol_hip_gammatone_tuning(visqol_ctx* c, uint32_t sample_rate, int kind, int* chosen, float* ms, int cap) {
+    if (!c) return invalid("ctx is NULL");
+    if (!chosen) return invalid("chosen is NULL");
+    if (kind < 0 || kind > 1 || sample_rate == 0) return invalid("invalid argument");
+    try {
+        const Engine::GtTuning t = c->engine->gammatone_tuning(sample_rate, kind);
+        *chosen = t.chosen;
+        for (int v = 0; v < cap && v < (int)t.ms.size(); v++) ms[v] = t.ms[v];
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return gammatone_shape_count();
 }
 
 int visqol_hip_alloc(size_t bytes, void** p) {

@@ -19,7 +19,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
+#include <mutex>
 #include <thread>
 
 namespace vq {
@@ -100,6 +102,11 @@ Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
                std::unique_ptr<LatticeModel, LatticeDeleter> lattice)
     : opts_(o), svr_(std::move(svr)), lattice_(std::move(lattice)) {
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    if (const char* e = getenv("VISQOL_HIP_GAMMATONE_SHAPE")) {
+        char* end = nullptr;
+        const long v = strtol(e, &end, 10);
+        if (end != e && *end == 0 && v >= 0 && v < gammatone_shape_count()) gt_force_ = (int)v;
+    }
 }
 
 Engine::~Engine() {
@@ -541,7 +548,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                 for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
             check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
         }
-        launch_gammatone(d_gj, (int)gj.size(), max_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, stream_);
+        gammatone(d_gj, (int)gj.size(), max_nf, dr, 0);
         if (profiling_) {
             check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
             timed = true;
@@ -808,7 +815,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                 rpj.push_back(PrepJob{rj.spec_r, rj.spec_d, rj.nf, rj.nf});
             }
             launch_slices(upload(tslj), (int)tslj.size(), max_t, stream_);
-            launch_gammatone(upload(rg), (int)rg.size(), rmax_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, stream_);
+            gammatone(upload(rg), (int)rg.size(), rmax_nf, dr, 1);
             launch_frame_copies(upload(copies), (int)copies.size(), nb, rmax_nf, stream_);
             auto* rscratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * std::max<size_t>(1, rpj.size()));
             launch_prep(upload(rpj), (int)rpj.size(), nb, rmax_nf, rscratch, stream_);
@@ -1024,7 +1031,7 @@ std::unique_ptr<PreparedSet> Engine::prepare(const double* const* refs, const lo
     sums(s1, d_out, !opts_.fast);
     sums(s2, d_out, !opts_.fast);
     if (!gj.empty())
-        launch_gammatone(upload(gj), (int)gj.size(), max_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, stream_);
+        gammatone(upload(gj), (int)gj.size(), max_nf, dr, 0);
     unsigned char* d_active = vad_bytes ? (unsigned char*)arena_.alloc(vad_bytes) : nullptr;
     if (!vj.empty()) {
         for (int i = 0; i < n; i++)
@@ -1048,6 +1055,94 @@ std::unique_ptr<PreparedSet> Engine::prepare(const double* const* refs, const lo
     return set;
 }
 
+// ---------------------------------------------------------------- spectrogram shape tuning
+namespace {
+// Process-wide tuning table: (device, fast, bands, window, kind) -> state.
+struct GtTuneEntry {
+    int chosen = -1;
+    bool busy = false;  // an engine is timing the shapes now
+    std::vector<float> ms;
+};
+std::mutex g_gt_mu;
+std::map<std::array<int, 5>, GtTuneEntry> g_gt_tune;
+// Launches of fewer blocks (of the first shape) than this run the first
+// shape untuned: too small to fill 256 CUs several times over.
+constexpr long long kTuneBlocks = 4096;
+}  // namespace
+
+void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate& dr, int kind) {
+    const int nb = dr.p.nb, W = dr.p.window, hop = dr.p.hop;
+    auto launch = [&](int shape) {
+        launch_gammatone(d_jobs, njobs, max_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, shape, stream_);
+    };
+    if (njobs == 0 || max_nf == 0) return;
+    if (gt_force_ >= 0) return launch(gt_force_);
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    const std::array<int, 5> key{dev, opts_.fast ? 1 : 0, nb, W, kind};
+    const GtShape d0 = gammatone_shape(0);
+    const long long blocks = (long long)njobs * ((max_nf + 63) / 64) *
+                             ((nb + d0.waves * d0.bands_per_lane - 1) / (d0.waves * d0.bands_per_lane));
+    {
+        std::lock_guard<std::mutex> g(g_gt_mu);
+        GtTuneEntry& t = g_gt_tune[key];
+        if (t.chosen >= 0) return launch(t.chosen);
+        if (t.busy || blocks < kTuneBlocks) return launch(0);
+        t.busy = true;
+    }
+    // Every shape writes the same output, so this launch's spectrograms
+    // come out of whichever runs last.  Two rounds; each shape keeps its
+    // faster time (the first round also absorbs one-time launch costs).
+    const int count = gammatone_shape_count();
+    std::vector<hipEvent_t> ev(count + 1, nullptr);
+    std::vector<float> best(count, 0.f);
+    int chosen = 0;
+    try {
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        for (int round = 0; round < 2; round++) {
+            check(hipEventRecord(ev[0], stream_), "hipEventRecord");
+            for (int v = 0; v < count; v++) {
+                launch(v);
+                check(hipEventRecord(ev[v + 1], stream_), "hipEventRecord");
+            }
+            check(hipEventSynchronize(ev[count]), "hipEventSynchronize");
+            for (int v = 0; v < count; v++) {
+                float ms = 0.f;
+                check(hipEventElapsedTime(&ms, ev[v], ev[v + 1]), "hipEventElapsedTime");
+                best[v] = round == 0 ? ms : std::min(best[v], ms);
+            }
+        }
+        for (int v = 1; v < count; v++)
+            if (best[v] < best[chosen]) chosen = v;
+    } catch (...) {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        std::lock_guard<std::mutex> g(g_gt_mu);
+        g_gt_tune[key].busy = false;
+        throw;
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    std::lock_guard<std::mutex> g(g_gt_mu);
+    GtTuneEntry& t = g_gt_tune[key];
+    t.busy = false;
+    t.ms = best;
+    t.chosen = chosen;
+}
+
+Engine::GtTuning Engine::gammatone_tuning(uint32_t sample_rate, int kind) {
+    const RateParams& rp = rate(sample_rate);
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> g(g_gt_mu);
+    auto it = g_gt_tune.find({dev, opts_.fast ? 1 : 0, rp.nb, rp.window, kind});
+    GtTuning out;
+    if (it != g_gt_tune.end()) {
+        out.chosen = it->second.chosen;
+        out.ms = it->second.ms;
+    }
+    return out;
+}
+
 // ---------------------------------------------------------------- stage access
 void Engine::spectrogram(uint32_t sr, const double* host_sig, long long len, std::vector<double>& out, int& nf) {
     arena_.reset();
@@ -1060,7 +1155,7 @@ void Engine::spectrogram(uint32_t sr, const double* host_sig, long long len, std
     check(hipMemcpyAsync(d, host_sig, sizeof(double) * len, hipMemcpyHostToDevice, stream_), "upload");
     double* o = (double*)arena_.alloc(sizeof(double) * out.size());
     std::vector<GtJob> j{GtJob{d, o, nf, 0}};
-    launch_gammatone(upload(j), 1, nf, dr.coef, dr.hann, dr.p.nb, dr.p.window, dr.p.hop, opts_.fast, stream_);
+    gammatone(upload(j), 1, nf, dr, 0);
     double* p = (double*)pinned_.alloc(sizeof(double) * out.size());
     readback(p, o, sizeof(double) * out.size());
     sync();

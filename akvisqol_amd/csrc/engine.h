@@ -161,6 +161,17 @@ class Engine {
         static constexpr int kPhases = 8;
         double phase_ms[kPhases] = {0};
     };
+    // Spectrogram block shape for launches of one kind (0: whole signals,
+    // 1: realignment slices) at one precision, band count and window: the
+    // shape chosen by timing every shape on the first launch large enough
+    // to fill the device (shared by the process's engines on the device),
+    // or -1 while untuned; ms holds the timed launch of each shape.
+    struct GtTuning {
+        int chosen = -1;
+        std::vector<float> ms;
+    };
+    GtTuning gammatone_tuning(uint32_t sample_rate, int kind);
+
     void set_profiling(bool on) { profiling_ = on; }
     Profile profile() const { return prof_; }
     void reset_profile() { prof_ = Profile(); }
@@ -193,6 +204,10 @@ class Engine {
     // d_sums); pos[i] is job i's slot in the returned device lag array.
     long long* lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos);
     void finish(PairResult& r, const RateParams& rp);
+    // Spectrograms of d_jobs with the tuned block shape (tuning on the
+    // first large launch of this kind); kind as in gammatone_tuning.
+    void gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate& dr, int kind);
+    int gt_force_ = -1;  // block shape forced by VISQOL_HIP_GAMMATONE_SHAPE, else -1
 
     EngineOptions opts_;
     std::unique_ptr<SvrModel> svr_;

@@ -1,32 +1,39 @@
 // Gammatone ERB filterbank spectrogram on gfx950 (gammatone.zig:268-306,
 // 432-467; accel.zig:116-124).
 //
-// Work decomposition: one lane per (frame, band pair).  A wave holds 64
-// consecutive frames of two bands, so the bands' coefficients are
+// Work decomposition: one lane per (frame, group of bands).  A wave holds 64
+// consecutive frames of BPL bands, so the bands' coefficients are
 // wave-uniform (scalar registers) and the four cascaded DF2T biquads run in
 // vector registers, in the reference's exact operation order; each lane
-// runs two bands' recurrences side by side.  A block holds four waves (eight
-// bands) over the same 64 frames; the frames' Hann-windowed
-// samples are staged chunk by chunk in LDS (coalesced rows of 32 samples,
+// runs its BPL bands' recurrences side by side.  A block holds WAVES waves
+// (WAVES·BPL bands) over the same 64 frames; the frames' Hann-windowed
+// samples are staged CHUNK samples at a time in LDS (coalesced rows,
 // x·hann computed once per block and shared by its bands), the next chunk's
-// loads in flight while the current chunk's recurrences run.
+// loads in flight while the current chunk's recurrences run.  The block
+// shape (WAVES, BPL, CHUNK) only maps work to lanes: every shape computes
+// each (frame, band) with the same operations in the same order, so all
+// shapes give bit-identical spectrograms and the fastest can be chosen by
+// timing them.
 //
 // Exact variant: mul and add rounded separately (-ffp-contract=off), the
 // output bit-identical to the CPU recurrence.  Fast variant: the same
 // recurrence with fused multiply-adds (17 instead of 30 operations per
 // sample and band).
+#include <array>
+#include <utility>
+
 #include "dev_common.h"
 #include "kernels.h"
 
 namespace vq {
 
 constexpr int kFramesPerWave = 64;
-constexpr int kChunk = 32;        // samples per staged chunk (LDS tile 64 x 33 doubles)
-constexpr int kWavesGt = 4;
-constexpr int kBandsPerWave = 4;  // independent recurrences per lane
-constexpr int kBandsPerBlock = kWavesGt * kBandsPerWave;
-constexpr int kThreadsGt = 64 * kWavesGt;
-constexpr int kStage = kFramesPerWave * kChunk / kThreadsGt;  // tile elements staged per thread
+
+// Block shapes the launcher offers, fastest-first guess at index 0.
+constexpr GtShape kGtShapes[] = {
+    {4, 4, 32}, {8, 4, 32}, {2, 4, 32}, {4, 2, 32}, {8, 2, 32}, {4, 4, 64}, {8, 4, 16},
+};
+constexpr int kGtShapeCount = (int)(sizeof(kGtShapes) / sizeof(kGtShapes[0]));
 
 // The four cascaded DF2T biquads of one band on one frame, and the sum of
 // squares of their output.
@@ -67,36 +74,39 @@ __device__ inline void band_step(BandState& b, const BandCoef& k, double s) {
     }
 }
 
-template <bool FAST>
-__global__ __launch_bounds__(kThreadsGt) void k_gammatone(const GtJob* __restrict__ jobs, const BandCoef* __restrict__ coef,
+template <bool FAST, int WAVES, int BPL, int CHUNK>
+__global__ __launch_bounds__(64 * WAVES) void k_gammatone(const GtJob* __restrict__ jobs, const BandCoef* __restrict__ coef,
                                                           const double* __restrict__ hann, int nb, int window, int hop) {
+    constexpr int kThreads = 64 * WAVES;
+    constexpr int kStage = kFramesPerWave * CHUNK / kThreads;  // tile elements staged per thread
+    static_assert(kStage >= 1 && kFramesPerWave * CHUNK % kThreads == 0, "tile must split evenly over the block");
     const GtJob job = jobs[blockIdx.z];
     const int f0 = blockIdx.x * kFramesPerWave;
     if (f0 >= job.nf) return;  // whole block idle: uniform exit
-    __shared__ double tile[kFramesPerWave][kChunk + 1];
+    __shared__ double tile[kFramesPerWave][CHUNK + 1];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int band0 = blockIdx.y * kBandsPerBlock + wave * kBandsPerWave;
+    const int band0 = blockIdx.y * (WAVES * BPL) + wave * BPL;
     const int frame = f0 + lane;
-    BandCoef k[kBandsPerWave];
-    BandState st[kBandsPerWave];
+    BandCoef k[BPL];
+    BandState st[BPL];
 #pragma unroll
-    for (int q = 0; q < kBandsPerWave; q++) {
+    for (int q = 0; q < BPL; q++) {
         if (band0 + q < nb) k[q] = coef[band0 + q];
         else k[q] = BandCoef{0, 0, 0, 0, 0, 0, 0, 0};
     }
     const bool active = band0 < nb && frame < job.nf;
 
-    // Staging: thread t stages sample i0 + (t mod kChunk) of frames
-    // f0 + t / kChunk + u·(256 / kChunk) (u < kStage), so each wave reads
-    // contiguous runs of kChunk samples and the Hann factor is one value per
+    // Staging: thread t stages sample i0 + (t mod CHUNK) of frames
+    // f0 + t / CHUNK + u·(kThreads / CHUNK) (u < kStage), so each wave reads
+    // contiguous runs of CHUNK samples and the Hann factor is one value per
     // thread and chunk.  The next chunk's loads are issued (unconditionally,
     // from clamped in-range addresses) before the current chunk's recurrence
     // runs; masking and x·hann happen at the LDS write.
-    constexpr int kRowsPerPass = kThreadsGt / kChunk;
+    constexpr int kRowsPerPass = kThreads / CHUNK;
     const auto* sig = as_global(job.sig);
-    const int ti = threadIdx.x % kChunk;
-    const int fr0 = threadIdx.x / kChunk;
+    const int ti = threadIdx.x % CHUNK;
+    const int fr0 = threadIdx.x / CHUNK;
     double v[kStage];
     double h = 0.0;
     auto load_chunk = [&](int i0) {
@@ -111,8 +121,8 @@ __global__ __launch_bounds__(kThreadsGt) void k_gammatone(const GtJob* __restric
         }
     };
     load_chunk(0);
-    for (int i0 = 0; i0 < window; i0 += kChunk) {
-        const int clen = min(kChunk, window - i0);
+    for (int i0 = 0; i0 < window; i0 += CHUNK) {
+        const int clen = min(CHUNK, window - i0);
         __syncthreads();
         const bool in = i0 + ti < window;
 #pragma unroll
@@ -121,32 +131,53 @@ __global__ __launch_bounds__(kThreadsGt) void k_gammatone(const GtJob* __restric
             tile[fr][ti] = (in && f0 + fr < job.nf) ? v[u] * h : 0.0;
         }
         __syncthreads();
-        if (i0 + kChunk < window) load_chunk(i0 + kChunk);
+        if (i0 + CHUNK < window) load_chunk(i0 + CHUNK);
         if (active) {
             for (int i = 0; i < clen; i++) {
                 const double s = tile[lane][i];
 #pragma unroll
-                for (int q = 0; q < kBandsPerWave; q++) band_step<FAST>(st[q], k[q], s);
+                for (int q = 0; q < BPL; q++) band_step<FAST>(st[q], k[q], s);
             }
         }
     }
     if (active) {
         const double inv_n = 1.0 / (double)window;
 #pragma unroll
-        for (int q = 0; q < kBandsPerWave; q++)
+        for (int q = 0; q < BPL; q++)
             if (band0 + q < nb) job.out[(size_t)(band0 + q) * job.nf + frame] = sqrt(st[q].sum_sq * inv_n);
     }
 }
 
+namespace {
+template <bool FAST, int V>
+void launch_shape(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const double* d_hann, int nb,
+                  int window, int hop, hipStream_t stream) {
+    constexpr GtShape sh = kGtShapes[V];
+    dim3 grid((max_nf + kFramesPerWave - 1) / kFramesPerWave, (nb + sh.waves * sh.bands_per_lane - 1) / (sh.waves * sh.bands_per_lane),
+              njobs);
+    hipLaunchKernelGGL((k_gammatone<FAST, sh.waves, sh.bands_per_lane, sh.chunk>), grid, dim3(64 * sh.waves), 0, stream,
+                       d_jobs, d_coef, d_hann, nb, window, hop);
+}
+
+using GtLaunch = void (*)(const GtJob*, int, int, const BandCoef*, const double*, int, int, int, hipStream_t);
+
+template <bool FAST, int... V>
+constexpr std::array<GtLaunch, sizeof...(V)> shape_table(std::integer_sequence<int, V...>) {
+    return {&launch_shape<FAST, V>...};
+}
+constexpr auto kFastLaunch = shape_table<true>(std::make_integer_sequence<int, kGtShapeCount>{});
+constexpr auto kExactLaunch = shape_table<false>(std::make_integer_sequence<int, kGtShapeCount>{});
+}  // namespace
+
+int gammatone_shape_count() { return kGtShapeCount; }
+
+GtShape gammatone_shape(int v) { return kGtShapes[v >= 0 && v < kGtShapeCount ? v : 0]; }
+
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const double* d_hann, int nb,
-                      int window, int hop, bool fast, hipStream_t stream) {
+                      int window, int hop, bool fast, int shape, hipStream_t stream) {
     if (njobs == 0 || max_nf == 0) return;
-    dim3 grid((max_nf + kFramesPerWave - 1) / kFramesPerWave, (nb + kBandsPerBlock - 1) / kBandsPerBlock, njobs);
-    dim3 block(kThreadsGt);
-    if (fast)
-        hipLaunchKernelGGL(k_gammatone<true>, grid, block, 0, stream, d_jobs, d_coef, d_hann, nb, window, hop);
-    else
-        hipLaunchKernelGGL(k_gammatone<false>, grid, block, 0, stream, d_jobs, d_coef, d_hann, nb, window, hop);
+    const int v = shape >= 0 && shape < kGtShapeCount ? shape : 0;
+    (fast ? kFastLaunch : kExactLaunch)[v](d_jobs, njobs, max_nf, d_coef, d_hann, nb, window, hop, stream);
 }
 
 }  // namespace vq

@@ -18,8 +18,16 @@ struct GtJob {
     int pad_;
 };
 
+// Block shape of the spectrogram kernel: waves per block, bands per lane,
+// samples per staged chunk (64 frames per block, one per lane).  Every shape
+// gives the same output bits.
+struct GtShape {
+    int waves, bands_per_lane, chunk;
+};
+int gammatone_shape_count();
+GtShape gammatone_shape(int v);
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const double* d_hann, int nb,
-                      int window, int hop, bool fast, hipStream_t stream);
+                      int window, int hop, bool fast, int shape, hipStream_t stream);
 
 // ---------------------------------------------------------------- reductions
 // Serial (reference-order) or tree reductions of one signal each.

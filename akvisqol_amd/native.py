@@ -133,6 +133,10 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_profile_read": (c_int, [c_void_p, D, D, D, POINTER(c_int64)]),
         "visqol_hip_profile_phases": (c_int, [c_void_p, D, c_int]),
         "visqol_hip_profile_sim": (c_int, [c_void_p, D, D, POINTER(c_int64)]),
+        "visqol_hip_gammatone_shape": (c_int, [c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+        "visqol_hip_gammatone_tuning": (c_int, [c_void_p, c_uint32, c_int, POINTER(c_int), POINTER(ctypes.c_float),
+                                                c_int]),
+        "visqol_hip_decode_wav": (c_int, [ctypes.c_char_p, c_size_t, D, c_size_t, S, POINTER(c_uint32)]),
         "visqol_hip_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
         "visqol_hip_free": (c_int, [c_void_p]),
         "visqol_hip_upload": (c_int, [c_void_p, c_void_p, c_size_t]),
@@ -271,6 +275,36 @@ class Context:
         finally:
             lib.visqol_result_free(out)
 
+    def measure_files(self, ref_path, deg_path) -> dict:
+        """visqol_measure_files: both WAVs decoded to mono f64 and scored."""
+        out = POINTER(Result)()
+        with self._lock:
+            rc = lib.visqol_measure_files(self._h, os.fsencode(ref_path), os.fsencode(deg_path), ctypes.byref(out))
+            _check(rc)
+        try:
+            return result_dict(out.contents)
+        finally:
+            lib.visqol_result_free(out)
+
+    def measure_files_batch(self, ref_paths, deg_paths, num_threads: int = 0):
+        """(first error code, [result dict or None], [status]) over WAV pairs;
+        pairs are grouped into one device batch per sample rate."""
+        n = len(ref_paths)
+        rp = (c_char_p * n)(*[os.fsencode(p) for p in ref_paths])
+        dp = (c_char_p * n)(*[os.fsencode(p) for p in deg_paths])
+        out = (POINTER(Result) * n)()
+        st = (c_int32 * n)()
+        with self._lock:
+            rc = lib.visqol_measure_files_batch(self._h, n, rp, dp, num_threads, out, st)
+        results = []
+        for i in range(n):
+            if out[i]:
+                results.append(result_dict(out[i].contents))
+                lib.visqol_result_free(out[i])
+            else:
+                results.append(None)
+        return rc, results, list(st)
+
     def measure_trace(self, ref, deg, sample_rate: int, want_stages: bool = False):
         """visqol_hip_measure_trace: (result, trace dict with optional stages)."""
         ref, deg = _f64(ref), _f64(deg)
@@ -361,6 +395,17 @@ class Context:
         return rc, res, list(st)
 
     PHASES = ("phase_a", "plan", "phase_b", "match_stats", "match_decode", "realign_lags", "realign_stats", "aggregate")
+
+    def gammatone_tuning(self, sample_rate: int, kind: int = 0) -> dict:
+        """Spectrogram block shape chosen for this context's precision and
+        mode at `sample_rate` (kind 0: whole signals, 1: realignment
+        slices): {"chosen": index or -1, "ms": per-shape times, "shapes": [...]}."""
+        count = gammatone_shapes()
+        chosen = c_int(-1)
+        ms = (ctypes.c_float * len(count))()
+        _check_count(lib.visqol_hip_gammatone_tuning(self._h, sample_rate, kind, ctypes.byref(chosen), ms, len(count)))
+        return {"chosen": chosen.value, "ms": [round(float(x), 4) for x in ms] if chosen.value >= 0 else [],
+                "shapes": count}
 
     def profile_phases(self) -> dict:
         """Host wall time (ms) of each stretch of the batch calls since the last reset."""
@@ -543,6 +588,35 @@ def spectrogram(signal, sample_rate: int, mode: int = MODE_AUDIO, precision: str
     _check(lib.visqol_hip_spectrogram(mode, sample_rate, PRECISIONS[precision], _dptr(x), x.size, _dptr(out),
                                       ctypes.byref(nf)))
     return out[: nb * nf.value].reshape(nb, nf.value)
+
+
+def _check_count(rc: int) -> int:
+    if rc < 0:
+        _check(rc)
+    return rc
+
+
+def gammatone_shapes() -> list[dict]:
+    """The spectrogram kernel's block shapes (64 frames per block each)."""
+    out = []
+    w, b, c = c_int(), c_int(), c_int()
+    n = _check_count(lib.visqol_hip_gammatone_shape(0, ctypes.byref(w), ctypes.byref(b), ctypes.byref(c)))
+    for v in range(n):
+        _check_count(lib.visqol_hip_gammatone_shape(v, ctypes.byref(w), ctypes.byref(b), ctypes.byref(c)))
+        out.append({"waves": w.value, "bands_per_lane": b.value, "chunk": c.value, "frames_per_block": 64,
+                    "bands_per_block": w.value * b.value})
+    return out
+
+
+def decode_wav(data: bytes) -> tuple[np.ndarray, int]:
+    """visqol_hip_decode_wav: (mono f64 samples, sample rate) of WAV bytes;
+    raises NativeError with the file entry points' code on a bad file."""
+    n = c_size_t(0)
+    sr = c_uint32(0)
+    _check(lib.visqol_hip_decode_wav(data, len(data), None, 0, ctypes.byref(n), ctypes.byref(sr)))
+    out = np.empty(n.value, dtype=np.float64)
+    _check(lib.visqol_hip_decode_wav(data, len(data), _dptr(out), out.size, ctypes.byref(n), ctypes.byref(sr)))
+    return out, sr.value
 
 
 def filters(sample_rate: int, mode: int = MODE_AUDIO):

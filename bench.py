@@ -165,6 +165,15 @@ def main() -> int:
               f"{simp['cells'] / simp['launches']:.4g} cells/launch", file=sys.stderr, flush=True)
     ctx.profile(False, True)
     step_s = float(np.mean(times))
+    # spectrogram block shapes timed by the engine on its first large launch
+    autotune = {}
+    for kind, name in ((0, "spectrogram"), (1, "realignment_spectrogram")):
+        t = ctx.gammatone_tuning(48000, kind)
+        if t["chosen"] >= 0:
+            autotune[name] = {"chosen": t["shapes"][t["chosen"]],
+                              "ms_per_shape": {f"{sh['waves']}w{sh['bands_per_lane']}b{sh['chunk']}c": m
+                                               for sh, m in zip(t["shapes"], t["ms"])}}
+    print(f"bench: autotune {autotune}", file=sys.stderr, flush=True)
     value = args.batch / step_s
 
     # The same batch handed over as host arrays (visqol_measure_batch): the
@@ -180,6 +189,29 @@ def main() -> int:
     t = time.perf_counter()
     exact_mos = [ex.measure(pairs[i][0], pairs[i][1], 48000)["moslqo"] for i in range(ne)]
     exact_pps = ne / (time.perf_counter() - t)
+
+    # Per-stage error of the throughput mode against f64 (the exact path,
+    # bit-identical to the oracle) on the first pairs of the batch.
+    fctx = native.Context(native.MODE_AUDIO, precision="fast")
+    err = {"spectrogram_max_rel": 0.0, "similarity_table_max_abs": 0.0, "fvnsim_max_abs": 0.0,
+           "vnsim_max_abs": 0.0, "moslqo_max_abs": 0.0, "same_alignments": True}
+    n_stage = min(4, args.batch)
+    for i in range(n_stage):
+        er, et = ex.measure_trace(pairs[i][0], pairs[i][1], 48000, want_stages=True)
+        fr, ft = fctx.measure_trace(pairs[i][0], pairs[i][1], 48000, want_stages=True)
+        for k in ("ref_spec", "deg_spec"):
+            rel = np.abs(ft[k] - et[k]) / np.maximum(np.abs(et[k]), 1e-300)
+            err["spectrogram_max_rel"] = max(err["spectrogram_max_rel"], float(np.max(rel)))
+        if ft["sims"].shape == et["sims"].shape:
+            err["similarity_table_max_abs"] = max(err["similarity_table_max_abs"],
+                                                  float(np.max(np.abs(ft["sims"] - et["sims"]))))
+        err["fvnsim_max_abs"] = max(err["fvnsim_max_abs"], float(np.max(np.abs(fr["fvnsim"] - er["fvnsim"]))))
+        err["vnsim_max_abs"] = max(err["vnsim_max_abs"], abs(fr["vnsim"] - er["vnsim"]))
+        err["moslqo_max_abs"] = max(err["moslqo_max_abs"], abs(fr["moslqo"] - er["moslqo"]))
+        for k in ("lag", "dp_offset", "realign_lag", "realign_accepted"):
+            err["same_alignments"] &= ft[k] == et[k]
+    err["pairs"] = n_stage
+    print(f"bench: fast-mode stage error vs f64 {err}", file=sys.stderr, flush=True)
 
     # Uneven batches: pairs of the 1-60 s mix (70% 48 kHz audio, 30% 16 kHz
     # speech, per-pair noise, delay, stretch, dropouts), device-resident,
@@ -283,6 +315,8 @@ def main() -> int:
         "exact_per_pair_pairs_per_s": round(exact_pps, 3),
         "speedup_vs_exact_per_pair": round(value / exact_pps, 2),
         "max_abs_dmos_fast_vs_exact_gpu": float(np.max(np.abs(fast_mos[:ne] - np.array(exact_mos)))),
+        "stage_error_fast_vs_f64": err,
+        "autotune": autotune,
     }
     if prof["launches"] and prof["ms"] > 0:
         ms_per_launch = prof["ms"] / prof["launches"]

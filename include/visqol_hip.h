@@ -119,6 +119,30 @@ int visqol_hip_server_stats(visqol_hip_server *s, uint64_t *batches, uint64_t *p
 void visqol_hip_server_destroy(visqol_hip_server *s);
 
 /* Device-resident buffers for the batch entry point above. */
+/* Spectrogram kernel block shapes.  visqol_hip_gammatone_shape describes
+ * shape `shape` (waves per block, bands per lane, samples per staged chunk;
+ * 64 frames per block) and returns the number of shapes.  Every shape gives
+ * bit-identical spectrograms; an engine times them all on its first launch
+ * large enough to fill the device and keeps the fastest for the process
+ * (per device, precision, band count and window; kind 0: whole signals,
+ * 1: realignment slices).  visqol_hip_gammatone_tuning reports that choice
+ * (*chosen = -1 while untuned) and, if `ms` holds `cap` floats, the timed
+ * launch of each shape; it returns the number of shapes.  The environment
+ * variable VISQOL_HIP_GAMMATONE_SHAPE=k, read when a context is created,
+ * forces shape k for that context. */
+int visqol_hip_gammatone_shape(int shape, int *waves, int *bands_per_lane, int *chunk);
+int visqol_hip_gammatone_tuning(visqol_ctx *ctx, uint32_t sample_rate, int kind, int *chosen, float *ms, int cap);
+
+/* WAV decoding as the file entry points do it (audio.zig:48-130: RIFF/WAVE,
+ * PCM 8/16/24/32-bit, float 32/64-bit, WAVE_FORMAT_EXTENSIBLE, channel
+ * mean), host only.  Sets *n_samples (frames) and *sample_rate; copies the
+ * mono samples when `samples` is non-NULL and holds `cap` >= frames.  Errors
+ * are the file entry points' codes (E_IO for format errors, E_SIGNAL for
+ * empty audio or zero channels) with the reference's error name as the
+ * last-error message. */
+int visqol_hip_decode_wav(const uint8_t *data, size_t len, double *samples, size_t cap, size_t *n_samples,
+                          uint32_t *sample_rate);
+
 int visqol_hip_alloc(size_t bytes, void **dev_ptr);
 int visqol_hip_free(void *dev_ptr);
 int visqol_hip_upload(void *dev_dst, const void *host_src, size_t bytes);

@@ -278,3 +278,61 @@ def test_plain_f64_loop_64_pairs(native, vo, synth, mode):
     assert rc == 0
     for g, w in zip(got, want):
         _bitsame(g, w)
+
+
+# ---------------------------------------------------------------- spectrogram block shapes
+def _ctx_with_shape(native, mode, precision, shape):
+    import os
+
+    key = "VISQOL_HIP_GAMMATONE_SHAPE"
+    old = os.environ.get(key)
+    os.environ[key] = str(shape)
+    try:
+        return native.Context(mode, precision=precision)
+    finally:
+        if old is None:
+            del os.environ[key]
+        else:
+            os.environ[key] = old
+
+
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_gammatone_shapes_bit_identical(native, synth, precision):
+    """Every block shape of the spectrogram kernel gives the same bits: the
+    spectrograms, the similarity tables and the scores (realignment
+    spectrograms included) equal those of shape 0."""
+    shapes = native.gammatone_shapes()
+    assert len(shapes) >= 2
+    cases = [(0, 48000, synth.audio_pair(2, duration=4.0, delay=300)),
+             (1, 16000, synth.speech_pair(3, duration=4.0, delay=40))]
+    for mode, sr, (ref, deg) in cases:
+        base = None
+        for v, sh in enumerate(shapes):
+            say(f"shape {v} {sh} mode={mode} precision={precision}")
+            res, tr = _ctx_with_shape(native, mode, precision, v).measure_trace(ref, deg, sr, want_stages=True)
+            if base is None:
+                base = (res, tr)
+                continue
+            for k in ("ref_spec", "deg_spec", "sims"):
+                assert np.array_equal(tr[k], base[1][k]), (v, k)
+            _bitsame(res, base[0])
+
+
+def test_tuned_batch_equals_first_shape(native, synth):
+    """A batch large enough to tune the spectrogram shape (whole signals and
+    realignment slices) scores exactly as the same batch on shape 0."""
+    # 384 pairs x 2 signals x 3 frame blocks x 2 band blocks = 4608 blocks,
+    # above the 4096 the engine needs before it times the shapes
+    pairs = [synth.audio_pair(40 + s, duration=2.7, delay=50 * s) for s in range(8)] * 48
+    refs, degs = [p[0] for p in pairs], [p[1] for p in pairs]
+    say(f"tuned batch: {len(pairs)} pairs of 2.7 s")
+    ctx = native.Context(native.MODE_AUDIO, precision="fast")
+    rc, tuned, st = ctx.measure_batch(refs, degs, 48000)
+    assert rc == 0
+    t = ctx.gammatone_tuning(48000, 0)
+    say("tuning", t)
+    assert t["chosen"] >= 0 and len(t["ms"]) == len(t["shapes"])
+    rc, first, _ = _ctx_with_shape(native, native.MODE_AUDIO, "fast", 0).measure_batch(refs, degs, 48000)
+    assert rc == 0
+    for a, b in zip(tuned, first):
+        _bitsame(a, b)
