@@ -115,6 +115,7 @@ Engine::~Engine() {
         if (e) (void)hipEventDestroy(e);
     for (auto& kv : rates_) {
         (void)hipFree(kv.second->coef);
+        (void)hipFree(kv.second->fir);
         (void)hipFree(kv.second->hann);
     }
     for (auto& kv : twiddles_) (void)hipFree(kv.second);
@@ -162,6 +163,8 @@ Engine::DevRate& Engine::dev_rate(uint32_t sr) {
     auto d = std::make_unique<DevRate>();
     d->p = make_rate_params(opts_.mode, sr);
     check(hipMalloc(&d->coef, sizeof(BandCoef) * d->p.nb), "hipMalloc");
+    check(hipMalloc(&d->fir, sizeof(FirCoef) * d->p.nb), "hipMalloc");
+    check(hipMemcpy(d->fir, d->p.fir.data(), sizeof(FirCoef) * d->p.nb, hipMemcpyHostToDevice), "upload");
     check(hipMalloc(&d->hann, sizeof(double) * std::max(1, d->p.window)), "hipMalloc");
     check(hipMemcpy(d->coef, d->p.coef.data(), sizeof(BandCoef) * d->p.nb, hipMemcpyHostToDevice), "upload");
     check(hipMemcpy(d->hann, d->p.hann.data(), sizeof(double) * d->p.window, hipMemcpyHostToDevice), "upload");
@@ -1065,29 +1068,28 @@ struct GtTuneEntry {
 };
 std::mutex g_gt_mu;
 std::map<std::array<int, 5>, GtTuneEntry> g_gt_tune;
-// Launches of fewer blocks (of the first shape) than this run the first
-// shape untuned: too small to fill 256 CUs several times over.
-constexpr long long kTuneBlocks = 4096;
+// Launches of less work than this, counted in (signal, 64 frames, 16 bands)
+// units, run the first shape untuned: too small to fill 256 CUs several
+// times over.
+constexpr long long kTuneUnits = 4096;
 }  // namespace
 
 void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate& dr, int kind) {
     const int nb = dr.p.nb, W = dr.p.window, hop = dr.p.hop;
     auto launch = [&](int shape) {
-        launch_gammatone(d_jobs, njobs, max_nf, dr.coef, dr.hann, nb, W, hop, opts_.fast, shape, stream_);
+        launch_gammatone(d_jobs, njobs, max_nf, dr.coef, dr.fir, dr.hann, nb, W, hop, opts_.fast, shape, stream_);
     };
     if (njobs == 0 || max_nf == 0) return;
     if (gt_force_ >= 0) return launch(gt_force_);
     int dev = 0;
     check(hipGetDevice(&dev), "hipGetDevice");
     const std::array<int, 5> key{dev, opts_.fast ? 1 : 0, nb, W, kind};
-    const GtShape d0 = gammatone_shape(0);
-    const long long blocks = (long long)njobs * ((max_nf + 63) / 64) *
-                             ((nb + d0.waves * d0.bands_per_lane - 1) / (d0.waves * d0.bands_per_lane));
+    const long long units = (long long)njobs * ((max_nf + 63) / 64) * ((nb + 15) / 16);
     {
         std::lock_guard<std::mutex> g(g_gt_mu);
         GtTuneEntry& t = g_gt_tune[key];
         if (t.chosen >= 0) return launch(t.chosen);
-        if (t.busy || blocks < kTuneBlocks) return launch(0);
+        if (t.busy || units < kTuneUnits) return launch(0);
         t.busy = true;
     }
     // Every shape writes the same output, so this launch's spectrograms

@@ -185,6 +185,7 @@ class Engine {
     struct DevRate {
         RateParams p;
         BandCoef* coef = nullptr;
+        FirCoef* fir = nullptr;
         double* hann = nullptr;
     };
     DevRate& dev_rate(uint32_t sample_rate);

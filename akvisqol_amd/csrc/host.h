@@ -29,6 +29,7 @@ struct RateParams {
     double frame_duration = 0;
     std::vector<double> cf;        // centre frequencies, ascending
     std::vector<BandCoef> coef;    // per band, ascending
+    std::vector<FirCoef> fir;      // the same bands in FIR + all-pole form
     std::vector<double> hann;      // window
     std::vector<double> raw_band;  // 8 per band: a0 a11 a12 a13 a14 b1 b2 gain
 };

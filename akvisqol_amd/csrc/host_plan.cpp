@@ -106,6 +106,23 @@ RateParams make_rate_params(int mode, uint32_t sample_rate) {
         const double gi = 1.0 / o[7];
         p.coef[asc] = BandCoef{o[0] * gi, o[1] * gi, o[2], o[3], o[4], o[0], o[5], o[6]};
     }
+    // FIR taps: (s1n0 + s1n1 z^-1)(ca0 + s2n1 z^-1)(ca0 + s3n1 z^-1)(ca0 + s4n1 z^-1),
+    // multiplied out in extended precision and rounded once.
+    p.fir.resize(p.nb);
+    for (int b = 0; b < p.nb; b++) {
+        const BandCoef& k = p.coef[b];
+        long double t[5] = {(long double)k.s1n0, (long double)k.s1n1, 0, 0, 0};
+        const long double n1[3] = {(long double)k.s2n1, (long double)k.s3n1, (long double)k.s4n1};
+        for (int st = 0; st < 3; st++) {
+            long double q[5] = {0, 0, 0, 0, 0};
+            for (int i = 0; i <= st + 1; i++) {
+                q[i] += t[i] * (long double)k.ca0;
+                q[i + 1] += t[i] * n1[st];
+            }
+            for (int i = 0; i < 5; i++) t[i] = q[i];
+        }
+        p.fir[b] = FirCoef{(double)t[0], (double)t[1], (double)t[2], (double)t[3], (double)t[4], k.d1, k.d2, 0.0};
+    }
     p.window = (int)round(sr * 0.08);
     p.hop = (int)((double)p.window * 0.25);
     if (p.hop == 0) p.hop = 1;

@@ -26,8 +26,8 @@ struct GtShape {
 };
 int gammatone_shape_count();
 GtShape gammatone_shape(int v);
-void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const double* d_hann, int nb,
-                      int window, int hop, bool fast, int shape, hipStream_t stream);
+void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const FirCoef* d_fir,
+                      const double* d_hann, int nb, int window, int hop, bool fast, int shape, hipStream_t stream);
 
 // ---------------------------------------------------------------- reductions
 // Serial (reference-order) or tree reductions of one signal each.

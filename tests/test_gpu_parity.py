@@ -321,8 +321,9 @@ def test_gammatone_shapes_bit_identical(native, synth, precision):
 def test_tuned_batch_equals_first_shape(native, synth):
     """A batch large enough to tune the spectrogram shape (whole signals and
     realignment slices) scores exactly as the same batch on shape 0."""
-    # 384 pairs x 2 signals x 3 frame blocks x 2 band blocks = 4608 blocks,
-    # above the 4096 the engine needs before it times the shapes
+    # 384 pairs x 2 signals x 3 blocks of 64 frames x 2 groups of 16 bands
+    # = 4608 units of work, above the 4096 the engine needs before it times
+    # the shapes
     pairs = [synth.audio_pair(40 + s, duration=2.7, delay=50 * s) for s in range(8)] * 48
     refs, degs = [p[0] for p in pairs], [p[1] for p in pairs]
     say(f"tuned batch: {len(pairs)} pairs of 2.7 s")
