@@ -1089,7 +1089,10 @@ void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate
         std::lock_guard<std::mutex> g(g_gt_mu);
         GtTuneEntry& t = g_gt_tune[key];
         if (t.chosen >= 0) return launch(t.chosen);
-        if (t.busy || units < kTuneUnits) return launch(0);
+        // Realignment launches (kind 1) keep the first shape: their sizes
+        // vary from batch to batch, and a timing run in the middle of a
+        // served batch would stall it for the little they could gain.
+        if (t.busy || kind != 0 || units < kTuneUnits) return launch(0);
         t.busy = true;
     }
     // Every shape writes the same output, so this launch's spectrograms

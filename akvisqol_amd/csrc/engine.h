@@ -162,10 +162,11 @@ class Engine {
         double phase_ms[kPhases] = {0};
     };
     // Spectrogram block shape for launches of one kind (0: whole signals,
-    // 1: realignment slices) at one precision, band count and window: the
-    // shape chosen by timing every shape on the first launch large enough
-    // to fill the device (shared by the process's engines on the device),
-    // or -1 while untuned; ms holds the timed launch of each shape.
+    // 1: realignment slices, which always run the first shape) at one
+    // precision, band count and window: the shape chosen by timing every
+    // shape on the first whole-signal launch large enough to fill the
+    // device (shared by the process's engines on the device), or -1 while
+    // untuned; ms holds the timed launch of each shape.
     struct GtTuning {
         int chosen = -1;
         std::vector<float> ms;

@@ -7,7 +7,7 @@
 // correlation; the lag is the first maximum of |Re| scanning lags
 // -max..-1 then 0..max.
 //
-// Each HBM pass runs up to 8 radix-2 stages on 2048-element groups in LDS:
+// Each HBM pass runs radix-2 stages on 2048- or 4096-element groups in LDS:
 // the stages with spans S_hi .. S_hi/2^(k-1) only mix elements
 // blk·S_hi + r + m·q (q = S_hi/2^k) of one residue r, so a workgroup loads R
 // residues of Bn span blocks (R·16 B contiguous per m).  Every butterfly is
@@ -85,14 +85,16 @@ __device__ inline long long scan_order(long long i, long long n, long long max_l
 }
 
 // First maximum of |Re| in scan order over a tile in LDS; writes the
-// (max bits, scan position) pair of this workgroup to part[0..1].
-template <typename GIDX>
-__device__ inline void tile_argmax(const double2* sm, int count, GIDX gidx, long long n, const LagJob& job,
+// (max bits, scan position) pair of this workgroup to part[0..1].  The
+// reduction reuses the tile's LDS (at least 2·THREADS words) once every
+// thread has read its elements.
+template <int THREADS, typename GIDX>
+__device__ inline void tile_argmax(double2* sm, int count, GIDX gidx, long long n, const LagJob& job,
                                    unsigned long long* part) {
     const long long big = job.la > job.lb ? job.la : job.lb;
     const long long max_lag = big - 1;
     unsigned long long best_v = 0, best_o = ~0ULL;
-    for (int e = threadIdx.x; e < count; e += kThreads) {
+    for (int e = threadIdx.x; e < count; e += THREADS) {
         const long long o = scan_order(gidx(e), n, max_lag);
         if (o < 0) continue;
         const unsigned long long v = f64_bits(fabs(sm[e].x));
@@ -101,11 +103,13 @@ __device__ inline void tile_argmax(const double2* sm, int count, GIDX gidx, long
             best_o = (unsigned long long)o;
         }
     }
-    __shared__ unsigned long long rv[kThreads], ro[kThreads];
+    __syncthreads();
+    unsigned long long* rv = reinterpret_cast<unsigned long long*>(sm);
+    unsigned long long* ro = rv + THREADS;
     rv[threadIdx.x] = best_v;
     ro[threadIdx.x] = best_o;
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
+    for (int s = THREADS / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             const unsigned long long v2 = rv[threadIdx.x + s], o2 = ro[threadIdx.x + s];
             if (v2 > rv[threadIdx.x] || (v2 == rv[threadIdx.x] && o2 < ro[threadIdx.x])) {
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const LagJob* __restrict_
         for (int e = threadIdx.x; e < count; e += kThreads) store2(z, gidx(e), sm[e]);
         return;
     }
-    tile_argmax(sm, count, gidx, n, job, partials + 2 * (jb * nwg + wg));
+    tile_argmax<kThreads>(sm, count, gidx, n, job, partials + 2 * (jb * nwg + wg));
 }
 
 template <int K, bool DIT>
@@ -241,7 +245,7 @@ struct TileStep {
 // one radix-2 stage.  Twiddles of the next step load while the current one
 // computes.  Twiddle of stage l (span S_l = 2^(lgq + K - l)) at position
 // j = p·q + r0 + rr: tw[S_l/2 + j].  Starts and ends with a barrier.
-template <int K, bool DIT>
+template <int K, bool DIT, int THREADS>
 __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, const double2* __restrict__ tw) {
     const int lgGR = K + lgR;
     // Radix-4 group g (two per thread): residue rr, quarter index mq, block bb.
@@ -271,7 +275,7 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
 #pragma unroll
             for (int u = 0; u < 2; u++) {
                 int i0, lo, rr;
-                quad(threadIdx.x + u * kThreads, lgH, i0, lo, rr);
+                quad(threadIdx.x + u * THREADS, lgH, i0, lo, rr);
                 dst[u][0] = twi(l_big, lo, rr);
                 dst[u][1] = twi(l_big, lo + (1 << lgH), rr);
                 dst[u][2] = twi(l_small, lo, rr);
@@ -281,7 +285,7 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
             const int lgHm = K - 1 - l;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int t = threadIdx.x + u * kThreads;
+                const int t = threadIdx.x + u * THREADS;
                 const int rem = t & ((1 << (lgGR - 1)) - 1);
                 const int rr = rem & ((1 << lgR) - 1), bm = rem >> lgR;
                 dst[u][0] = twi(l, bm & ((1 << lgHm) - 1), rr);
@@ -301,7 +305,7 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
 #pragma unroll
             for (int u = 0; u < 2; u++) {
                 int lo, rr;
-                quad(threadIdx.x + u * kThreads, lgH, ib[u], lo, rr);
+                quad(threadIdx.x + u * THREADS, lgH, ib[u], lo, rr);
 #pragma unroll
                 for (int c = 0; c < 4; c++) x[u][c] = sm[ib[u] + ((c << lgH) << lgR)];
             }
@@ -329,7 +333,7 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
             int ia[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int t = threadIdx.x + u * kThreads;
+                const int t = threadIdx.x + u * THREADS;
                 const int bb = t >> (lgGR - 1), rem = t & ((1 << (lgGR - 1)) - 1);
                 const int rr = rem & ((1 << lgR) - 1), bm = rem >> lgR;
                 const int m0 = ((bm >> lgHm) << (lgHm + 1)) | (bm & ((1 << lgHm) - 1));
@@ -355,15 +359,18 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
     }
 }
 
-// Full-tile pass (n >= kTile, so every tile holds kTile elements) with K
+// Full-tile pass (every tile holds TILE elements, eight per thread) with K
 // stages: load the tile, run tile_stages, store it (or, on the inverse
 // transform's last pass, keep only the tile's first maximum).
 
-template <int K, bool DIT, int SRC, bool ARGMAX>
-__global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict__ jobs, int lgn, int lgS, int lgR, int lgBn,
+// At least two workgroups per CU (TILE / 1024 waves per SIMD): for
+// 4096-element tiles (8 waves each) at most 128 VGPRs.
+template <int K, bool DIT, int SRC, bool ARGMAX, int TILE>
+__global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TILE / 1024))) void k_fft_tile(const LagJob* __restrict__ jobs, int lgn, int lgS, int lgR, int lgBn,
                                                        const double2* __restrict__ tw, const double* __restrict__ sums,
                                                        unsigned long long* __restrict__ partials, int njobs) {
-    __shared__ double2 sm[kTile];
+    constexpr int THREADS = TILE / 8;
+    __shared__ double2 sm[TILE];
     const int jb = (int)(blockIdx.x % (unsigned)njobs);
     const long long wg = blockIdx.x / (unsigned)njobs;
     const long long nwg = gridDim.x / (unsigned)njobs;
@@ -382,7 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict_
         const int rr = e & ((1 << lgR) - 1);
         return base + ((long long)bb << lgS) + ((long long)m << lgq) + rr;
     };
-    constexpr int kPer = kTile / kThreads;
+    constexpr int kPer = 8;
     if (SRC == kSrcEnvelope) {
         // both signals' samples for the tile, loads first, envelope after
         const EnvSide ea = env_side(job.a, job.la, sums[job.sum_a], sums[job.nyq_a]);
@@ -390,34 +397,34 @@ __global__ __launch_bounds__(kThreads) void k_fft_tile(const LagJob* __restrict_
         double xa[kPer], xb[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
-            const long long i = gidx(threadIdx.x + u * kThreads);
+            const long long i = gidx(threadIdx.x + u * THREADS);
             xa[u] = env_load(ea, i);
             xb[u] = env_load(eb, i);
         }
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
-            const long long i = gidx(threadIdx.x + u * kThreads);
-            sm[threadIdx.x + u * kThreads] = make_double2(i < ea.len ? env_value(ea, xa[u], i) : 0.0,
+            const long long i = gidx(threadIdx.x + u * THREADS);
+            sm[threadIdx.x + u * THREADS] = make_double2(i < ea.len ? env_value(ea, xa[u], i) : 0.0,
                                                           i < eb.len ? env_value(eb, xb[u], i) : 0.0);
         }
     } else {
         double2 v[kPer];
 #pragma unroll
-        for (int u = 0; u < kPer; u++) v[u] = load2(z, gidx(threadIdx.x + u * kThreads));
+        for (int u = 0; u < kPer; u++) v[u] = load2(z, gidx(threadIdx.x + u * THREADS));
 #pragma unroll
-        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * kThreads] = v[u];
+        for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * THREADS] = v[u];
     }
 
-    tile_stages<K, DIT>(sm, lgR, lgq, r0, tw);
+    tile_stages<K, DIT, THREADS>(sm, lgR, lgq, r0, tw);
     if (!ARGMAX) {
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
-            const int e = threadIdx.x + u * kThreads;
+            const int e = threadIdx.x + u * THREADS;
             store2(z, gidx(e), sm[e]);
         }
         return;
     }
-    tile_argmax(sm, kTile, gidx, n, job, partials + 2 * (jb * nwg + wg));
+    tile_argmax<THREADS>(sm, TILE, gidx, n, job, partials + 2 * (jb * nwg + wg));
 }
 
 // xcorr.zig:211-229 cross terms, in place in bit-reversed order.
@@ -447,15 +454,15 @@ __global__ __launch_bounds__(256) void k_cross_terms(const LagJob* __restrict__ 
     }
 }
 
-// Middle of the lag search (n >= kTile): the forward transform's last pass,
+// Middle of the lag search (n >= TILE): the forward transform's last pass,
 // the cross terms and the inverse transform's first pass in one kernel.
-// Both passes work on aligned contiguous blocks of 2^K elements, and the
-// cross term of bit-reversed position i pairs it with 3h-1-i (h the largest
-// power of two <= i), which maps chunk c of 1024 elements in [h, 2h) onto
-// the whole chunk 2h/1024 - 1 - c; chunks 0 and 1 pair within themselves.
+// Both passes work on aligned contiguous blocks of 2^K elements (K <=
+// log2(TILE/2)), and the cross term of bit-reversed position i pairs it
+// with 3h-1-i (h the largest power of two <= i), which maps chunk c of
+// C = TILE/2 elements in [h, 2h) onto the whole chunk 2h/C - 1 - c; chunks
+// 0 and 1 pair within themselves.
 // Workgroup w holds the chunk pair (0, 1), or for w >= 1 with
 // 2^k <= w < 2^(k+1): chunks 2^(k+1) + t and 2^(k+2) - 1 - t, t = w - 2^k.
-constexpr int kHalfTile = kTile / 2;
 
 // One radix-2^B step (B = 1, 2, 3) over the eight elements a thread holds in
 // a pass over contiguous blocks (one residue): 8 >> B groups, element c of
@@ -497,13 +504,13 @@ __device__ inline void radix_step(double2 (&x)[8], const int (&lo)[8 >> B], int 
 
 // Tile positions (block bb of 2^K elements, position m) of the thread's
 // eight elements for a radix-2^B step with smallest distance 2^lgH: group
-// G = thread + gi·256, q = G mod 2^(K-B) its index in the block, lo = q mod
-// 2^lgH, m0 = q with B zero bits inserted at bit lgH.
-template <int K, int B>
+// G = thread + gi·THREADS, q = G mod 2^(K-B) its index in the block, lo = q
+// mod 2^lgH, m0 = q with B zero bits inserted at bit lgH.
+template <int K, int B, int THREADS>
 __device__ inline void group_layout(int lgH, int (&idx)[8], int (&lo)[8 >> B]) {
 #pragma unroll
     for (int gi = 0; gi < (8 >> B); gi++) {
-        const int g = (int)threadIdx.x + gi * kThreads;
+        const int g = (int)threadIdx.x + gi * THREADS;
         const int q = g & ((1 << (K - B)) - 1);
         const int bb = g >> (K - B);
         lo[gi] = q & ((1 << lgH) - 1);
@@ -514,7 +521,8 @@ __device__ inline void group_layout(int lgH, int (&idx)[8], int (&lo)[8 >> B]) {
 }
 
 // Stage counts of the steps before the central one (K - 3 stages in steps of
-// at most three, larger first): K = 7: 2,2; 8: 3,2; 9: 3,3; 10: 3,2,2.
+// at most three, larger first): K = 7: 2,2; 8: 3,2; 9: 3,3; 10: 3,2,2;
+// 11: 3,3,2.
 template <int K>
 struct MidPlan {
     static constexpr int kPre = K - 3;
@@ -522,12 +530,13 @@ struct MidPlan {
     static constexpr int b(int i) { return kPre / kSteps + (i < kPre % kSteps ? 1 : 0); }
 };
 
-// Middle of the lag search (n >= kTile): the forward transform's last pass,
+// Middle of the lag search (n >= TILE): the forward transform's last pass,
 // the cross terms and the inverse transform's first pass in one kernel.
-// Both passes work on aligned contiguous blocks of 2^K elements, and the
-// cross term of bit-reversed position i pairs it with 3h-1-i (h the largest
-// power of two <= i), which maps chunk c of 1024 elements in [h, 2h) onto
-// the whole chunk 2h/1024 - 1 - c; chunks 0 and 1 pair within themselves.
+// Both passes work on aligned contiguous blocks of 2^K elements (K <=
+// log2(TILE/2)), and the cross term of bit-reversed position i pairs it
+// with 3h-1-i (h the largest power of two <= i), which maps chunk c of
+// C = TILE/2 elements in [h, 2h) onto the whole chunk 2h/C - 1 - c; chunks
+// 0 and 1 pair within themselves.
 // Workgroup w holds the chunk pair (0, 1), or for w >= 1 with
 // 2^k <= w < 2^(k+1): chunks 2^(k+1) + t and 2^(k+2) - 1 - t, t = w - 2^k.
 //
@@ -539,10 +548,13 @@ struct MidPlan {
 // terms and runs the inverse stages of distance 1, 2, 4 in registers; the
 // remaining inverse steps mirror the forward ones, the last storing straight
 // to memory.  Every butterfly and twiddle is the stage-by-stage one.
-template <int K>
-__global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__ jobs, const double2* __restrict__ tw,
+template <int K, int TILE>
+__global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TILE / 1024))) void k_fft_mid(const LagJob* __restrict__ jobs, const double2* __restrict__ tw,
                                                       int njobs) {
-    __shared__ double2 sm[kTile];
+    constexpr int THREADS = TILE / 8;
+    constexpr int kHalfTile = TILE / 2;
+    static_assert((1 << K) <= kHalfTile, "blocks of 2^K elements must fit a chunk");
+    __shared__ double2 sm[TILE];
     using P = MidPlan<K>;
     const int jb = (int)(blockIdx.x % (unsigned)njobs);
     const long long wg = blockIdx.x / (unsigned)njobs;
@@ -580,7 +592,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__
         top -= B;
         if (B == 3) {
             int lo[1];
-            group_layout<K, 3>(lgH, idx, lo);
+            group_layout<K, 3, THREADS>(lgH, idx, lo);
             if (i == 0) {
 #pragma unroll
                 for (int c = 0; c < 8; c++) x[c] = load2(z, gidx(idx[c]));
@@ -590,7 +602,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__
             radix_step<3, false>(x, lo, lgH, tw);
         } else {
             int lo[2];
-            group_layout<K, 2>(lgH, idx, lo);
+            group_layout<K, 2, THREADS>(lgH, idx, lo);
             if (i == 0) {
 #pragma unroll
                 for (int c = 0; c < 8; c++) x[c] = load2(z, gidx(idx[c]));
@@ -605,7 +617,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__
     // central step: forward distances 4, 2, 1; cross terms; inverse 1, 2, 4
     {
         int lo[1];
-        group_layout<K, 3>(0, idx, lo);
+        group_layout<K, 3, THREADS>(0, idx, lo);
         get();
         radix_step<3, false>(x, lo, 0, tw);
         put();
@@ -638,12 +650,12 @@ __global__ __launch_bounds__(kThreads) void k_fft_mid(const LagJob* __restrict__
         __syncthreads();
         if (B == 3) {
             int lo[1];
-            group_layout<K, 3>(lgH, idx, lo);
+            group_layout<K, 3, THREADS>(lgH, idx, lo);
             get();
             radix_step<3, true>(x, lo, lgH, tw);
         } else {
             int lo[2];
-            group_layout<K, 2>(lgH, idx, lo);
+            group_layout<K, 2, THREADS>(lgH, idx, lo);
             get();
             radix_step<2, true>(x, lo, lgH, tw);
         }
@@ -684,20 +696,25 @@ __global__ __launch_bounds__(256) void k_lag_reduce(const LagJob* __restrict__ j
 }
 
 namespace {
-constexpr int kMaxFuse = 8;     // stages per pass below one full tile
-constexpr int kMaxLast = 10;    // stages of the contiguous last pass (2^10 <= kTile / 2)
-constexpr int kMinStrided = 4;  // strided passes: 4..6 stages, runs of 2 KiB..512 B
-constexpr int kMaxStrided = 6;
-
-constexpr int kMinLast = 7;
+constexpr int kMaxFuse = 8;        // stages per pass below one full tile
+constexpr int kBigTile = 4096;     // complex doubles per workgroup (64 KiB LDS, 512 threads)
+constexpr int kMinStrided = 4;     // strided passes: 4..6 stages on 2048-element tiles (runs of >= 32
+constexpr int kMaxSmallStrided = 6;  // residues, 512 B), 7..9 on 4096-element tiles (>= 8 residues, 128 B)
+constexpr int kMaxStrided = 9;
+constexpr int kMinLast = 7;        // the contiguous last pass: 7..10 stages on 2048-element tiles,
+constexpr int kMaxSmallLast = 10;  // 11 on 4096-element tiles
+constexpr int kMaxLast = 11;
 
 // Stage counts per pass, the pass over the largest spans first.  Below one
-// full tile: as even as possible.  With full tiles: the fewest passes such
-// that the strided passes take 4..6 stages each (each workgroup then reads
-// runs of at least 32 residues, 512 B) and the contiguous last pass 7..10;
-// within that, the strided passes take as many stages as they can (the
-// fused last pass costs more per stage), split as evenly as possible,
-// smaller first.
+// full tile: as even as possible.  With full tiles: the fewest passes
+// (strided passes of at most 9 stages, the last pass of at most 11).  A
+// strided pass of up to 6 stages on 2048-element tiles costs about the
+// same whatever its stage count (HBM-bound, runs of >= 512 B), one of 7-9
+// stages on 4096-element tiles reads shorter runs and costs up to ~1.7x
+// that, while the fused last pass grows slowly with its stages: so the
+// strided passes take up to 6 stages each, the last pass the rest up to 11,
+// and only what is left beyond that goes to wider strided passes; split as
+// evenly as possible, smaller first.
 int plan_passes(long long n, int* ks) {
     int L = 0;
     while ((1LL << L) < n) L++;
@@ -708,12 +725,11 @@ int plan_passes(long long n, int* ks) {
     }
     int ns = (L - kMaxLast + kMaxStrided - 1) / kMaxStrided;
     if (ns < 1) ns = 1;
-    int last = L - kMaxStrided * ns;
-    if (last < kMinLast) last = kMinLast;
-    if (L - last < kMinStrided * ns) last = L - kMinStrided * ns;
-    const int T = L - last;
+    int T = kMaxSmallStrided * ns < L - kMinLast ? kMaxSmallStrided * ns : L - kMinLast;
+    if (T < kMinStrided * ns) T = kMinStrided * ns;
+    if (L - T > kMaxLast) T = L - kMaxLast;
     for (int p = 0; p < ns; p++) ks[p] = T / ns + (p >= ns - T % ns ? 1 : 0);
-    ks[ns] = last;
+    ks[ns] = L - T;
     return ns + 1;
 }
 
@@ -723,41 +739,52 @@ int log2i(long long v) {
     return l;
 }
 
+// Tile of a full-tile pass of k stages over n elements.
+int tile_for(long long n, int k) { return k > kMaxSmallStrided && k <= kMaxStrided && n >= kBigTile ? kBigTile : kTile; }
+
 struct PassShape {
     int lgR, lgBn;
     long long wgs;
 };
 
 // R residues (<= q) and Bn span blocks (<= n / S_hi) filling one tile.
-PassShape pass_shape(long long n, long long S_hi, int k) {
+PassShape pass_shape(long long n, long long S_hi, int k, int tile) {
     const int lgn = log2i(n), lgS = log2i(S_hi), lgq = lgS - k;
-    const int lgWant = log2i(kTile) - k;
+    const int lgWant = log2i(tile) - k;
     const int lgR = lgWant < lgq ? lgWant : lgq;
     int lgBn = lgWant - lgR;
     if (lgBn > lgn - lgS) lgBn = lgn - lgS;
     return PassShape{lgR, lgBn, 1LL << ((lgn - lgS - lgBn) + (lgq - lgR))};
 }
 
-template <int K, bool DIT, int SRC, bool ARGMAX>
+template <int K, bool DIT, int SRC, bool ARGMAX, int TILE>
 void tile_pass(const PassShape& s, const LagJob* d_jobs, int njobs, long long n, long long S_hi, const double2* tw,
                const double* sums, unsigned long long* partials, hipStream_t stream) {
-    hipLaunchKernelGGL((k_fft_tile<K, DIT, SRC, ARGMAX>), dim3((unsigned)(s.wgs * njobs)), dim3(kThreads), 0, stream,
-                       d_jobs, log2i(n), log2i(S_hi), s.lgR, s.lgBn, tw, sums, partials, njobs);
+    hipLaunchKernelGGL((k_fft_tile<K, DIT, SRC, ARGMAX, TILE>), dim3((unsigned)(s.wgs * njobs)), dim3(TILE / 8), 0,
+                       stream, d_jobs, log2i(n), log2i(S_hi), s.lgR, s.lgBn, tw, sums, partials, njobs);
 }
 
 template <bool DIT, int SRC, bool ARGMAX>
 void pass(const LagJob* d_jobs, int njobs, long long n, long long S_hi, int k, const double2* tw, const double* sums,
           unsigned long long* partials, hipStream_t stream) {
-    const PassShape s = pass_shape(n, S_hi, k);
-    if (n >= kTile && k >= kMinStrided && k <= kMaxLast) {  // every tile full
+    const int tile = tile_for(n, k);
+    const PassShape s = pass_shape(n, S_hi, k, tile);
+    if (n >= tile && k >= kMinStrided && k <= kMaxSmallLast) {  // every tile full
+        if (tile == kBigTile) {
+            switch (k) {
+                case 7: tile_pass<7, DIT, SRC, ARGMAX, kBigTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+                case 8: tile_pass<8, DIT, SRC, ARGMAX, kBigTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+                default: tile_pass<9, DIT, SRC, ARGMAX, kBigTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            }
+        }
         switch (k) {
-            case 4: tile_pass<4, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
-            case 5: tile_pass<5, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
-            case 6: tile_pass<6, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
-            case 7: tile_pass<7, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
-            case 8: tile_pass<8, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
-            case 9: tile_pass<9, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
-            default: tile_pass<10, DIT, SRC, ARGMAX>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 4: tile_pass<4, DIT, SRC, ARGMAX, kTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 5: tile_pass<5, DIT, SRC, ARGMAX, kTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 6: tile_pass<6, DIT, SRC, ARGMAX, kTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 7: tile_pass<7, DIT, SRC, ARGMAX, kTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 8: tile_pass<8, DIT, SRC, ARGMAX, kTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            case 9: tile_pass<9, DIT, SRC, ARGMAX, kTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
+            default: tile_pass<10, DIT, SRC, ARGMAX, kTile>(s, d_jobs, njobs, n, S_hi, tw, sums, partials, stream); return;
         }
     }
     hipLaunchKernelGGL((k_fft_pass<DIT, SRC, ARGMAX>), dim3((unsigned)(s.wgs * njobs)), dim3(kThreads), 0, stream, d_jobs,
@@ -769,7 +796,7 @@ long long lag_partials_per_job(long long n) {
     if (n < 2) return 1;
     int ks[8];
     plan_passes(n, ks);
-    return pass_shape(n, n, ks[0]).wgs;  // the last DIT pass covers the spans up to n
+    return pass_shape(n, n, ks[0], tile_for(n, ks[0])).wgs;  // the last DIT pass covers the spans up to n
 }
 
 void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, const double* d_sums,
@@ -786,7 +813,7 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
     const int np = plan_passes(n, ks);
     // Fused middle for full tiles: the last forward pass, the cross terms and
     // the first inverse pass in one kernel (np >= 2 whenever n >= kTile).
-    const bool fuse = n >= kTile && np >= 2 && ks[np - 1] >= 7 && ks[np - 1] <= kMaxLast;
+    const bool fuse = n >= kTile && np >= 2 && ks[np - 1] >= kMinLast && ks[np - 1] <= kMaxLast;
     long long S = n;
     for (int p = 0; p < (fuse ? np - 1 : np); p++) {
         if (p == 0) pass<false, kSrcEnvelope, false>(d_jobs, njobs, n, S, ks[p], d_tw, d_sums, nullptr, stream);
@@ -796,10 +823,14 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
     if (fuse) {
         const dim3 g((unsigned)((n / kTile) * njobs));
         switch (ks[np - 1]) {
-            case 7: hipLaunchKernelGGL(k_fft_mid<7>, g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            case 8: hipLaunchKernelGGL(k_fft_mid<8>, g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            case 9: hipLaunchKernelGGL(k_fft_mid<9>, g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            default: hipLaunchKernelGGL(k_fft_mid<10>, g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 7: hipLaunchKernelGGL((k_fft_mid<7, kTile>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 8: hipLaunchKernelGGL((k_fft_mid<8, kTile>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 9: hipLaunchKernelGGL((k_fft_mid<9, kTile>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 10: hipLaunchKernelGGL((k_fft_mid<10, kTile>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            default:
+                hipLaunchKernelGGL((k_fft_mid<11, kBigTile>), dim3((unsigned)((n / kBigTile) * njobs)),
+                                   dim3(kBigTile / 8), 0, stream, d_jobs, d_tw, njobs);
+                break;
         }
     } else {
         hipLaunchKernelGGL(k_cross_terms, dim3((unsigned)((n + 255) / 256), njobs), dim3(256), 0, stream, d_jobs, n);
@@ -811,7 +842,7 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
         else pass<true, kSrcZ, false>(d_jobs, njobs, n, S, ks[p], d_tw, d_sums, nullptr, stream);
     }
     if (keep_xcorr) return;
-    const long long parts = pass_shape(n, n, ks[0]).wgs;
+    const long long parts = pass_shape(n, n, ks[0], tile_for(n, ks[0])).wgs;
     hipLaunchKernelGGL(k_lag_reduce, dim3(njobs), dim3(256), 0, stream, d_jobs, (const unsigned long long*)d_partials,
                        (int)parts, d_lags);
 }

@@ -122,10 +122,10 @@ void visqol_hip_server_destroy(visqol_hip_server *s);
 /* Spectrogram kernel block shapes.  visqol_hip_gammatone_shape describes
  * shape `shape` (waves per block, bands per lane, samples per staged chunk;
  * 64 frames per block) and returns the number of shapes.  Every shape gives
- * bit-identical spectrograms; an engine times them all on its first launch
- * large enough to fill the device and keeps the fastest for the process
- * (per device, precision, band count and window; kind 0: whole signals,
- * 1: realignment slices).  visqol_hip_gammatone_tuning reports that choice
+ * bit-identical spectrograms; an engine times them all on its first
+ * whole-signal launch large enough to fill the device and keeps the fastest
+ * for the process (per device, precision, band count and window; kind 0:
+ * whole signals; kind 1, realignment slices, always runs shape 0).  visqol_hip_gammatone_tuning reports that choice
  * (*chosen = -1 while untuned) and, if `ms` holds `cap` floats, the timed
  * launch of each shape; it returns the number of shapes.  The environment
  * variable VISQOL_HIP_GAMMATONE_SHAPE=k, read when a context is created,
