@@ -1117,8 +1117,18 @@ void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate
                 best[v] = round == 0 ? ms : std::min(best[v], ms);
             }
         }
+        // The fastest shape; within 2% of it, the one with the most bands
+        // per block (it reads each frame's samples from HBM fewer times,
+        // leaving bandwidth to work running beside it).
+        int fastest = 0;
         for (int v = 1; v < count; v++)
-            if (best[v] < best[chosen]) chosen = v;
+            if (best[v] < best[fastest]) fastest = v;
+        chosen = fastest;
+        auto bands = [](int v) { const GtShape g = gammatone_shape(v); return g.waves * g.bands_per_lane; };
+        for (int v = 0; v < count; v++)
+            if (best[v] <= 1.02f * best[fastest] &&
+                (bands(v) > bands(chosen) || (bands(v) == bands(chosen) && best[v] < best[chosen])))
+                chosen = v;
     } catch (...) {
         for (auto e : ev)
             if (e) (void)hipEventDestroy(e);

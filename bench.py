@@ -45,11 +45,11 @@ def make_pairs(n: int, workers: int):
         return list(ex.map(_gen, range(n), chunksize=4))
 
 
-def _traffic_ratio(path: Path):
-    """`traffic_per_algorithmic_byte = X` from a counter summary, or None."""
+def _traffic_ratio(path: Path, shape: str):
+    """`traffic_per_algorithmic_byte <shape> = X` from a counter summary, or None."""
     try:
         for ln in path.read_text().splitlines():
-            if ln.startswith("traffic_per_algorithmic_byte"):
+            if ln.startswith(f"traffic_per_algorithmic_byte {shape} ="):
                 return float(ln.split("=")[1])
     except OSError:
         return None
@@ -326,10 +326,15 @@ def main() -> int:
         ms_per_launch = prof["ms"] / prof["launches"]
         achieved = prof["flop"] / (prof["ms"] * 1e-3) / 1e12
         alg_bytes = prof["bytes"] / prof["launches"]
-        # HBM bytes per algorithmic byte of this kernel on this workload, from
-        # the committed FETCH_SIZE / WRITE_SIZE counter runs (copy-calibrated)
+        # HBM bytes per algorithmic byte of this kernel on this workload for
+        # the block shape the engine chose, from the committed FETCH_SIZE /
+        # WRITE_SIZE counter runs (copy-calibrated)
         traffic = None
-        ratio = _traffic_ratio(ROOT / "profiles" / "k_gammatone_fast_1024x10s48k_hbm_traffic.txt")
+        ratio = None
+        if "spectrogram" in autotune:
+            sh = autotune["spectrogram"]["chosen"]
+            ratio = _traffic_ratio(ROOT / "profiles" / "k_gammatone_fast_1024x10s48k_hbm_traffic.txt",
+                                   f"{sh['waves']}w{sh['bands_per_lane']}b{sh['chunk']}c")
         if ratio is not None and args.batch == 1024:
             traffic = round(ratio * alg_bytes)
         line["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
