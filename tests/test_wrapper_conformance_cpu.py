@@ -1,0 +1,130 @@
+"""The reference's callers would build and link against this library unchanged.
+
+The reference's C++ compat layer (src/cpp/src/visqol_compat.cc) needs absl
+and generated protobuf headers that this image lacks, so it cannot be built
+here; its Python wrapper is read, not run.  What both depend on is the C
+header, src/include/visqol.h, which is plain C: a probe compiled once
+against that header and once against this repository's include/visqol.h
+prints every struct size, alignment, field offset and field size and every
+enum constant the reference declares; the two outputs must be identical.
+A program built against the reference header must link against
+libvisqol.so and find all twelve functions with the header's prototypes.
+The error codes the wrappers map (visqol_compat.cc:19-36,
+visqol_lib_py.py:139-146) are the enum values compared here.
+
+These tests read the reference checkout and skip where it is absent.
+"""
+import os
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+LIB = ROOT / "akvisqol_amd" / "libvisqol.so"
+REF_INCLUDE = Path(os.environ.get("VISQOL_REFERENCE", "/root/reference")) / "src" / "include"
+
+pytestmark = [
+    pytest.mark.skipif(not (REF_INCLUDE / "visqol.h").exists(), reason="reference checkout not present"),
+    pytest.mark.skipif(not LIB.exists(), reason="libvisqol.so not built (run __graft_entry__.build())"),
+]
+
+FIELDS = {
+    "visqol_options": ["mode", "search_window_radius", "use_unscaled_speech_mos_mapping",
+                       "disable_global_alignment", "disable_realignment", "use_lattice_model", "model_path",
+                       "backend", "precision"],
+    "visqol_patch": ["similarity", "ref_patch_start_time", "ref_patch_end_time", "deg_patch_start_time",
+                     "deg_patch_end_time", "freq_band_means", "freq_band_stddevs", "freq_band_deg_energy"],
+    "visqol_result": ["moslqo", "vnsim", "alignment_lag_s", "num_bands", "fvnsim", "fvnsim10", "fstdnsim",
+                      "fvdegenergy", "center_freq_bands", "num_patches", "patch_sims"],
+}
+CONSTANTS = ["VISQOL_OK", "VISQOL_E_INVALID_ARGUMENT", "VISQOL_E_OUT_OF_MEMORY", "VISQOL_E_IO", "VISQOL_E_MODEL",
+             "VISQOL_E_SIGNAL", "VISQOL_E_BACKEND", "VISQOL_E_INTERNAL", "VISQOL_MODE_AUDIO", "VISQOL_MODE_SPEECH",
+             "VISQOL_BACKEND_CPU", "VISQOL_BACKEND_METAL", "VISQOL_BACKEND_CUDA", "VISQOL_BACKEND_VULKAN",
+             "VISQOL_PRECISION_EXACT", "VISQOL_PRECISION_FAST", "VISQOL_PRECISION_BALANCED",
+             "VISQOL_PRECISION_AUTO"]
+
+
+def _layout_probe() -> str:
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "visqol.h"', "int main(void) {"]
+    for t, fields in FIELDS.items():
+        lines.append(f'    printf("{t} %zu %zu\\n", sizeof({t}), _Alignof({t}));')
+        for f in fields:
+            lines.append(f'    printf("{t}.{f} %zu %zu\\n", offsetof({t}, {f}), sizeof((({t} *)0)->{f}));')
+    for c in CONSTANTS:
+        lines.append(f'    printf("{c} %d\\n", (int)({c}));')
+    lines += ["    return 0;", "}"]
+    return "\n".join(lines) + "\n"
+
+
+def _run_probe(tmp_path: Path, include: Path, tag: str) -> str:
+    src = tmp_path / f"layout_{tag}.c"
+    exe = tmp_path / f"layout_{tag}"
+    src.write_text(_layout_probe())
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{include}", str(src), "-o", str(exe)], check=True)
+    return subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+
+
+def test_header_layouts_and_constants_identical(tmp_path):
+    ref = _run_probe(tmp_path, REF_INCLUDE, "ref")
+    ours = _run_probe(tmp_path, ROOT / "include", "ours")
+    assert ref.count("\n") == 3 + sum(len(f) for f in FIELDS.values()) + len(CONSTANTS)
+    assert ours == ref
+
+
+LINK_PROBE = r"""
+#include <stdio.h>
+#include <string.h>
+#include "visqol.h"
+/* Every function of the reference header, taken through pointers of the
+   header's own prototypes. */
+static uint32_t (*p_abi)(void) = visqol_abi_version;
+static const char *(*p_version)(void) = visqol_version;
+static const char *(*p_last_error)(void) = visqol_last_error;
+static uint32_t (*p_backends)(void) = visqol_available_backends;
+static void (*p_defaults)(visqol_options *) = visqol_options_default;
+static int (*p_create)(const visqol_options *, visqol_ctx **) = visqol_create;
+static void (*p_destroy)(visqol_ctx *) = visqol_destroy;
+static int (*p_measure)(visqol_ctx *, const double *, size_t, const double *, size_t, uint32_t,
+                        visqol_result **) = visqol_measure;
+static int (*p_files)(visqol_ctx *, const char *, const char *, visqol_result **) = visqol_measure_files;
+static int (*p_batch)(visqol_ctx *, size_t, const double *const *, const size_t *, const double *const *,
+                      const size_t *, uint32_t, uint32_t, visqol_result **, int32_t *) = visqol_measure_batch;
+static int (*p_files_batch)(visqol_ctx *, size_t, const char *const *, const char *const *, uint32_t,
+                            visqol_result **, int32_t *) = visqol_measure_files_batch;
+static void (*p_free)(visqol_result *) = visqol_result_free;
+
+int main(void) {
+    visqol_options o;
+    p_defaults(&o);
+    /* argument checks that need no device, as capi.zig:170-371 orders them */
+    visqol_result *r = (visqol_result *)1;
+    const int bad_measure = p_measure(NULL, NULL, 0, NULL, 0, 48000, &r);
+    const int empty_batch = p_batch(NULL, 0, NULL, NULL, NULL, NULL, 48000, 0, NULL, NULL);
+    const int null_files = p_files(NULL, "a.wav", "b.wav", &r);
+    p_free(NULL);
+    p_destroy(NULL);
+    printf("abi %u\nversion %s\nradius %d\nlattice %u\nprecision %u\n", p_abi(), p_version(),
+           o.search_window_radius, (unsigned)o.use_lattice_model, o.precision);
+    printf("measure_null_ctx %d\nbatch_null_ctx %d\nfiles_null_ctx %d\n", bad_measure, empty_batch, null_files);
+    printf("last_error %s\n", p_last_error());
+    printf("have %d %d %d\n", p_backends != 0, p_create != 0, p_files_batch != 0);
+    return 0;
+}
+"""
+
+
+def test_reference_header_program_links_and_runs(tmp_path):
+    src = tmp_path / "link.c"
+    exe = tmp_path / "link"
+    src.write_text(LINK_PROBE)
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{REF_INCLUDE}", str(src), "-o", str(exe),
+                    f"-L{LIB.parent}", "-lvisqol", f"-Wl,-rpath,{LIB.parent}"], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = dict(line.split(" ", 1) for line in out.strip().splitlines())
+    assert got["abi"] == "1" and got["version"] == "0.1.0"
+    assert got["radius"] == "60" and got["lattice"] == "1" and got["precision"] == "3"
+    assert got["measure_null_ctx"] == "-1" and got["files_null_ctx"] == "-1"
+    assert got["batch_null_ctx"] == "-1"  # ctx is checked before n == 0 (capi.zig:344)
+    assert got["last_error"] == "ctx is NULL"
+    assert got["have"] == "1 1 1"
