@@ -540,7 +540,15 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         sj.push_back(SimJob{s.spec_ref, s.plane, s.sims, s.d_idx, s.d_lo, s.d_hi, s.np, s.nf_ref, s.nf_deg, s.width});
         dj.push_back(DpJob{s.sims, s.dp, s.bt, s.d_lo, s.d_hi, s.d_off, s.d_state, s.np, s.nf_deg});
         max_np = std::max(max_np, s.np);
-        for (int p = 0; p < s.np; p++) max_win = std::max(max_win, s.hi[p] - s.lo[p] + 1);
+        // widest union of the search windows of a similarity-table patch group
+        for (int p0 = 0; p0 < s.np; p0 += kSimPatchGroup) {
+            int lo = s.lo[p0], hi = s.hi[p0];
+            for (int p = p0 + 1; p < std::min(s.np, p0 + kSimPatchGroup); p++) {
+                lo = std::min(lo, s.lo[p]);
+                hi = std::max(hi, s.hi[p]);
+            }
+            max_win = std::max(max_win, hi - lo + 1);
+        }
     }
     bool timed = false;
     if (!scj.empty()) {

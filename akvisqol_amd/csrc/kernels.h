@@ -167,9 +167,14 @@ struct SimJob {
     const int* hi;           // [np]
     int np, nf_ref, nf_deg, width;
 };
+// Patches of one pair scored together by a similarity-table block: they
+// share the degraded signal's loads at each offset.
+constexpr int kSimPatchGroup = 2;
 // sims[p][off] for off in [lo[p], hi[p]] (accel.zig:126-146); d_terms holds
-// 4 * nb * patch * max_np * njobs doubles of ref-patch terms.  `fast`: fused
-// multiply-adds and one division per cell instead of the reference's order.
+// 4 * nb * patch * max_np * njobs doubles of ref-patch terms; max_window is
+// the widest union of the search windows of patches p..p+kSimPatchGroup-1
+// (p a multiple of kSimPatchGroup).  `fast`: fused multiply-adds and one
+// division per cell instead of the reference's order.
 void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int max_np, int max_window, double* d_terms,
                       bool fast, hipStream_t stream);
 

@@ -193,39 +193,50 @@ __device__ inline double sim_cell_fast(double mr, double rms, double ssr, double
     return num * q;
 }
 
-constexpr int kSimThreads = 256;
+constexpr int kSimThreads = 512;
 
-// measureSimilarityAt (nsim.zig:364-424) for one (patch, offset) per lane,
-// lanes on consecutive offsets.  The patch's terms and values sit in LDS
-// (read by broadcast).  Along a row the 3x3 taps slide one column per cell:
-// the products X = ref·deg of rows rm, r, rp at columns c-1, c, c+1
+// measureSimilarityAt (nsim.zig:364-424) for kSimPatchGroup patches of one
+// pair at one offset per lane, lanes on consecutive offsets of the union of
+// the patches' search windows (one window for every patch of a signal
+// shorter than the search radius).  The patches' terms and values sit in
+// LDS (read by broadcast).  Along a row the 3x3 taps slide one column per
+// cell: the products X = ref·deg of rows rm, r, rp at columns c-1, c, c+1
 // (patch-clamped) stay in registers, so each cell loads three degraded
 // values plus its window's mean and variance (from the plane that matches
-// its column: interior, a patch's first or its last).  Exact: the
-// reference's tap order and row-major serial sum, every product rounded on
-// its own.  FAST: the same cells with fused multiply-adds.
+// its column: interior, a patch's first or its last) — once for all the
+// group's patches, which see the same degraded columns at the same offset.
+// Each (patch, offset) keeps its own sum in row-major order.  Exact: the
+// reference's tap order and serial sum, every product rounded on its own.
+// FAST: the same cells with fused multiply-adds.
 template <bool FAST>
 __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restrict__ jobs, int nb, int S,
                                                            const double* __restrict__ terms, int max_np) {
-    extern __shared__ double sh[];  // [4][nb*S]: mu_ref, mu_ref², sigma_ref² (or root), ref
+    constexpr int PP = kSimPatchGroup;
+    extern __shared__ double sh[];  // [PP][4][nb*S]: mu_ref, mu_ref², sigma_ref² (or root), ref
     const SimJob j = jobs[blockIdx.z];
-    const int p = blockIdx.y;
-    if (p >= j.np) return;
-    const int lo = j.lo[p], hi = j.hi[p];
-    const int first = lo + (int)blockIdx.x * kSimThreads;
-    if (first > hi) return;  // uniform
+    const int p0 = blockIdx.y * PP;
+    if (p0 >= j.np) return;
+    const int npres = j.np - p0 < PP ? j.np - p0 : PP;  // patches present in this group
+    int lo_u = j.lo[p0], hi_u = j.hi[p0];
+    int lo[PP], hi[PP];
+#pragma unroll
+    for (int q = 0; q < PP; q++) {
+        lo[q] = q < npres ? j.lo[p0 + q] : 1;
+        hi[q] = q < npres ? j.hi[p0 + q] : 0;  // absent: empty window
+        if (q < npres) {
+            lo_u = lo[q] < lo_u ? lo[q] : lo_u;
+            hi_u = hi[q] > hi_u ? hi[q] : hi_u;
+        }
+    }
+    const int first = lo_u + (int)blockIdx.x * kSimThreads;
+    if (first > hi_u) return;  // uniform
     const int cells = nb * S;
-    const double* t = terms + ((long long)blockIdx.z * max_np + p) * 4 * cells;
-    for (int e = threadIdx.x; e < 4 * cells; e += kSimThreads) sh[e] = t[e];
+    const double* t = terms + ((long long)blockIdx.z * max_np + p0) * 4 * cells;
+    for (int e = threadIdx.x; e < 4 * cells * npres; e += kSimThreads) sh[e] = t[e];
     __syncthreads();
-    const double* s_mr = sh;
-    const double* s_rms = sh + cells;
-    const double* s_srs = sh + 2 * cells;
-    const double* s_ref = sh + 3 * cells;
 
     const int off = first + (int)threadIdx.x;
-    const bool active = off <= hi;
-    const int o = active ? off : lo;  // idle lanes shadow a valid column
+    const int o = off <= hi_u ? off : lo_u;  // idle lanes shadow a valid column
     const int w = j.width;
     const long long n = (long long)nb * w;
     // Loads are (uniform column pointer)[per-lane 32-bit index]: plane sizes
@@ -237,53 +248,63 @@ __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restr
     const double* var_first = j.plane + kPlaneVarFirst * n;
     const double* mu_last = j.plane + kPlaneMuLast * n;
     const double* var_last = j.plane + kPlaneVarLast * n;
-    double sum = 0.0;
+    double sum[PP];
+#pragma unroll
+    for (int q = 0; q < PP; q++) sum[q] = 0.0;
     enum : int { kFirst, kMid, kLast };
     for (int r = 0; r < nb; r++) {
         const int rm = r == 0 ? 0 : r - 1, rp = r + 1 >= nb ? nb - 1 : r + 1;
         const unsigned q0 = (unsigned)(rm * w + o), q1 = (unsigned)(r * w + o), q2 = (unsigned)(rp * w + o);
-        const double* f0 = s_ref + rm * S;
-        const double* f1 = s_ref + r * S;
-        const double* f2 = s_ref + rp * S;
-        // Three column slots of products; cell c uses slots (a, b, c) as
-        // columns c-1, c, c+1, then refills slot a with column c+2 (the
-        // next cell's c+1): the roles rotate.
-        double X[3][3];
-        auto fill = [&](double (&Xd)[3], int col) {
+        // Three column slots of products per patch; cell c uses slots
+        // (a, b, c) as columns c-1, c, c+1, then refills slot a with column
+        // c+2 (the next cell's c+1): the roles rotate.
+        double X[3][PP][3];
+        auto fill = [&](double (&Xd)[PP][3], int col) {
             const auto* g = as_global(pad + col);
-            Xd[0] = f0[col] * g[q0];
-            Xd[1] = f1[col] * g[q1];
-            Xd[2] = f2[col] * g[q2];
+            const double d0 = g[q0], d1 = g[q1], d2 = g[q2];
+#pragma unroll
+            for (int q = 0; q < PP; q++) {
+                const double* f = sh + (q * 4 + 3) * cells;
+                Xd[q][0] = f[rm * S + col] * d0;
+                Xd[q][1] = f[r * S + col] * d1;
+                Xd[q][2] = f[rp * S + col] * d2;
+            }
         };
         fill(X[1], 0);
-        X[0][0] = X[1][0];
-        X[0][1] = X[1][1];
-        X[0][2] = X[1][2];
+#pragma unroll
+        for (int q = 0; q < PP; q++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) X[0][q][k] = X[1][q][k];
         fill(X[2], S > 1 ? 1 : 0);
-        auto cell = [&](auto kind, int c, double (&Xa)[3], double (&Xb)[3], double (&Xc)[3]) {
+        auto cell = [&](auto kind, int c, double (&Xa)[PP][3], double (&Xb)[PP][3], double (&Xc)[PP][3]) {
             constexpr int K = decltype(kind)::value;
             const double* pm = K == kFirst ? mu_first : (K == kLast ? mu_last : mu);
             const double* pv = K == kFirst ? var_first : (K == kLast ? var_last : var);
             const double md = as_global(pm + c)[q1];
             const double v = as_global(pv + c)[q1];
-            double crd;
-            if (FAST) {
-                crd = VQ_WC * Xa[0];
-                crd = __builtin_fma(VQ_WE, Xb[0], crd);
-                crd = __builtin_fma(VQ_WC, Xc[0], crd);
-                crd = __builtin_fma(VQ_WE, Xa[1], crd);
-                crd = __builtin_fma(VQ_WM, Xb[1], crd);
-                crd = __builtin_fma(VQ_WE, Xc[1], crd);
-                crd = __builtin_fma(VQ_WC, Xa[2], crd);
-                crd = __builtin_fma(VQ_WE, Xb[2], crd);
-                crd = __builtin_fma(VQ_WC, Xc[2], crd);
-            } else {
-                crd = VQ_WC * Xa[0] + VQ_WE * Xb[0] + VQ_WC * Xc[0] + VQ_WE * Xa[1] + VQ_WM * Xb[1] + VQ_WE * Xc[1] +
-                      VQ_WC * Xa[2] + VQ_WE * Xb[2] + VQ_WC * Xc[2];
-            }
             const int e = r * S + c;
-            if (FAST) sum += sim_cell_fast(s_mr[e], s_rms[e], s_srs[e], md, v, crd);
-            else sum += sim_cell(s_mr[e], s_rms[e], s_srs[e], md, md * md, v, crd);
+#pragma unroll
+            for (int q = 0; q < PP; q++) {
+                double crd;
+                if (FAST) {
+                    crd = VQ_WC * Xa[q][0];
+                    crd = __builtin_fma(VQ_WE, Xb[q][0], crd);
+                    crd = __builtin_fma(VQ_WC, Xc[q][0], crd);
+                    crd = __builtin_fma(VQ_WE, Xa[q][1], crd);
+                    crd = __builtin_fma(VQ_WM, Xb[q][1], crd);
+                    crd = __builtin_fma(VQ_WE, Xc[q][1], crd);
+                    crd = __builtin_fma(VQ_WC, Xa[q][2], crd);
+                    crd = __builtin_fma(VQ_WE, Xb[q][2], crd);
+                    crd = __builtin_fma(VQ_WC, Xc[q][2], crd);
+                } else {
+                    crd = VQ_WC * Xa[q][0] + VQ_WE * Xb[q][0] + VQ_WC * Xc[q][0] + VQ_WE * Xa[q][1] +
+                          VQ_WM * Xb[q][1] + VQ_WE * Xc[q][1] + VQ_WC * Xa[q][2] + VQ_WE * Xb[q][2] +
+                          VQ_WC * Xc[q][2];
+                }
+                const double* tq = sh + q * 4 * cells;
+                if (FAST) sum[q] += sim_cell_fast(tq[e], tq[cells + e], tq[2 * cells + e], md, v, crd);
+                else sum[q] += sim_cell(tq[e], tq[cells + e], tq[2 * cells + e], md, md * md, v, crd);
+            }
             if (K != kLast) fill(Xa, c + 2 < S ? c + 2 : S - 1);
         };
         using First = std::integral_constant<int, kFirst>;
@@ -311,15 +332,18 @@ __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restr
             }
         }
     }
-    if (active) j.sims[(long long)p * j.nf_deg + off] = sum / (double)(nb * S);
+#pragma unroll
+    for (int q = 0; q < PP; q++)
+        if (off >= lo[q] && off <= hi[q]) j.sims[(long long)(p0 + q) * j.nf_deg + off] = sum[q] / (double)(nb * S);
 }
 
 void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int max_np, int max_window, double* d_terms,
                       bool fast, hipStream_t stream) {
     if (njobs == 0 || max_np == 0 || max_window == 0) return;
     const dim3 gt((nb * patch + 255) / 256, max_np, njobs);
-    const dim3 grid((max_window + kSimThreads - 1) / kSimThreads, max_np, njobs);
-    const size_t lds = sizeof(double) * 4 * nb * patch;
+    const dim3 grid((max_window + kSimThreads - 1) / kSimThreads, (max_np + kSimPatchGroup - 1) / kSimPatchGroup,
+                    njobs);
+    const size_t lds = sizeof(double) * 4 * nb * patch * kSimPatchGroup;
     if (fast) {
         hipLaunchKernelGGL(k_ref_terms<true>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
         hipLaunchKernelGGL(k_sim_table<true>, grid, dim3(kSimThreads), lds, stream, d_jobs, nb, patch,
