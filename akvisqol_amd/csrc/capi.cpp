@@ -187,8 +187,9 @@ namespace vq_capi {
 // Engine options and models for validated options (visqol.zig:346-359 model
 // selection; capi.zig:188-203): the embedded model unless model_path names
 // one, read only in audio mode or in speech mode with the lattice.
-int build_engine(const visqol_options& o, EngineOptions& eo, std::unique_ptr<Engine>& engine) {
+int build_engine(const visqol_options& o, EngineOptions& eo, std::unique_ptr<Engine>& engine, bool interleave) {
     eo = EngineOptions();
+    eo.interleave_chunks = interleave;
     eo.mode = (int)o.mode;
     eo.radius = o.search_window_radius > 0 ? o.search_window_radius : 60;
     eo.unscaled_speech = o.use_unscaled_speech_mos_mapping != 0;

@@ -17,6 +17,9 @@ visqol_result* box(const vq::PairResult& r);
 // A gfx950 HIP device is current.
 bool device_ok(std::string& why);
 // Engine and models for validated options; a visqol.h code.
-int build_engine(const visqol_options& o, vq::EngineOptions& eo, std::unique_ptr<vq::Engine>& engine);
+// `interleave`: batches of several chunks run every other chunk on a helper
+// engine (off for the server, whose workers already overlap).
+int build_engine(const visqol_options& o, vq::EngineOptions& eo, std::unique_ptr<vq::Engine>& engine,
+                 bool interleave = true);
 
 }  // namespace vq_capi

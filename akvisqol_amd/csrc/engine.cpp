@@ -100,7 +100,7 @@ void PinnedArena::reset() {
 // ---------------------------------------------------------------- engine
 Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
                std::unique_ptr<LatticeModel, LatticeDeleter> lattice)
-    : opts_(o), svr_(std::move(svr)), lattice_(std::move(lattice)) {
+    : opts_(o), svr_(std::move(svr)), lattice_(lattice.release(), LatticeDeleter()) {
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     if (const char* e = getenv("VISQOL_HIP_GAMMATONE_SHAPE")) {
         char* end = nullptr;
@@ -109,7 +109,28 @@ Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
     }
 }
 
+Engine::Engine(HelperTag, Engine& parent)
+    : gt_force_(parent.gt_force_), opts_(parent.opts_), svr_(parent.svr_), lattice_(parent.lattice_) {
+    opts_.interleave_chunks = false;
+    stream_ = parent.stream_;
+    owns_stream_ = false;
+    enqueue_mu_ = parent.enqueue_mu_;
+    profiling_ = parent.profiling_;
+}
+
+void Engine::merge_profile(const Profile& p) {
+    prof_.gammatone_ms += p.gammatone_ms;
+    prof_.gammatone_flop += p.gammatone_flop;
+    prof_.gammatone_bytes += p.gammatone_bytes;
+    prof_.launches += p.launches;
+    prof_.sim_ms += p.sim_ms;
+    prof_.sim_cells += p.sim_cells;
+    prof_.sim_launches += p.sim_launches;
+    for (int i = 0; i < Profile::kPhases; i++) prof_.phase_ms[i] += p.phase_ms[i];
+}
+
 Engine::~Engine() {
+    helper_.reset();  // before the stream it shares
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto e : ev_)
         if (e) (void)hipEventDestroy(e);
@@ -119,7 +140,7 @@ Engine::~Engine() {
         (void)hipFree(kv.second->hann);
     }
     for (auto& kv : twiddles_) (void)hipFree(kv.second);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (stream_ && owns_stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Engine::check(hipError_t e, const char* what) {
@@ -264,6 +285,7 @@ void fail(PairResult& r, PairState& s, int code, const char* msg) {
 void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages) {
     // Chunks bounded by samples so a batch of any size fits in HBM.
     const long long kChunkSamples = 1LL << 28;
+    std::vector<std::pair<int, int>> chunks;
     int start = 0;
     while (start < n) {
         long long samples = 0;
@@ -272,9 +294,42 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
             samples += pairs[end].ref_len + pairs[end].deg_len;
             end++;
         }
-        run_chunk(pairs + start, end - start, sr, out + start, want_stages);
+        chunks.emplace_back(start, end);
         start = end;
     }
+    if (chunks.size() < 2 || !opts_.interleave_chunks || want_stages) {
+        for (const auto& c : chunks) run_chunk(pairs + c.first, c.second - c.first, sr, out + c.first, want_stages);
+        return;
+    }
+    // Odd chunks on the helper's thread, even ones here; both enqueue to the
+    // one in-order stream, so kernels still run one at a time while either
+    // side's host work proceeds.
+    if (!helper_) helper_.reset(new Engine(HelperTag{}, *this));
+    helper_->profiling_ = profiling_;
+    std::exception_ptr helper_error;
+    std::thread t([&] {
+        try {
+            for (size_t k = 1; k < chunks.size(); k += 2)
+                helper_->run_chunk(pairs + chunks[k].first, chunks[k].second - chunks[k].first, sr, out + chunks[k].first,
+                                   false);
+        } catch (...) {
+            helper_error = std::current_exception();
+        }
+    });
+    std::exception_ptr own_error;
+    try {
+        for (size_t k = 0; k < chunks.size(); k += 2)
+            run_chunk(pairs + chunks[k].first, chunks[k].second - chunks[k].first, sr, out + chunks[k].first, false);
+    } catch (...) {
+        own_error = std::current_exception();
+    }
+    t.join();
+    if (profiling_) {
+        merge_profile(helper_->prof_);
+        helper_->prof_ = Profile();
+    }
+    if (own_error) std::rethrow_exception(own_error);
+    if (helper_error) std::rethrow_exception(helper_error);
 }
 
 void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages) {
@@ -554,14 +609,19 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     if (!scj.empty()) {
         launch_scale(upload(scj), (int)scj.size(), max_len, stream_);
         GtJob* d_gj = upload(gj);
-        if (profiling_) {
-            if (!ev_[0])
-                for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
-            check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+        if (profiling_ && !ev_[0])
+            for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+        {
+            // the events bracket this launch alone on a shared stream
+            std::unique_lock<std::recursive_mutex> g(*enqueue_mu_, std::defer_lock);
+            if (profiling_) {
+                g.lock();
+                check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+            }
+            gammatone(d_gj, (int)gj.size(), max_nf, dr, 0);
+            if (profiling_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
         }
-        gammatone(d_gj, (int)gj.size(), max_nf, dr, 0);
         if (profiling_) {
-            check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
             timed = true;
             for (const GtJob& g : gj) {
                 prof_.gammatone_flop += 30.0 * W * nb * (double)g.nf;
@@ -581,10 +641,16 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         launch_planes(upload(plj), (int)plj.size(), nb, S, max_width, opts_.fast, stream_);
         double* terms = (double*)arena_.alloc(sizeof(double) * 4 * (size_t)nb * S * max_np * sj.size());
         const SimJob* d_sj = upload(sj);
-        if (timed) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
-        launch_sim_table(d_sj, (int)sj.size(), nb, S, max_np, max_win, terms, opts_.fast, stream_);
+        {
+            std::unique_lock<std::recursive_mutex> g(*enqueue_mu_, std::defer_lock);
+            if (timed) {
+                g.lock();
+                check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
+            }
+            launch_sim_table(d_sj, (int)sj.size(), nb, S, max_np, max_win, terms, opts_.fast, stream_);
+            if (timed) check(hipEventRecord(ev_[3], stream_), "hipEventRecord");
+        }
         if (timed) {
-            check(hipEventRecord(ev_[3], stream_), "hipEventRecord");
             for (int i = 0; i < n; i++)
                 if (st[i].live)
                     for (int p = 0; p < st[i].np; p++)
@@ -1112,6 +1178,8 @@ void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate
     int chosen = 0;
     try {
         for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        // nothing else enters the stream between the events
+        std::lock_guard<std::recursive_mutex> eg(*enqueue_mu_);
         for (int round = 0; round < 2; round++) {
             check(hipEventRecord(ev[0], stream_), "hipEventRecord");
             for (int v = 0; v < count; v++) {

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,10 @@ struct EngineOptions {
     bool no_realign = false;
     bool lattice = true;
     bool fast = false;
+    // Batches of several chunks: a second engine on the same stream scores
+    // every other chunk from its own host thread, so one chunk's host work
+    // (planning, decoding, aggregation) overlaps the other's kernels.
+    bool interleave_chunks = true;
 };
 
 // A reference prepared for scoring many degraded signals against it
@@ -173,11 +178,26 @@ class Engine {
     };
     GtTuning gammatone_tuning(uint32_t sample_rate, int kind);
 
-    void set_profiling(bool on) { profiling_ = on; }
+    void set_profiling(bool on) {
+        profiling_ = on;
+        if (helper_) helper_->set_profiling(on);
+    }
     Profile profile() const { return prof_; }
-    void reset_profile() { prof_ = Profile(); }
+    void reset_profile() {
+        prof_ = Profile();
+        if (helper_) helper_->reset_profile();
+    }
 
    private:
+    // A helper for interleaved chunks: shares the parent's options, models,
+    // stream and enqueue lock; owns its device arena, staging and tables.
+    struct HelperTag {};
+    Engine(HelperTag, Engine& parent);
+    std::unique_ptr<Engine> helper_;
+    std::recursive_mutex own_enqueue_mu_;
+    std::recursive_mutex* enqueue_mu_ = &own_enqueue_mu_;  // held around timed launches on a shared stream
+    bool owns_stream_ = true;
+    void merge_profile(const Profile& p);
     bool profiling_ = false;
     std::chrono::steady_clock::time_point lap_t_;
     void lap(int phase);
@@ -212,8 +232,8 @@ class Engine {
     int gt_force_ = -1;  // block shape forced by VISQOL_HIP_GAMMATONE_SHAPE, else -1
 
     EngineOptions opts_;
-    std::unique_ptr<SvrModel> svr_;
-    std::unique_ptr<LatticeModel, LatticeDeleter> lattice_;
+    std::shared_ptr<const SvrModel> svr_;
+    std::shared_ptr<const LatticeModel> lattice_;
     hipStream_t stream_ = nullptr;
     DeviceArena arena_;
     PinnedArena pinned_;

@@ -161,7 +161,7 @@ void visqol_hip_server::work() {
         if (!eng[mode]) {
             visqol_options o = opts;
             o.mode = (uint32_t)mode;
-            rc = build_engine(o, eo[mode], eng[mode]);
+            rc = build_engine(o, eo[mode], eng[mode], false);
         }
         std::vector<PairInput> in(batch.size());
         std::vector<PairResult> out(batch.size());

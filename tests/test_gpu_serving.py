@@ -184,3 +184,20 @@ def test_prepared_references_equal_batch(native, synth, mode, precision):
     rc, res, _ = ctx.measure_prepared(prep, [pairs[0][1]], ref_index=[len(refs)])
     assert rc == native.E_INVALID_ARGUMENT
     prep.free()
+
+
+@pytest.mark.parametrize("precision", ["fast", "exact"])
+def test_multi_chunk_batch_equals_single_chunk(native, synth, precision):
+    """A batch of more than 512 pairs runs as several chunks, every other
+    chunk on a helper engine sharing the stream: each pair's result equals
+    the same pair scored in a one-chunk batch, bit for bit."""
+    base = [synth.audio_pair(50 + s, duration=1.2, snr_db=12.0 + 3 * s, delay=20 * s) for s in range(8)]
+    pairs = base * 65  # 520 pairs: chunks of 512 and 8
+    ctx = native.Context(0, precision=precision)
+    say(f"multi-chunk batch precision={precision}: {len(pairs)} pairs")
+    rc, got, st = ctx.measure_batch([p[0] for p in pairs], [p[1] for p in pairs], 48000)
+    assert rc == 0 and set(st) == {0}
+    rc, want, _ = ctx.measure_batch([p[0] for p in base], [p[1] for p in base], 48000)
+    assert rc == 0
+    for i, g in enumerate(got):
+        _same(g, want[i % len(base)])
