@@ -111,17 +111,26 @@ __device__ inline void band_step_fir(BandState& b, FirState& f, const FirCoef& c
 
 // One wave's share of a block: V valid slots, the first M of them in DF2T
 // form.  Every variant stages its part of the tile and meets the block's
-// barriers the same number of times.
-template <bool FAST, int WAVES, int BPL, int CHUNK, int M, int V>
-__device__ __forceinline__ void gt_run(const GtJob& job, const BandCoef* __restrict__ coef,
+// barriers the same number of times.  PAIR: the block holds 32 frames of
+// two jobs (tile rows and lanes 0-31 job a, 32-63 job b), for launches whose
+// jobs have at most 32 frames; otherwise 64 frames of job a.
+template <bool FAST, int WAVES, int BPL, int CHUNK, bool PAIR, int M, int V>
+__device__ __forceinline__ void gt_run(const GtJob& ja, const GtJob& jb, const BandCoef* __restrict__ coef,
                                        const FirCoef* __restrict__ fir, const double* __restrict__ hann, int band0,
                                        int window, int hop, double (*tile)[CHUNK + 1]) {
     constexpr int kThreads = 64 * WAVES;
     constexpr int kStage = kFramesPerWave * CHUNK / kThreads;  // tile elements staged per thread
     constexpr int kSlots = V > 0 ? V : 1;
-    const int f0 = blockIdx.x * kFramesPerWave;
+    constexpr int kFramesPerJob = PAIR ? kFramesPerWave / 2 : kFramesPerWave;
+    const int f0 = blockIdx.x * kFramesPerJob;
+    // tile row (= lane) -> job and frame
+    auto row_b = [](int row) { return PAIR && row >= kFramesPerWave / 2; };
+    auto row_frame = [&](int row) { return f0 + (PAIR ? row & (kFramesPerWave / 2 - 1) : row); };
+    // a job holding frame f0 (the kernel exits before any block that has none)
+    const GtJob& jv = ja.nf > f0 ? ja : jb;
     const int lane = threadIdx.x & 63;
-    const int frame = f0 + lane;
+    const GtJob& job = row_b(lane) ? jb : ja;
+    const int frame = row_frame(lane);
     BandCoef k[kSlots];
     FirCoef c[kSlots];
     BandState st[kSlots];
@@ -141,9 +150,15 @@ __device__ __forceinline__ void gt_run(const GtJob& job, const BandCoef* __restr
     // from clamped in-range addresses) before the current chunk's recurrence
     // runs; masking and x·hann happen at the LDS write.
     constexpr int kRowsPerPass = kThreads / CHUNK;
-    const auto* sig = as_global(job.sig);
     const int ti = threadIdx.x % CHUNK;
     const int fr0 = threadIdx.x / CHUNK;
+    // A staged row past its job's frames reads frame f0 of a job that has
+    // it, and is masked at the LDS write.
+    auto row_valid = [&](int fr) { return row_frame(fr) < (row_b(fr) ? jb.nf : ja.nf); };
+    auto row_src = [&](int fr) -> const double* {
+        if (!PAIR) return ja.sig + (size_t)(row_valid(fr) ? row_frame(fr) : f0) * hop;
+        return row_valid(fr) ? (row_b(fr) ? jb.sig : ja.sig) + (size_t)row_frame(fr) * hop : jv.sig + (size_t)f0 * hop;
+    };
     double v[kStage];
     double h = 0.0;
     auto load_chunk = [&](int i0) {
@@ -151,11 +166,7 @@ __device__ __forceinline__ void gt_run(const GtJob& job, const BandCoef* __restr
         const int ic = i < window ? i : 0;
         h = hann[ic];
 #pragma unroll
-        for (int u = 0; u < kStage; u++) {
-            const int fr = fr0 + u * kRowsPerPass;
-            const int fc = f0 + fr < job.nf ? f0 + fr : f0;
-            v[u] = sig[(size_t)fc * hop + ic];
-        }
+        for (int u = 0; u < kStage; u++) v[u] = as_global(row_src(fr0 + u * kRowsPerPass))[ic];
     };
     load_chunk(0);
     for (int i0 = 0; i0 < window; i0 += CHUNK) {
@@ -165,7 +176,7 @@ __device__ __forceinline__ void gt_run(const GtJob& job, const BandCoef* __restr
 #pragma unroll
         for (int u = 0; u < kStage; u++) {
             const int fr = fr0 + u * kRowsPerPass;
-            tile[fr][ti] = (in && f0 + fr < job.nf) ? v[u] * h : 0.0;
+            tile[fr][ti] = (in && row_valid(fr)) ? v[u] * h : 0.0;
         }
         __syncthreads();
         if (i0 + CHUNK < window) load_chunk(i0 + CHUNK);
@@ -192,46 +203,56 @@ __device__ __forceinline__ void gt_run(const GtJob& job, const BandCoef* __restr
 
 // Runs gt_run for the wave's valid slot count v (wave-uniform), VV = BPL
 // down to 0, with min(MM, VV) DF2T slots.
-template <bool FAST, int WAVES, int BPL, int CHUNK, int MM, int VV = BPL>
-__device__ __forceinline__ void gt_dispatch(int v, const GtJob& job, const BandCoef* __restrict__ coef,
+template <bool FAST, int WAVES, int BPL, int CHUNK, bool PAIR, int MM, int VV = BPL>
+__device__ __forceinline__ void gt_dispatch(int v, const GtJob& ja, const GtJob& jb, const BandCoef* __restrict__ coef,
                                             const FirCoef* __restrict__ fir, const double* __restrict__ hann,
                                             int band0, int window, int hop, double (*tile)[CHUNK + 1]) {
     if (v == VV) {
-        gt_run<FAST, WAVES, BPL, CHUNK, (MM < VV ? MM : VV), VV>(job, coef, fir, hann, band0, window, hop, tile);
+        gt_run<FAST, WAVES, BPL, CHUNK, PAIR, (MM < VV ? MM : VV), VV>(ja, jb, coef, fir, hann, band0, window, hop,
+                                                                        tile);
     } else if constexpr (VV > 0) {
-        gt_dispatch<FAST, WAVES, BPL, CHUNK, MM, VV - 1>(v, job, coef, fir, hann, band0, window, hop, tile);
+        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, MM, VV - 1>(v, ja, jb, coef, fir, hann, band0, window, hop, tile);
     }
 }
 
-template <bool FAST, int WAVES, int BPL, int CHUNK>
-__global__ __launch_bounds__(64 * WAVES) void k_gammatone(const GtJob* __restrict__ jobs, const BandCoef* __restrict__ coef,
+template <bool FAST, int WAVES, int BPL, int CHUNK, bool PAIR>
+__global__ __launch_bounds__(64 * WAVES) void k_gammatone(const GtJob* __restrict__ jobs, int njobs,
+                                                          const BandCoef* __restrict__ coef,
                                                           const FirCoef* __restrict__ fir,
                                                           const double* __restrict__ hann, int nb, int window, int hop) {
     static_assert(kFramesPerWave * CHUNK % (64 * WAVES) == 0, "tile must split evenly over the block");
     static_assert(kDf2tBands % WAVES == 0 && WAVES * BPL >= kDf2tBands,
                   "the DF2T bands must fill whole slots of block row 0");
-    const GtJob job = jobs[blockIdx.z];
-    if ((int)blockIdx.x * kFramesPerWave >= job.nf) return;  // whole block idle: uniform exit
+    const int za = PAIR ? 2 * (int)blockIdx.z : (int)blockIdx.z;
+    const GtJob ja = jobs[za];
+    GtJob jb = ja;
+    if (PAIR) {
+        if (za + 1 < njobs) jb = jobs[za + 1];
+        else jb.nf = 0;  // odd job count: lanes 32-63 idle
+    }
+    const int frames = PAIR ? kFramesPerWave / 2 : kFramesPerWave;
+    if ((int)blockIdx.x * frames >= (ja.nf > jb.nf ? ja.nf : jb.nf)) return;  // whole block idle: uniform exit
     __shared__ double tile[kFramesPerWave][CHUNK + 1];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int band0 = blockIdx.y * (WAVES * BPL) + wave;
     const int v = band0 < nb ? min(BPL, (nb - band0 + WAVES - 1) / WAVES) : 0;
     constexpr int kM0 = FAST ? (BPL < kDf2tBands / WAVES ? BPL : kDf2tBands / WAVES) : BPL;
     if (!FAST || blockIdx.y == 0)
-        gt_dispatch<FAST, WAVES, BPL, CHUNK, kM0>(v, job, coef, fir, hann, band0, window, hop, tile);
+        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, kM0>(v, ja, jb, coef, fir, hann, band0, window, hop, tile);
     else
-        gt_dispatch<FAST, WAVES, BPL, CHUNK, 0>(v, job, coef, fir, hann, band0, window, hop, tile);
+        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, 0>(v, ja, jb, coef, fir, hann, band0, window, hop, tile);
 }
 
 namespace {
-template <bool FAST, int V>
+template <bool FAST, int V, bool PAIR = false>
 void launch_shape(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const FirCoef* d_fir,
                   const double* d_hann, int nb, int window, int hop, hipStream_t stream) {
     constexpr GtShape sh = kGtShapes[V];
-    dim3 grid((max_nf + kFramesPerWave - 1) / kFramesPerWave, (nb + sh.waves * sh.bands_per_lane - 1) / (sh.waves * sh.bands_per_lane),
-              njobs);
-    hipLaunchKernelGGL((k_gammatone<FAST, sh.waves, sh.bands_per_lane, sh.chunk>), grid, dim3(64 * sh.waves), 0, stream,
-                       d_jobs, d_coef, d_fir, d_hann, nb, window, hop);
+    constexpr int frames = PAIR ? kFramesPerWave / 2 : kFramesPerWave;
+    dim3 grid((max_nf + frames - 1) / frames, (nb + sh.waves * sh.bands_per_lane - 1) / (sh.waves * sh.bands_per_lane),
+              PAIR ? (njobs + 1) / 2 : njobs);
+    hipLaunchKernelGGL((k_gammatone<FAST, sh.waves, sh.bands_per_lane, sh.chunk, PAIR>), grid, dim3(64 * sh.waves), 0,
+                       stream, d_jobs, njobs, d_coef, d_fir, d_hann, nb, window, hop);
 }
 
 using GtLaunch = void (*)(const GtJob*, int, int, const BandCoef*, const FirCoef*, const double*, int, int, int,
@@ -252,6 +273,11 @@ GtShape gammatone_shape(int v) { return kGtShapes[v >= 0 && v < kGtShapeCount ? 
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const FirCoef* d_fir,
                       const double* d_hann, int nb, int window, int hop, bool fast, int shape, hipStream_t stream) {
     if (njobs == 0 || max_nf == 0) return;
+    if (max_nf <= kFramesPerWave / 2 && njobs > 1) {  // short jobs (realignment patches): two per block
+        if (fast) launch_shape<true, 0, true>(d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
+        else launch_shape<false, 0, true>(d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
+        return;
+    }
     const int v = shape >= 0 && shape < kGtShapeCount ? shape : 0;
     (fast ? kFastLaunch : kExactLaunch)[v](d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
 }
