@@ -21,8 +21,8 @@ LIB = HERE / "libvisqol.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["gammatone.hip", "signal.hip", "xcorr.hip", "nsim.hip", "engine.cpp", "host_plan.cpp", "models.cpp", "wav.cpp",
-           "capi.cpp", "server.cpp"]
+SOURCES = ["gammatone.hip", "signal.hip", "xcorr.hip", "nsim.hip", "adjoint.hip", "engine.cpp", "engine_grad.cpp",
+           "host_plan.cpp", "models.cpp", "wav.cpp", "capi.cpp", "server.cpp", "grad.cpp"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -16,6 +16,7 @@
 #include "../../include/visqol_hip.h"
 #include "capi_internal.h"
 #include "engine.h"
+#include "grad.h"
 #include "wav.h"
 
 using namespace vq;
@@ -131,14 +132,17 @@ bool read_file(const std::string& path, std::string& out) {
     return true;
 }
 
-int run_pairs(visqol_ctx* c, const PairInput* pairs, size_t n, uint32_t sr, std::vector<PairResult>& res) {
+int run_pairs(visqol_ctx* c, const PairInput* pairs, size_t n, uint32_t sr, std::vector<PairResult>& res,
+              double* const* grads = nullptr) {
     res.assign(n, PairResult());
     if (sr == 0) {
         for (auto& r : res) { r.status = VISQOL_E_INVALID_ARGUMENT; r.error = "InvalidInput"; }
+        for (size_t i = 0; grads && i < n; i++)
+            if (grads[i]) std::fill(grads[i], grads[i] + pairs[i].deg_len, 0.0);
         return VISQOL_OK;
     }
     try {
-        c->engine->run(pairs, (int)n, sr, res.data(), false);
+        c->engine->run(pairs, (int)n, sr, res.data(), false, grads);
     } catch (const BackendError& e) {
         set_error("%s", e.msg.c_str());
         return e.code;
@@ -326,6 +330,29 @@ int visqol_measure_batch(visqol_ctx* c, size_t n, const double* const* refs, con
     for (size_t i = 0; i < n; i++) pairs[i] = PairInput{refs[i], (long long)ref_lens[i], degs[i], (long long)deg_lens[i], false};
     std::vector<PairResult> res;
     const int rc = run_pairs(c, pairs.data(), n, sample_rate, res);
+    if (rc != VISQOL_OK) {
+        for (size_t i = 0; i < n; i++) {
+            out[i] = nullptr;
+            if (status) status[i] = rc;
+        }
+        return rc;
+    }
+    return finish_batch(res, out, status);
+}
+
+int visqol_hip_measure_batch_grad(visqol_ctx* c, size_t n, const double* const* refs, const size_t* ref_lens,
+                                  const double* const* degs, const size_t* deg_lens, uint32_t sample_rate,
+                                  visqol_result** out, int32_t* status, double* const* grads) {
+    if (!c) return invalid("ctx is NULL");
+    if (n == 0) return VISQOL_OK;
+    if (!out || !refs || !ref_lens || !degs || !deg_lens || !grads) return invalid("NULL argument");
+    std::vector<PairInput> pairs(n);
+    for (size_t i = 0; i < n; i++) {
+        if ((!refs[i] && ref_lens[i]) || (!degs[i] && deg_lens[i])) return invalid("NULL signal with nonzero length");
+        pairs[i] = PairInput{refs[i], (long long)ref_lens[i], degs[i], (long long)deg_lens[i], false};
+    }
+    std::vector<PairResult> res;
+    const int rc = run_pairs(c, pairs.data(), n, sample_rate, res, grads);
     if (rc != VISQOL_OK) {
         for (size_t i = 0; i < n; i++) {
             out[i] = nullptr;
@@ -534,6 +561,12 @@ int visqol_hip_prepare(visqol_ctx* c, size_t n, const double* const* refs, const
 int visqol_hip_measure_prepared(visqol_ctx* c, const visqol_hip_prepared* prep, size_t n, const size_t* ref_index,
                                 const double* const* degs, const size_t* deg_lens, visqol_result** out,
                                 int32_t* status) {
+    return visqol_hip_measure_prepared_grad(c, prep, n, ref_index, degs, deg_lens, out, status, nullptr);
+}
+
+int visqol_hip_measure_prepared_grad(visqol_ctx* c, const visqol_hip_prepared* prep, size_t n,
+                                     const size_t* ref_index, const double* const* degs, const size_t* deg_lens,
+                                     visqol_result** out, int32_t* status, double* const* grads) {
     if (!c || !prep || !prep->set) return invalid("NULL argument");
     if (n == 0) return VISQOL_OK;
     if (!out || !degs || !deg_lens) return invalid("NULL argument");
@@ -548,7 +581,7 @@ int visqol_hip_measure_prepared(visqol_ctx* c, const visqol_hip_prepared* prep, 
         pairs[i] = PairInput{nullptr, 0, degs[i], (long long)deg_lens[i], false, &ps.refs[k]};
     }
     std::vector<PairResult> res;
-    const int rc = run_pairs(c, pairs.data(), n, ps.sample_rate, res);
+    const int rc = run_pairs(c, pairs.data(), n, ps.sample_rate, res, grads);
     if (rc != VISQOL_OK) {
         for (size_t i = 0; i < n; i++) {
             out[i] = nullptr;
@@ -593,6 +626,24 @@ int visqol_hip_profile_phases(visqol_ctx* c, double* ms, int n) {
     const Engine::Profile p = c->engine->profile();
     for (int i = 0; i < n && i < Engine::Profile::kPhases; i++) ms[i] = p.phase_ms[i];
     return Engine::Profile::kPhases;
+}
+
+int visqol_hip_patch_similarity_grad(const double* ref, const double* deg, size_t nrows, size_t ncols,
+                                     const double* g_means, const double* g_stddevs, const double* g_energy,
+                                     double* g_ref, double* g_deg) {
+    if (!ref || !deg || !g_ref || !g_deg || nrows == 0 || ncols == 0) return invalid("invalid argument");
+    memset(g_ref, 0, sizeof(double) * nrows * ncols);
+    memset(g_deg, 0, sizeof(double) * nrows * ncols);
+    patch_similarity_grad(ref, deg, (int)nrows, (int)ncols, g_means, g_stddevs, g_energy, g_ref, g_deg);
+    return VISQOL_OK;
+}
+
+int visqol_hip_prepare_grad(const double* raw_ref, const double* raw_deg, size_t nb, size_t nfr, size_t nfd,
+                            const double* g_ref_prep, const double* g_deg_prep, double* g_raw_ref, double* g_raw_deg) {
+    if (!raw_ref || !raw_deg || !g_ref_prep || !g_deg_prep || !g_raw_ref || !g_raw_deg || nb == 0)
+        return invalid("invalid argument");
+    prepare_grad(raw_ref, raw_deg, (int)nb, (int)nfr, (int)nfd, g_ref_prep, g_deg_prep, g_raw_ref, g_raw_deg);
+    return VISQOL_OK;
 }
 
 int visqol_hip_gammatone_shape(int shape, int* waves, int* bands_per_lane, int* chunk) {
@@ -659,6 +710,24 @@ int visqol_hip_spectrogram(uint32_t mode, uint32_t sample_rate, uint32_t precisi
             ->spectrogram(sample_rate, signal, (long long)len, spec, nf);
         memcpy(out, spec.data(), sizeof(double) * spec.size());
         if (num_frames_out) *num_frames_out = (size_t)nf;
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return VISQOL_OK;
+}
+
+int visqol_hip_frame_grad(uint32_t mode, uint32_t sample_rate, const double* frame, size_t window, const double* g,
+                          size_t lo, size_t hi, double* out) {
+    if (!frame || !g || !out || mode > 1 || sample_rate == 0) return invalid("invalid argument");
+    const RateParams p = make_rate_params((int)mode, sample_rate);
+    if (window != (size_t)p.window) return invalid("window is not the sample rate's frame length");
+    if (lo > hi || hi > window) return invalid("lo, hi must satisfy lo <= hi <= window");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    try {
+        stage_engine(mode, false)->frame_grad(sample_rate, frame, g, (int)lo, (int)hi, out);
     } catch (const BackendError& e) {
         set_error("%s", e.msg.c_str());
         return e.code;

@@ -14,6 +14,7 @@
 //   patch unless it is strictly worse (realign.zig:294-460).
 // Host: aggregation and MOS mapping (visqol.zig:595-709).
 #include "engine.h"
+#include "engine_state.h"
 
 #include <math.h>
 #include <string.h>
@@ -171,6 +172,12 @@ T* Engine::upload(const std::vector<T>& v) {
     return upload(v.data(), v.size());
 }
 
+// the gradient pass (engine_grad.cpp) uploads these
+template double* Engine::upload(const std::vector<double>&);
+template int* Engine::upload(const std::vector<int>&);
+template FrameGradJob* Engine::upload(const std::vector<FrameGradJob>&);
+template AccRun* Engine::upload(const std::vector<AccRun>&);
+
 void Engine::readback(void* host_dst, const void* dev_src, size_t bytes) {
     if (bytes == 0) return;
     check(hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, stream_), "readback");
@@ -255,25 +262,6 @@ long long* Engine::lag_search(const std::vector<LagJob>& jobs, const double* d_s
 
 namespace {
 
-struct PairState {
-    bool live = true;
-    const PreparedRef* prep = nullptr;
-    const double* dref = nullptr;
-    const double* ddeg = nullptr;
-    long long rl = 0, dl = 0;
-    int lagjob = -1, vadjob = -1;
-    long long lag = 0, shift = 0, dl2 = 0;
-    double lag_s = 0, factor = 1.0;
-    int nf_ref = 0, nf_deg = 0, np = 0, width = 0;
-    std::vector<int> idx, lo, hi;
-    VadPlan vp;
-    size_t vad_off = 0;
-    double* degs = nullptr;
-    double *spec_ref = nullptr, *spec_deg = nullptr, *raw_ref = nullptr, *raw_deg = nullptr;
-    double *plane = nullptr, *sims = nullptr, *dp = nullptr;
-    int *bt = nullptr, *d_idx = nullptr, *d_lo = nullptr, *d_hi = nullptr, *d_off = nullptr, *d_state = nullptr;
-};
-
 void fail(PairResult& r, PairState& s, int code, const char* msg) {
     r.status = code;
     r.error = msg;
@@ -282,7 +270,8 @@ void fail(PairResult& r, PairState& s, int code, const char* msg) {
 
 }  // namespace
 
-void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages) {
+void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages,
+                 double* const* grads) {
     // Chunks bounded by samples so a batch of any size fits in HBM.
     const long long kChunkSamples = 1LL << 28;
     std::vector<std::pair<int, int>> chunks;
@@ -297,8 +286,10 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
         chunks.emplace_back(start, end);
         start = end;
     }
-    if (chunks.size() < 2 || !opts_.interleave_chunks || want_stages) {
-        for (const auto& c : chunks) run_chunk(pairs + c.first, c.second - c.first, sr, out + c.first, want_stages);
+    if (chunks.size() < 2 || !opts_.interleave_chunks || want_stages || grads) {
+        for (const auto& c : chunks)
+            run_chunk(pairs + c.first, c.second - c.first, sr, out + c.first, want_stages,
+                      grads ? grads + c.first : nullptr);
         return;
     }
     // Odd chunks on the helper's thread, even ones here; both enqueue to the
@@ -311,7 +302,7 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
         try {
             for (size_t k = 1; k < chunks.size(); k += 2)
                 helper_->run_chunk(pairs + chunks[k].first, chunks[k].second - chunks[k].first, sr, out + chunks[k].first,
-                                   false);
+                                   false, nullptr);
         } catch (...) {
             helper_error = std::current_exception();
         }
@@ -319,7 +310,8 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
     std::exception_ptr own_error;
     try {
         for (size_t k = 0; k < chunks.size(); k += 2)
-            run_chunk(pairs + chunks[k].first, chunks[k].second - chunks[k].first, sr, out + chunks[k].first, false);
+            run_chunk(pairs + chunks[k].first, chunks[k].second - chunks[k].first, sr, out + chunks[k].first, false,
+                      nullptr);
     } catch (...) {
         own_error = std::current_exception();
     }
@@ -332,7 +324,8 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
     if (helper_error) std::rethrow_exception(helper_error);
 }
 
-void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages) {
+void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages,
+                       double* const* grads) {
     arena_.reset();
     pinned_.reset();
     DevRate& dr = dev_rate(sr);
@@ -483,7 +476,9 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             }
         }
         // audio.zig:132-159 (scaleToMatchSpl)
-        s.factor = spl_factor(spl_of_energy(h_sums[6 * i + 4], s.rl), spl_of_energy(h_sums[6 * i + 5], s.dl2));
+        s.e_ref = h_sums[6 * i + 4];
+        s.e_deg = h_sums[6 * i + 5];
+        s.factor = spl_factor(spl_of_energy(s.e_ref, s.rl), spl_of_energy(s.e_deg, s.dl2));
         r.spl_factor = s.factor;
         r.lag_s = s.lag_s;
         if (s.rl <= W || s.dl2 <= W) { fail(r, s, -5, "SpectrogramTooShort"); continue; }
@@ -540,7 +535,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     }
     double* d_spec = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, spec_total));
     double* d_raw = nullptr;
-    if (!opts_.no_realign || want_stages) d_raw = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, spec_total));
+    if (!opts_.no_realign || want_stages || grads)
+        d_raw = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, spec_total));
     double* d_degs = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, degs_total));
     double* d_planes = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, plane_total));
     double* d_sims = (double*)arena_.alloc(sizeof(double) * std::max<size_t>(1, cells_total));
@@ -692,6 +688,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             if (!st[i].live) continue;
             h_off[i].assign(p_offs + np_off[i], p_offs + np_off[i] + st[i].np);
             h_state[i].assign(p_states + np_off[i], p_states + np_off[i] + st[i].np);
+            st[i].match_state = h_state[i];
             if (want_stages) {
                 out[i].ref_spec.assign(pr[i], pr[i] + (size_t)nb * st[i].nf_ref);
                 out[i].deg_spec.assign(pd[i], pd[i] + (size_t)nb * st[i].nf_deg);
@@ -846,7 +843,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             std::vector<StatJob> rstj;
             int rmax_nf = 0;
             long long max_t = 0;
-            std::vector<FrameCopyJob> copies;
+            std::vector<FrameCopyJob> copies, raw_copies;
             for (size_t k = 0; k < rjs.size(); k++) {
                 RJ& rj = rjs[k];
                 PairState& s = st[rj.pair];
@@ -868,6 +865,21 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                     return g0 + rj.nf <= nf_global;
                 };
                 long long g0 = 0;
+                RowTrace tr;
+                if (grads) {
+                    tr.match = rj.match;
+                    tr.nf = rj.nf;
+                    tr.spec_r = rj.spec_r;
+                    tr.spec_d = rj.spec_d;
+                    double* raw = (double*)arena_.alloc(sizeof(double) * 2 * nb * rj.nf);
+                    tr.raw_r = raw;
+                    tr.raw_d = raw + (size_t)nb * rj.nf;
+                    raw_copies.push_back(FrameCopyJob{raw, rj.spec_r, rj.nf, rj.nf});
+                    raw_copies.push_back(FrameCopyJob{raw + (size_t)nb * rj.nf, rj.spec_d, rj.nf, rj.nf});
+                    tr.first = rj.dl.base - rj.dl.pad + rj.geo.deg_off;
+                    tr.valid_lo = rj.dl.base;
+                    tr.valid_hi = rj.dl.base + rj.dl.len;
+                }
                 if (reuse(rj.rl, rj.geo.ref_off, s.nf_ref, g0)) {
                     copies.push_back(FrameCopyJob{rj.spec_r, s.raw_ref + g0, s.nf_ref, rj.nf});
                 } else {
@@ -875,6 +887,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                 }
                 if (reuse(rj.dl, rj.geo.deg_off, s.nf_deg, g0)) {
                     copies.push_back(FrameCopyJob{rj.spec_d, s.raw_deg + g0, s.nf_deg, rj.nf});
+                    tr.deg_reused = true;
+                    tr.g0 = g0;
                 } else {
                     const long long off = rj.geo.deg_off;
                     if (off >= 0 && off + rj.geo.len <= rj.dlen) {
@@ -890,10 +904,13 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                     }
                 }
                 rpj.push_back(PrepJob{rj.spec_r, rj.spec_d, rj.nf, rj.nf});
+                if (grads) s.rows.push_back(tr);
             }
             launch_slices(upload(tslj), (int)tslj.size(), max_t, stream_);
             gammatone(upload(rg), (int)rg.size(), rmax_nf, dr, 1);
             launch_frame_copies(upload(copies), (int)copies.size(), nb, rmax_nf, stream_);
+            // the rows' raw spectrograms, kept for the gradient before the dB preparation
+            launch_frame_copies(upload(raw_copies), (int)raw_copies.size(), nb, rmax_nf, stream_);
             auto* rscratch = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * std::max<size_t>(1, rpj.size()));
             launch_prep(upload(rpj), (int)rpj.size(), nb, rmax_nf, rscratch, stream_);
             double* d_rstats = (double*)arena_.alloc(sizeof(double) * stat_len * std::max<size_t>(1, rjs.size()));
@@ -956,6 +973,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             if (st[i].live) finish(out[i], rp);
     }
     lap(7);
+    if (grads) backward(pairs, n, sr, out, st, grads);
 }
 
 // visqol.zig:595-709 (buildResult) and visqol.zig:81-98 (mapping).

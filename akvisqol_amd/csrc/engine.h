@@ -117,6 +117,8 @@ class PinnedArena {
     std::vector<Chunk> chunks_;
 };
 
+struct PairState;  // engine_state.h
+
 struct BackendError {
     int code;
     std::string msg;
@@ -128,8 +130,13 @@ class Engine {
            std::unique_ptr<LatticeModel, LatticeDeleter> lattice);
     ~Engine();
     // Scores n pairs; per-pair outcome in out[i].status.  Throws
-    // BackendError for device failures that poison the whole batch.
-    void run(const PairInput* pairs, int n, uint32_t sample_rate, PairResult* out, bool want_stages);
+    // BackendError for device failures that poison the whole batch.  With
+    // grads, grads[i] (host, deg_len values, or NULL to skip the pair)
+    // receives ∂moslqo/∂deg of each pair scored (backend.zig:101-110), the
+    // discrete choices of the forward pass held fixed; zero for a pair that
+    // fails.
+    void run(const PairInput* pairs, int n, uint32_t sample_rate, PairResult* out, bool want_stages,
+             double* const* grads = nullptr);
     const EngineOptions& options() const { return opts_; }
 
     // References of one sample rate prepared for run() (PairInput::prep);
@@ -143,6 +150,10 @@ class Engine {
     // (the values best_lag maximises), fft_size_for(max(la, lb)) of them.
     void xcorr(const double* a, long long la, const double* b, long long lb, std::vector<double>& out);
     void math_probe(int op, const double* a, const double* b, double* out, long long n);
+    // The spectrogram adjoint on one frame (window samples, nb band slopes
+    // g): out = ∂(Σ_b g_b · rms_b)/∂frame, samples outside [lo, hi) read as
+    // zero and given zero slope.
+    void frame_grad(uint32_t sample_rate, const double* frame, const double* g, int lo, int hi, double* out);
     void fft_probe(double* z, long long n, bool dit);
     const RateParams& rate(uint32_t sample_rate);
 
@@ -219,7 +230,15 @@ class Engine {
     void sync();
     void check(hipError_t e, const char* what);
     // One chunk of pairs (bounded HBM footprint).
-    void run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages);
+    void run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bool want_stages,
+                   double* const* grads);
+    // The gradient pass of a chunk whose forward pass left its state in st.
+    void backward(const PairInput* pairs, int n, uint32_t sr, PairResult* out, std::vector<PairState>& st,
+                  double* const* grads);
+    // ∂moslqo/∂(fvnsim, fvnsim10, fstdnsim, fvdegenergy) of this engine's
+    // quality mapping at the given features (visqol.zig:100-149).
+    void mapper_grad(int nb, const double* f0, const double* f1, const double* f2, const double* f3, double* g0,
+                     double* g1, double* g2, double* g3) const;
     // Reductions of `jobs` into d_out (serial: the reference's order).
     void sums(const std::vector<SumJob>& jobs, double* d_out, bool serial);
     // FFT cross-correlation lags of `jobs` (envelope sums already in

@@ -80,6 +80,8 @@ class SvrModel {
     // svr.zig:80-165; returns an error message on failure.
     static std::unique_ptr<SvrModel> parse(const std::string& text, std::string& err);
     double predict(const double* x) const;  // svr.zig:37-61 (f64 kernel)
+    // ∂predict/∂x (visqol.zig:106-116), num_features() values
+    void gradient(const double* x, double* g) const;
     int num_features() const { return nf_; }
 
    private:

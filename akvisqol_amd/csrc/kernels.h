@@ -202,6 +202,38 @@ struct StatJob {
 };
 void launch_stats(const StatJob* d_jobs, int njobs, int nb, int max_cols, hipStream_t stream);
 
+// ---------------------------------------------------------------- gradients
+// One frame's slope (gammatone_bwd.slang:84-150): frame sample t is
+// sig[start + t] for t in [lo, hi) and zero elsewhere (padding the forward
+// added); band b's spectrogram slope is g_col[b * stride]; out receives the
+// window's W sample slopes (zero outside [lo, hi)).  The host keeps
+// start + lo >= 0 and start + hi within the signal.
+struct FrameGradJob {
+    const double* sig;
+    long long start;
+    const double* g_col;
+    double* out;
+    int stride, lo, hi, pad_;
+};
+// Samples per checkpoint of the adjoint's forward pass.
+constexpr int kAdjSpan = 64;
+// Doubles of checkpoint scratch a launch of njobs frames needs.
+size_t gammatone_adjoint_scratch(int njobs, int window);
+void launch_gammatone_adjoint(const FrameGradJob* d_jobs, int njobs, const BandCoef* d_coef, const double* d_hann,
+                              int nb, int window, double* d_ckpt, hipStream_t stream);
+
+// dst[i] = Σ over frames list[first .. first+count) (in that order) of
+// frame.out[pos + i - frame.start] where that index is in the frame's
+// [lo, hi), for i < len (gammatone_bwd.slang:153-183's overlap sum, in a
+// fixed order).
+struct AccRun {
+    double* dst;
+    long long pos;
+    int first, count, len, pad_;
+};
+void launch_frame_accumulate(const AccRun* d_runs, int nruns, const int* d_list, const FrameGradJob* d_frames,
+                             hipStream_t stream);
+
 // Device math checks (log10, sqrt, division) against the host.
 void launch_math_probe(int op, const double* a, const double* b, double* out, long long n, hipStream_t stream);
 

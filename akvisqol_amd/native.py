@@ -133,6 +133,9 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_profile_read": (c_int, [c_void_p, D, D, D, POINTER(c_int64)]),
         "visqol_hip_profile_phases": (c_int, [c_void_p, D, c_int]),
         "visqol_hip_profile_sim": (c_int, [c_void_p, D, D, POINTER(c_int64)]),
+        "visqol_hip_frame_grad": (c_int, [c_uint32, c_uint32, D, c_size_t, D, c_size_t, c_size_t, D]),
+        "visqol_hip_patch_similarity_grad": (c_int, [D, D, c_size_t, c_size_t, D, D, D, D, D]),
+        "visqol_hip_prepare_grad": (c_int, [D, D, c_size_t, c_size_t, c_size_t, D, D, D, D]),
         "visqol_hip_gammatone_shape": (c_int, [c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
         "visqol_hip_gammatone_tuning": (c_int, [c_void_p, c_uint32, c_int, POINTER(c_int), POINTER(ctypes.c_float),
                                                 c_int]),
@@ -150,6 +153,10 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_measure_prepared": (c_int, [c_void_p, c_void_p, c_size_t, S, POINTER(D), S,
                                                 POINTER(POINTER(Result)), POINTER(c_int32)]),
         "visqol_hip_prepared_free": (None, [c_void_p]),
+        "visqol_hip_measure_batch_grad": (c_int, [c_void_p, c_size_t, POINTER(D), S, POINTER(D), S, c_uint32,
+                                                  POINTER(POINTER(Result)), POINTER(c_int32), POINTER(D)]),
+        "visqol_hip_measure_prepared_grad": (c_int, [c_void_p, c_void_p, c_size_t, S, POINTER(D), S,
+                                                     POINTER(POINTER(Result)), POINTER(c_int32), POINTER(D)]),
         "visqol_hip_server_config_default": (None, [POINTER(ServerConfig)]),
         "visqol_hip_server_create": (c_int, [POINTER(Options), POINTER(ServerConfig), POINTER(c_void_p)]),
         "visqol_hip_server_submit": (c_int, [c_void_p, c_uint32, D, c_size_t, D, c_size_t, c_uint32,
@@ -394,6 +401,53 @@ class Context:
                 res.append(None)
         return rc, res, list(st)
 
+    def measure_batch_grad(self, refs, degs, sample_rate: int):
+        """(first error code, [result dict or None], [status], [∂moslqo/∂deg])
+        as the reference's measureBatchGrad (backend.zig:101-110): the
+        forward pass's discrete choices held fixed; zeros for a failed pair."""
+        refs = [_f64(r) for r in refs]
+        degs = [_f64(d) for d in degs]
+        n = len(refs)
+        D = POINTER(c_double)
+        grads = [np.zeros(d.size) for d in degs]
+        rp = (D * n)(*[_dptr(r) for r in refs])
+        dp = (D * n)(*[_dptr(d) for d in degs])
+        gp = (D * n)(*[_dptr(g) for g in grads])
+        rl = (c_size_t * n)(*[r.size for r in refs])
+        dl = (c_size_t * n)(*[d.size for d in degs])
+        out = (POINTER(Result) * n)()
+        st = (c_int32 * n)()
+        with self._lock:
+            rc = lib.visqol_hip_measure_batch_grad(self._h, n, rp, rl, dp, dl, sample_rate, out, st, gp)
+        return rc, self._take(out, n), list(st), grads
+
+    def measure_prepared_grad(self, prep: "Prepared", degs, ref_index=None):
+        """measure_prepared with ∂moslqo/∂deg of every pair."""
+        degs = [_f64(d) for d in degs]
+        n = len(degs)
+        D = POINTER(c_double)
+        grads = [np.zeros(d.size) for d in degs]
+        dp = (D * n)(*[_dptr(d) for d in degs])
+        gp = (D * n)(*[_dptr(g) for g in grads])
+        dl = (c_size_t * n)(*[d.size for d in degs])
+        ri = None if ref_index is None else (c_size_t * n)(*[int(k) for k in ref_index])
+        out = (POINTER(Result) * n)()
+        st = (c_int32 * n)()
+        with self._lock:
+            rc = lib.visqol_hip_measure_prepared_grad(self._h, prep._h, n, ri, dp, dl, out, st, gp)
+        return rc, self._take(out, n), list(st), grads
+
+    @staticmethod
+    def _take(out, n):
+        res = []
+        for i in range(n):
+            if out[i]:
+                res.append(result_dict(out[i].contents))
+                lib.visqol_result_free(out[i])
+            else:
+                res.append(None)
+        return res
+
     PHASES = ("phase_a", "plan", "phase_b", "match_stats", "match_decode", "realign_lags", "realign_stats", "aggregate")
 
     def gammatone_tuning(self, sample_rate: int, kind: int = 0) -> dict:
@@ -594,6 +648,42 @@ def _check_count(rc: int) -> int:
     if rc < 0:
         _check(rc)
     return rc
+
+
+def frame_grad(mode: int, sample_rate: int, frame, g, lo: int = 0, hi: int | None = None) -> np.ndarray:
+    """∂(Σ_b g_b · rms_b)/∂frame on the device for one spectrogram frame."""
+    frame = _f64(frame)
+    g = _f64(g)
+    hi = frame.size if hi is None else hi
+    out = np.zeros(frame.size)
+    _check(lib.visqol_hip_frame_grad(mode, sample_rate, _dptr(frame), frame.size, _dptr(g), lo, hi, _dptr(out)))
+    return out
+
+
+def patch_similarity_grad(ref, deg, g_means=None, g_stddevs=None, g_energy=None):
+    """(∂/∂ref cells, ∂/∂deg cells) of Σ g·(band means, stddevs, deg energy)
+    of measurePatchSimilarity on the [nrows, ncols] patches."""
+    ref = np.ascontiguousarray(ref, dtype=np.float64)
+    deg = np.ascontiguousarray(deg, dtype=np.float64)
+    nrows, ncols = ref.shape
+    opt = [None if g is None else _f64(g) for g in (g_means, g_stddevs, g_energy)]
+    gr, gd = np.zeros_like(ref), np.zeros_like(deg)
+    _check(lib.visqol_hip_patch_similarity_grad(_dptr(ref), _dptr(deg), nrows, ncols,
+                                                *[None if g is None else _dptr(g) for g in opt], _dptr(gr), _dptr(gd)))
+    return gr, gd
+
+
+def prepare_grad(raw_ref, raw_deg, g_ref_prep, g_deg_prep):
+    """prepareForComparison's slopes taken back to the raw [nb, nf] spectrograms."""
+    raw_ref = np.ascontiguousarray(raw_ref, dtype=np.float64)
+    raw_deg = np.ascontiguousarray(raw_deg, dtype=np.float64)
+    nb, nfr = raw_ref.shape
+    nfd = raw_deg.shape[1]
+    g1, g2 = _f64(np.ascontiguousarray(g_ref_prep)), _f64(np.ascontiguousarray(g_deg_prep))
+    gr, gd = np.zeros_like(raw_ref), np.zeros_like(raw_deg)
+    _check(lib.visqol_hip_prepare_grad(_dptr(raw_ref), _dptr(raw_deg), nb, nfr, nfd, _dptr(g1), _dptr(g2), _dptr(gr),
+                                       _dptr(gd)))
+    return gr, gd
 
 
 def gammatone_shapes() -> list[dict]:

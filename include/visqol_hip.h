@@ -84,6 +84,20 @@ int visqol_hip_measure_prepared(visqol_ctx *ctx, const visqol_hip_prepared *prep
                                 visqol_result **out, int32_t *status);
 void visqol_hip_prepared_free(visqol_hip_prepared *prep);
 
+/* Gradients (backend.zig:101-110 measureBatchGrad, :93-99 measurePrepared
+ * with grads): visqol_measure_batch's scores, and into grads[i] (deg_lens[i]
+ * values, caller-allocated host memory; NULL skips the pair) ∂moslqo/∂deg[i]
+ * with every discrete choice of the forward pass held fixed (the global and
+ * realignment lags, the matched patches, which realignments are accepted,
+ * the cells the dB floors pin).  A pair that fails, or whose score is
+ * clamped, gets zeros.  Both precisions compute in f64. */
+int visqol_hip_measure_batch_grad(visqol_ctx *ctx, size_t n, const double *const *refs, const size_t *ref_lens,
+                                  const double *const *degs, const size_t *deg_lens, uint32_t sample_rate,
+                                  visqol_result **out, int32_t *status, double *const *grads);
+int visqol_hip_measure_prepared_grad(visqol_ctx *ctx, const visqol_hip_prepared *prep, size_t n,
+                                     const size_t *ref_index, const double *const *degs, const size_t *deg_lens,
+                                     visqol_result **out, int32_t *status, double *const *grads);
+
 /* Continuous-batching server (no counterpart in the reference's C ABI; a
  * serving front end for callers with a stream of requests, in the manner of
  * the reference's batch lanes, backend.zig:182-221).
@@ -119,6 +133,18 @@ int visqol_hip_server_stats(visqol_hip_server *s, uint64_t *batches, uint64_t *p
 void visqol_hip_server_destroy(visqol_hip_server *s);
 
 /* Device-resident buffers for the batch entry point above. */
+/* Host-side pieces of the gradient, for checks: measurePatchSimilarity's
+ * per-band (mean, stddev, deg energy) slopes (any of the three may be NULL)
+ * taken back to the comparison-ready ref and deg patch cells (row-major
+ * [nrows][ncols]), and prepareForComparison's prepared-spectrogram slopes
+ * taken back to the raw spectrograms ([nb][nfr] and [nb][nfd]); the outputs
+ * are overwritten. */
+int visqol_hip_patch_similarity_grad(const double *ref, const double *deg, size_t nrows, size_t ncols,
+                                     const double *g_means, const double *g_stddevs, const double *g_energy,
+                                     double *g_ref, double *g_deg);
+int visqol_hip_prepare_grad(const double *raw_ref, const double *raw_deg, size_t nb, size_t nfr, size_t nfd,
+                            const double *g_ref_prep, const double *g_deg_prep, double *g_raw_ref, double *g_raw_deg);
+
 /* Spectrogram kernel block shapes.  visqol_hip_gammatone_shape describes
  * shape `shape` (waves per block, bands per lane, samples per staged chunk;
  * 64 frames per block) and returns the number of shapes.  Every shape gives
@@ -152,6 +178,13 @@ int visqol_hip_upload(void *dev_dst, const void *host_src, size_t bytes);
  * as visqol.h; returns the frame count through *num_frames. */
 int visqol_hip_spectrogram(uint32_t mode, uint32_t sample_rate, uint32_t precision, const double *signal, size_t len,
                            double *out, size_t *num_frames);
+/* The spectrogram's adjoint on one frame (gammatone_bwd.slang:84-150):
+ * out[t] = ∂(Σ_b g[b] · rms_b)/∂frame[t] for the `window` samples of one
+ * frame (window must be the sample rate's frame length) with the
+ * filterbank's band slopes g; samples outside [lo, hi) read as zero and get
+ * zero slope. */
+int visqol_hip_frame_grad(uint32_t mode, uint32_t sample_rate, const double *frame, size_t window, const double *g,
+                          size_t lo, size_t hi, double *out);
 /* Filter-bank design shared with the device: 8 doubles per band
  * (a0 a11 a12 a13 a14 b1 b2 gain), bands ascending, plus centre
  * frequencies. */
