@@ -127,7 +127,7 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
         std::vector<RowHost> rows;  // parallel to PairState::rows
         std::vector<double> g_raw_deg;
         std::vector<std::vector<double>> g_row_deg;  // raw-level slopes of rows computed on their own slices
-        std::vector<double> g_al;
+        const double* g_al = nullptr;                // the aligned signal's slopes (pinned readback)
     };
     std::vector<Host> hs(todo.size());
     std::vector<std::pair<const double*, std::vector<double>*>> staged;
@@ -280,7 +280,6 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
                 gcol_off.push_back(rbase + f);
             }
         }
-        h.g_al.assign(s.dl2, 0.0);
     }
     first_job[todo.size()] = jobs.size();
     if (!jobs.empty()) {
@@ -325,17 +324,28 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
         double* p = (double*)pinned_.alloc(sizeof(double) * std::max<size_t>(1, gal_total));
         readback(p, d_gal, sizeof(double) * gal_total);
         sync();
-        for (size_t q = 0; q < todo.size(); q++)
-            memcpy(hs[q].g_al.data(), p + gal_off[q], sizeof(double) * hs[q].g_al.size());
+        for (size_t q = 0; q < todo.size(); q++) hs[q].g_al = p + gal_off[q];
     }
 
     // ---------------------------------------------------------------- host: aligned → input
-    for (size_t q = 0; q < todo.size(); q++) {
+    // (a pair whose frames all have zero slope keeps the zeros written above)
+    auto input_pass = [&](size_t q) {
+        if (!hs[q].g_al) return;
         const int i = todo[q];
         const PairState& s = st[i];
         const double* deg = pairs[i].on_device ? hs[q].deg.data() : pairs[i].deg;
         const bool scaled = isfinite(spl_of_energy(s.e_ref, s.rl)) && isfinite(spl_of_energy(s.e_deg, s.dl2));
-        aligned_to_input_grad(deg, s.dl, s.shift, s.dl2, s.factor, s.e_deg, scaled, hs[q].g_al.data(), grads[i]);
+        aligned_to_input_grad(deg, s.dl, s.shift, s.dl2, s.factor, s.e_deg, scaled, hs[q].g_al, grads[i]);
+    };
+    if (nthreads > 1) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthreads; t++)
+            pool.emplace_back([&, t] {
+                for (size_t q = t; q < todo.size(); q += nthreads) input_pass(q);
+            });
+        for (auto& th : pool) th.join();
+    } else {
+        for (size_t q = 0; q < todo.size(); q++) input_pass(q);
     }
 }
 
