@@ -11,7 +11,7 @@ namespace vq {
 
 // ---------------------------------------------------------------- reductions
 __device__ inline double sum_term(const SumJob& j, long long i, double mean) {
-    const double x = j.x[i];
+    const double x = as_global(j.x)[i];
     switch (j.kind) {
         case kSum: return x;
         case kNyquist: {
@@ -102,17 +102,55 @@ __global__ __launch_bounds__(256) void k_sums(const SumJob* __restrict__ jobs, d
     const long long lo = start + per * blockIdx.y;
     const long long hi = lo + per < j.len ? lo + per : j.len;
     __shared__ double part[kSumOutputs][256];
+    // Each thread adds its terms lo + t, lo + t + 256, ... in order; eight
+    // loads are issued before they are added, which keeps more loads in
+    // flight without changing any sum's order.
+    constexpr int kUnroll = 8;
+    const long long stride = blockDim.x;
     double s = 0.0, alt = 0.0, sq = 0.0;
+    long long i = lo + threadIdx.x;
     if (j.kind == kMoments) {
         const auto* x = as_global(j.x);
-        for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        for (; i + (kUnroll - 1) * stride < hi; i += kUnroll * stride) {
+            double v[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) v[u] = x[i + u * stride];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                s += v[u];
+                alt += ((i + u * stride) & 1) ? -v[u] : v[u];
+                sq = __builtin_fma(v[u], v[u], sq);
+            }
+        }
+        for (; i < hi; i += stride) {
             const double v = x[i];
             s += v;
             alt += (i & 1) ? -v : v;
             sq = __builtin_fma(v, v, sq);
         }
     } else {
-        for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += sum_term(j, i, mean);
+        // one loop per kind, each term as sum_term forms it
+        const auto* x = as_global(j.x);
+        auto run = [&](auto term) {
+            for (; i + (kUnroll - 1) * stride < hi; i += kUnroll * stride) {
+                double v[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) v[u] = x[i + u * stride];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) s += term(v[u], i + u * stride);
+            }
+            for (; i < hi; i += stride) s += term(x[i], i);
+        };
+        if (j.kind == kSum) {
+            run([](double v, long long) { return v; });
+        } else if (j.kind == kNyquist) {
+            run([mean](double v, long long k) {
+                const double d = v - mean;
+                return (k & 1) ? -d : d;
+            });
+        } else {
+            run([](double v, long long) { return v * v; });
+        }
     }
     part[0][threadIdx.x] = s;
     part[1][threadIdx.x] = alt;
@@ -161,30 +199,55 @@ void launch_sums(const SumJob* d_jobs, int njobs, long long max_len, double* d_o
 }
 
 // ---------------------------------------------------------------- signal prep
+// Elementwise copies: each block covers kCopyPer·256 consecutive elements
+// (each thread kCopyPer of them, 256 apart, loads first).
+constexpr int kCopyPer = 4;
+constexpr int kCopyBlock = 256 * kCopyPer;
+
 __global__ __launch_bounds__(256) void k_scale(const ScaleJob* __restrict__ jobs) {
     const ScaleJob j = jobs[blockIdx.y];
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= j.dst_len) return;
-    const long long s = i - j.shift;
-    const double x = (s >= 0 && s < j.src_len) ? j.src[s] : 0.0;
-    j.dst[i] = x * j.factor;
+    const long long i0 = (long long)blockIdx.x * kCopyBlock + threadIdx.x;
+    if (i0 >= j.dst_len) return;
+    const auto* src = as_global(j.src);
+    double x[kCopyPer];
+#pragma unroll
+    for (int u = 0; u < kCopyPer; u++) {
+        const long long s = i0 + u * 256 - j.shift;
+        x[u] = (s >= 0 && s < j.src_len) ? src[s] : 0.0;
+    }
+    auto* dst = as_global(j.dst);
+#pragma unroll
+    for (int u = 0; u < kCopyPer; u++)
+        if (i0 + u * 256 < j.dst_len) dst[i0 + u * 256] = x[u] * j.factor;
 }
 
 void launch_scale(const ScaleJob* d_jobs, int njobs, long long max_len, hipStream_t stream) {
     if (njobs == 0 || max_len == 0) return;
-    hipLaunchKernelGGL(k_scale, dim3((unsigned)((max_len + 255) / 256), njobs), dim3(256), 0, stream, d_jobs);
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((max_len + kCopyBlock - 1) / kCopyBlock), njobs), dim3(256), 0, stream,
+                       d_jobs);
 }
 
 __global__ __launch_bounds__(256) void k_slices(const SliceJob* __restrict__ jobs) {
     const SliceJob j = jobs[blockIdx.y];
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= j.total) return;
-    j.dst[i] = (i >= j.pad && i < j.pad + j.len) ? j.src[j.base + i - j.pad] : 0.0;
+    const long long i0 = (long long)blockIdx.x * kCopyBlock + threadIdx.x;
+    if (i0 >= j.total) return;
+    const auto* src = as_global(j.src);
+    double x[kCopyPer];
+#pragma unroll
+    for (int u = 0; u < kCopyPer; u++) {
+        const long long i = i0 + u * 256;
+        x[u] = (i >= j.pad && i < j.pad + j.len) ? src[j.base + i - j.pad] : 0.0;
+    }
+    auto* dst = as_global(j.dst);
+#pragma unroll
+    for (int u = 0; u < kCopyPer; u++)
+        if (i0 + u * 256 < j.total) dst[i0 + u * 256] = x[u];
 }
 
 void launch_slices(const SliceJob* d_jobs, int njobs, long long max_len, hipStream_t stream) {
     if (njobs == 0 || max_len == 0) return;
-    hipLaunchKernelGGL(k_slices, dim3((unsigned)((max_len + 255) / 256), njobs), dim3(256), 0, stream, d_jobs);
+    hipLaunchKernelGGL(k_slices, dim3((unsigned)((max_len + kCopyBlock - 1) / kCopyBlock), njobs), dim3(256), 0, stream,
+                       d_jobs);
 }
 
 __global__ __launch_bounds__(256) void k_frame_copies(const FrameCopyJob* __restrict__ jobs, int nb) {
@@ -192,7 +255,7 @@ __global__ __launch_bounds__(256) void k_frame_copies(const FrameCopyJob* __rest
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nb * j.nf) return;
     const int b = e / j.nf, f = e % j.nf;
-    j.dst[e] = j.src[(long long)b * j.src_stride + f];
+    as_global(j.dst)[e] = as_global(j.src)[(long long)b * j.src_stride + f];
 }
 
 void launch_frame_copies(const FrameCopyJob* d_jobs, int njobs, int nb, int max_nf, hipStream_t stream) {
