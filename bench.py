@@ -119,6 +119,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--uneven", type=int, default=512, help="pairs of the 1-60 s mix scored as batches (0: skip)")
     ap.add_argument("--pool", type=int, default=64, help="distinct pairs of the 1-120 s mix drawn by requests")
+    ap.add_argument("--grad-pairs", type=int, default=64,
+                    help="pairs of the batch scored with dMOS/ddeg (visqol_hip_measure_batch_grad); 0: skip")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer three times (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -179,6 +181,27 @@ def main() -> int:
                                                for sh, m in zip(t["shapes"], t["ms"])}}
     print(f"bench: autotune {autotune}", file=sys.stderr, flush=True)
     value = args.batch / step_s
+
+    # Gradients (visqol_hip_measure_batch_grad) of the first pairs, host
+    # arrays in and out: the batch scored alone, then with dMOS/ddeg, each
+    # timed after a warm call of the same size.
+    grad = None
+    ng = min(args.grad_pairs, args.batch)
+    if ng > 0:
+        gr = [p[0] for p in pairs[:ng]]
+        gd = [p[1] for p in pairs[:ng]]
+        ctx.measure_batch(gr, gd, 48000, convert=False)
+        ctx.measure_batch_grad(gr, gd, 48000)
+        t = time.perf_counter()
+        ctx.measure_batch(gr, gd, 48000, convert=False)
+        t_score = time.perf_counter() - t
+        t = time.perf_counter()
+        rc_g, _, _, g = ctx.measure_batch_grad(gr, gd, 48000)
+        t_grad = time.perf_counter() - t
+        grad = {"pairs": ng, "score_ms": round(t_score * 1e3, 2), "score_and_grad_ms": round(t_grad * 1e3, 2),
+                "grad_ms_per_pair": round((t_grad - t_score) * 1e3 / ng, 3), "rc": rc_g,
+                "grad_l2_median": round(float(np.median([np.linalg.norm(x) for x in g])), 4)}
+        print(f"bench: gradients {grad}", file=sys.stderr, flush=True)
 
     # The same batch handed over as host arrays (visqol_measure_batch): the
     # PCIe-inclusive rate of the plain C entry point, one step.
@@ -322,6 +345,7 @@ def main() -> int:
                                            else None),
         "stage_error_fast_vs_f64": err,
         "autotune": autotune,
+        "gradients": grad,
     }
     if prof["launches"] and prof["ms"] > 0:
         ms_per_launch = prof["ms"] / prof["launches"]
