@@ -548,8 +548,11 @@ struct MidPlan {
 // terms and runs the inverse stages of distance 1, 2, 4 in registers; the
 // remaining inverse steps mirror the forward ones, the last storing straight
 // to memory.  Every butterfly and twiddle is the stage-by-stage one.
+// Four waves per SIMD (at most 128 VGPRs): 2048-element tiles then fit four
+// workgroups per CU instead of three (a few spilled registers cost less
+// than the lost occupancy); 4096-element tiles are held to two by LDS.
 template <int K, int TILE>
-__global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TILE / 1024))) void k_fft_mid(const LagJob* __restrict__ jobs, const double2* __restrict__ tw,
+__global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_mid(const LagJob* __restrict__ jobs, const double2* __restrict__ tw,
                                                       int njobs) {
     constexpr int THREADS = TILE / 8;
     constexpr int kHalfTile = TILE / 2;
@@ -567,10 +570,15 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TILE /
         cb = (4LL << k) - 1 - t;
     }
     auto gidx = [&](int e) -> long long { return (e < kHalfTile ? ca : cb) * kHalfTile + (e & (kHalfTile - 1)); };
-    // Tile element e sits in LDS at e ^ ((e >> 4) & 7), a permutation within
-    // each run of 16: lanes that each hold eight consecutive elements then
-    // spread over all banks instead of two (an 8-way conflict unswizzled).
-    auto swz = [](int e) { return e ^ ((e >> 4) & 7); };
+    // Tile element e sits in LDS at e ^ h(e), h taking bits 0-2 from e's
+    // bits 3-5 and bit 3 from bits 5 ^ 6: a permutation within each run of
+    // 16 (bit 3 of the result recovers e's bit 3 from bits 5, 6, then bits
+    // 0-2 from bits 3-5).  Every access of a step — eight lanes or sixteen
+    // (16-byte elements over 128- or 256-byte bank rows): consecutive runs
+    // (distances of 32 and up), two runs of eight 32 apart (distance 8),
+    // stride 8 (the central step and its mirror reads) — then falls on
+    // distinct banks.
+    auto swz = [](int e) { return e ^ (((e >> 3) & 7) | ((((e >> 5) ^ (e >> 6)) & 1) << 3)); };
     double2 x[8];
     int idx[8];
     // LDS hand-over between steps: store at the current layout, barrier,
