@@ -86,22 +86,30 @@ def _choices(trace):
             trace["realign_frames"])
 
 
-# (name, mode, lattice, step, relative bar)
+# (name, mode, lattice, step, relative bar, context options, degraded lead
+# in samples: the degraded signal starts that many samples into its
+# delayed copy, so it leads the reference and the global lag prepends
+# zeros instead of dropping samples)
 CASES = [
-    ("audio_svr", 0, True, 1e-5, 1e-3),
-    ("speech_exponential", 1, False, 1e-5, 1e-3),
+    ("audio_svr", 0, True, 1e-5, 1e-3, {}, 0),
+    ("audio_svr_leading_deg", 0, True, 1e-5, 1e-3, {}, 400),
+    ("audio_no_realignment", 0, True, 1e-5, 1e-3, {"disable_realignment": True}, 0),
+    ("audio_no_global_alignment", 0, True, 1e-5, 1e-3, {"disable_global_alignment": True}, 0),
+    ("speech_exponential", 1, False, 1e-5, 1e-3, {}, 0),
+    ("speech_exponential_unscaled", 1, False, 1e-5, 1e-3, {"use_unscaled_speech_mos_mapping": True}, 0),
 ]
 
 
-@pytest.mark.parametrize("name,mode,lattice,eps,rel", CASES, ids=[c[0] for c in CASES])
-def test_gradient_matches_central_differences(native, synth, name, mode, lattice, eps, rel):
+@pytest.mark.parametrize("name,mode,lattice,eps,rel,opts,lead", CASES, ids=[c[0] for c in CASES])
+def test_gradient_matches_central_differences(native, synth, name, mode, lattice, eps, rel, opts, lead):
     if mode == 0:
         ref, deg = synth.audio_pair(21, duration=3.0)
         sr = 48000
     else:
         ref, deg = synth.speech_pair(22, duration=3.0)
         sr = 16000
-    ctx = native.Context(mode, use_lattice_model=lattice, precision="exact")
+    deg = deg[lead:].copy()
+    ctx = native.Context(mode, use_lattice_model=lattice, precision="exact", **opts)
     say(f"gradient {name}: {deg.size} samples at {sr}")
     rc, res, st, grads = ctx.measure_batch_grad([ref], [deg], sr)
     assert rc == 0 and st == [0]
@@ -109,6 +117,8 @@ def test_gradient_matches_central_differences(native, synth, name, mode, lattice
     base_res, base_tr = ctx.measure_trace(ref, deg, sr)
     assert res[0]["moslqo"] == base_res["moslqo"]
     assert 1.0 < base_res["moslqo"] < 5.0
+    if lead:
+        assert base_tr["lag"] > 0  # zeros prepended to the aligned signal
     g_norm = np.linalg.norm(g)
     assert g_norm > 0 and np.all(np.isfinite(g))
     rng = np.random.default_rng(0x5EED)
