@@ -98,6 +98,15 @@ void PinnedArena::reset() {
     for (auto& c : chunks_) c.used = 0;
 }
 
+// Host threads for per-pair host work over n pairs, one per `per` pairs,
+// at most kMaxHostThreads (the host cores a process on one GPU has).
+int host_threads(size_t n, size_t per) {
+    constexpr unsigned kMaxHostThreads = 16;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    if (n < 2 * per) return 1;
+    return (int)std::min<size_t>({(size_t)kMaxHostThreads, (size_t)hw, n / per});
+}
+
 // ---------------------------------------------------------------- engine
 Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
                std::unique_ptr<LatticeModel, LatticeDeleter> lattice)
@@ -957,9 +966,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     // the pair's result (the lattice interpreter only reads the model's
     // constant tensors and writes its own per-call values), so larger
     // batches spread them over a few host threads.
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const int per = opts_.mode == 1 && opts_.lattice ? 1 : 8;  // pairs per thread worth a thread
-    const int nthreads = n >= 2 * per ? (int)std::min<unsigned>({8u, hw, (unsigned)(n / per)}) : 1;
+    const int nthreads = host_threads(n, per);
     if (nthreads > 1) {
         std::vector<std::thread> pool;
         for (int t = 0; t < nthreads; t++)

@@ -119,6 +119,10 @@ class PinnedArena {
 
 struct PairState;  // engine_state.h
 
+// Host threads for per-pair host work over n pairs: one per `per` pairs, at
+// most 16 and the machine's hardware threads, 1 below 2·per pairs.
+int host_threads(size_t n, size_t per);
+
 struct BackendError {
     int code;
     std::string msg;

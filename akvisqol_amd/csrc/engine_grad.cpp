@@ -232,8 +232,7 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
                     h.g_raw_deg[(size_t)b * s.nf_deg + tr.g0 + f] += rg.second[(size_t)b * tr.nf + f];
         }
     };
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nthreads = (int)std::min<size_t>({8u, (size_t)hw, todo.size()});
+    const int nthreads = host_threads(todo.size(), 1);
     if (nthreads > 1) {
         std::vector<std::thread> pool;
         for (int t = 0; t < nthreads; t++)
