@@ -646,6 +646,24 @@ int visqol_hip_prepare_grad(const double* raw_ref, const double* raw_deg, size_t
     return VISQOL_OK;
 }
 
+int visqol_hip_lattice_predict(const uint8_t* model, size_t len, size_t nb, size_t count, const double* fvnsim,
+                               const double* fvnsim10, const double* fstdnsim, const double* fvdegenergy, double* y) {
+    if (!model || !y || nb == 0 || nb > kMaxBands || (count && (!fvnsim || !fvnsim10 || !fstdnsim || !fvdegenergy)))
+        return invalid("invalid argument");
+    std::string err;
+    auto m = load_lattice(std::string((const char*)model, len), (int)nb, err);
+    if (!m) {
+        set_error("lattice model: %s", err.c_str());
+        return VISQOL_E_MODEL;
+    }
+    for (size_t k = 0; k < count; k++)
+        if (!lattice_predict(*m, fvnsim + k * nb, fvnsim10 + k * nb, fstdnsim + k * nb, fvdegenergy + k * nb, y[k])) {
+            set_error("lattice model evaluation failed");
+            return VISQOL_E_MODEL;
+        }
+    return VISQOL_OK;
+}
+
 int visqol_hip_gammatone_shape(int shape, int* waves, int* bands_per_lane, int* chunk) {
     if (shape < 0 || shape >= gammatone_shape_count()) return invalid("no such spectrogram block shape");
     const GtShape s = gammatone_shape(shape);

@@ -133,6 +133,7 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_profile_read": (c_int, [c_void_p, D, D, D, POINTER(c_int64)]),
         "visqol_hip_profile_phases": (c_int, [c_void_p, D, c_int]),
         "visqol_hip_profile_sim": (c_int, [c_void_p, D, D, POINTER(c_int64)]),
+        "visqol_hip_lattice_predict": (c_int, [ctypes.c_char_p, c_size_t, c_size_t, c_size_t, D, D, D, D, D]),
         "visqol_hip_frame_grad": (c_int, [c_uint32, c_uint32, D, c_size_t, D, c_size_t, c_size_t, D]),
         "visqol_hip_patch_similarity_grad": (c_int, [D, D, c_size_t, c_size_t, D, D, D, D, D]),
         "visqol_hip_prepare_grad": (c_int, [D, D, c_size_t, c_size_t, c_size_t, D, D, D, D]),
@@ -648,6 +649,15 @@ def _check_count(rc: int) -> int:
     if rc < 0:
         _check(rc)
     return rc
+
+
+def lattice_predict(model: bytes, fvnsim, fvnsim10, fstdnsim, fvdegenergy) -> np.ndarray:
+    """The lattice mapping of [count, nb] feature arrays (host)."""
+    f = [np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64) for x in (fvnsim, fvnsim10, fstdnsim, fvdegenergy)]
+    count, nb = f[0].shape
+    y = np.zeros(count)
+    _check(lib.visqol_hip_lattice_predict(model, len(model), nb, count, *[_dptr(x) for x in f], _dptr(y)))
+    return y
 
 
 def frame_grad(mode: int, sample_rate: int, frame, g, lo: int = 0, hi: int | None = None) -> np.ndarray:

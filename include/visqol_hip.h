@@ -145,6 +145,12 @@ int visqol_hip_patch_similarity_grad(const double *ref, const double *deg, size_
 int visqol_hip_prepare_grad(const double *raw_ref, const double *raw_deg, size_t nb, size_t nfr, size_t nfd,
                             const double *g_ref_prep, const double *g_deg_prep, double *g_raw_ref, double *g_raw_deg);
 
+/* The TF-Lattice quality mapping on the host (lattice.zig:49-71): the
+ * model's flatbuffer bytes, then `count` feature vectors of nb bands each
+ * (row k of each array at k·nb), one MOS per vector into y. */
+int visqol_hip_lattice_predict(const uint8_t *model, size_t len, size_t nb, size_t count, const double *fvnsim,
+                               const double *fvnsim10, const double *fstdnsim, const double *fvdegenergy, double *y);
+
 /* Spectrogram kernel block shapes.  visqol_hip_gammatone_shape describes
  * shape `shape` (waves per block, bands per lane, samples per staged chunk;
  * 64 frames per block) and returns the number of shapes.  Every shape gives
