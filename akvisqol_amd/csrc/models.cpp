@@ -214,6 +214,13 @@ struct Val {
     }
 };
 
+// An elementwise node's broadcast for one pair of operand shapes: output
+// shape and each output element's source indices.
+struct BroadcastCache {
+    bool ok = false;
+    std::vector<size_t> xs, ys, shape, ix, iy;
+};
+
 struct Node {
     Op op;
     std::vector<int> in, out;
@@ -222,6 +229,9 @@ struct Node {
     bool adj_x = false, adj_y = false, keep_dims = false;
     bool has_shape = false;
     std::vector<int32_t> new_shape;
+    // filled by the model's warm-up call at load (single-threaded), read-only
+    // afterwards; used only when the operands' shapes match
+    mutable BroadcastCache bc;
 };
 
 inline float act_apply(int act, float x) {
@@ -296,6 +306,10 @@ class LatticeModel {
     // dropping them there keeps a call's working set small and its heap
     // blocks reused instead of returned to the system and faulted back in.
     std::vector<std::vector<int>> release;
+    // Tensors a call sets that no node releases (never read, or the output):
+    // cleared after the call so the per-thread workspace starts empty.
+    std::vector<int> unreleased;
+    bool warming = false;  // the load-time call that fills the broadcast caches
 
     void plan_release() {
         std::vector<int> last(consts.size(), -1);
@@ -305,6 +319,15 @@ class LatticeModel {
         release.assign(nodes.size(), {});
         for (size_t t = 0; t < last.size(); t++)
             if (last[t] >= 0 && (int)t != output) release[last[t]].push_back((int)t);
+        std::vector<char> set(consts.size(), 0);
+        for (const Node& n : nodes)
+            for (int t : n.out)
+                if (t >= 0 && (size_t)t < set.size()) set[t] = 1;
+        for (int t : feat) set[t] = 1;
+        if (tau >= 0) set[tau] = 1;
+        unreleased.clear();
+        for (size_t t = 0; t < set.size(); t++)
+            if (set[t] && (last[t] < 0 || (int)t == output)) unreleased.push_back((int)t);
     }
 
     void run(std::vector<Val>& v) const {
@@ -346,7 +369,9 @@ class LatticeModel {
             case Op::Add: case Op::Sub: case Op::Mul: case Op::Min: case Op::Max: {
                 const Val &x = in(v, n, 0), &y = in(v, n, 1);
                 if (x.dt != Dt::F32 || y.dt != Dt::F32) throw GraphError{};
-                auto shape = bshape(x.shape, y.shape);
+                BroadcastCache& bc = n.bc;
+                const bool hit = bc.ok && x.shape == bc.xs && y.shape == bc.ys;
+                auto shape = hit ? bc.shape : bshape(x.shape, y.shape);
                 size_t total = 1;
                 for (size_t d : shape) total *= d;
                 std::vector<float> o(total);
@@ -374,9 +399,23 @@ class LatticeModel {
                 if (xc == total && yc == total && x.shape == y.shape) apply(dense(xf), dense(yf));
                 else if (xc == total && x.shape == shape && yc == 1) apply(dense(xf), scalar(yf));
                 else if (yc == total && y.shape == shape && xc == 1) apply(scalar(xf), dense(yf));
-                else {
+                else if (hit) {
+                    const size_t* ix = bc.ix.data();
+                    const size_t* iy = bc.iy.data();
+                    apply([&](size_t e) { return xf[ix[e]]; }, [&](size_t e) { return yf[iy[e]]; });
+                } else {
                     auto ix = bindex(x.shape, shape), iy = bindex(y.shape, shape);
                     apply([&](size_t e) { return xf[ix[e]]; }, [&](size_t e) { return yf[iy[e]]; });
+                    if (warming) {
+                        bc.ix = std::move(ix);
+                        bc.iy = std::move(iy);
+                    }
+                }
+                if (warming && !hit) {
+                    bc.xs = x.shape;
+                    bc.ys = y.shape;
+                    bc.shape = shape;
+                    bc.ok = true;
                 }
                 r = floats(std::move(shape), std::move(o));
                 break;
@@ -501,7 +540,11 @@ class LatticeModel {
                 if (kk != k2) throw GraphError{};
                 std::vector<size_t> xb(x.shape.begin(), x.shape.end() - 2), yb(y.shape.begin(), y.shape.end() - 2);
                 auto ob = bshape(xb, yb);
-                auto ix = bindex(xb, ob), iy = bindex(yb, ob);
+                std::vector<size_t> ix{0}, iy{0};  // no batch dimensions: one matrix each
+                if (!ob.empty()) {
+                    ix = bindex(xb, ob);
+                    iy = bindex(yb, ob);
+                }
                 std::vector<float> o(ix.size() * m * nn);
                 for (size_t bi = 0; bi < ix.size(); bi++) {
                     const float* xm = x.f->data() + ix[bi] * m * kk;
@@ -509,9 +552,18 @@ class LatticeModel {
                     if (kk == 1) {
                         // outer product: fma(p, q, +0) rounds p*q once and turns
                         // a -0 product into +0, which is exactly p*q + 0.0f
-                        float* ob_ = o.data() + bi * m * nn;
-                        for (size_t a = 0; a < m; a++)
-                            for (size_t c = 0; c < nn; c++) ob_[a * nn + c] = xm[a] * ym[c] + 0.0f;
+                        float* __restrict__ ob_ = o.data() + bi * m * nn;
+                        const float* __restrict__ xa = xm;
+                        if (nn == 2) {
+                            const float y0 = ym[0], y1 = ym[1];
+                            for (size_t a = 0; a < m; a++) {
+                                ob_[2 * a] = xa[a] * y0 + 0.0f;
+                                ob_[2 * a + 1] = xa[a] * y1 + 0.0f;
+                            }
+                        } else {
+                            for (size_t a = 0; a < m; a++)
+                                for (size_t c = 0; c < nn; c++) ob_[a * nn + c] = xa[a] * ym[c] + 0.0f;
+                        }
                         continue;
                     }
                     for (size_t a = 0; a < m; a++)
@@ -804,12 +856,25 @@ std::unique_ptr<LatticeModel, LatticeDeleter> load_lattice(const std::string& by
     m->output = m->find(m->outputs, "predictions");
     if (m->tau < 0 || m->output < 0) { err = "MissingInput"; return nullptr; }
     m->plan_release();
+    // one call on zero features fills the broadcast caches (shapes here do
+    // not depend on the feature values; each use checks its shapes anyway)
+    {
+        std::vector<double> z(nb, 0.0);
+        double y = 0.0;
+        m->warming = true;
+        lattice_predict(*m, z.data(), z.data(), z.data(), z.data(), y);
+        m->warming = false;
+    }
     return m;
 }
 
 bool lattice_predict(const LatticeModel& m, const double* f0, const double* f1, const double* f2, const double* f3,
                      double& out) {
-    std::vector<Val> v(m.consts.size());  // per-call values; constants are read in place
+    // Per-call values (constants are read in place) in a per-thread table
+    // that every call leaves empty: released tensors are cleared as their
+    // last reader finishes, the rest below.
+    thread_local std::vector<Val> v;
+    if (v.size() < m.consts.size()) v.resize(m.consts.size());
     const double* feats[4] = {f0, f1, f2, f3};
     for (int f = 0; f < 4; f++)
         for (int b = 0; b < m.nb; b++) {
@@ -826,17 +891,23 @@ bool lattice_predict(const LatticeModel& m, const double* f0, const double* f1, 
     t.shape = {1};
     t.f = std::make_shared<std::vector<float>>(1, 0.5f);
     v[m.tau] = t;
+    bool ok = true;
     try {
         m.run(v);
     } catch (const GraphError&) {
-        return false;
+        ok = false;
     } catch (const FbError&) {
-        return false;
+        ok = false;
     }
-    const Val& o = v[m.output];
-    if (!o.set || o.dt != Dt::F32 || o.f->empty()) return false;
-    out = (double)(*o.f)[0];
-    return true;
+    if (ok) {
+        const Val& o = v[m.output];
+        ok = o.set && o.dt == Dt::F32 && !o.f->empty();
+        if (ok) out = (double)(*o.f)[0];
+        for (int t : m.unreleased) v[t] = Val{};
+    } else {
+        for (Val& x : v) x = Val{};
+    }
+    return ok;
 }
 
 }  // namespace vq
