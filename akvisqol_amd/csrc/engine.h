@@ -241,8 +241,9 @@ class Engine {
                   double* const* grads);
     // ∂moslqo/∂(fvnsim, fvnsim10, fstdnsim, fvdegenergy) of this engine's
     // quality mapping at the given features (visqol.zig:100-149).
+    // The lattice's slopes use up to `threads` host threads.
     void mapper_grad(int nb, const double* f0, const double* f1, const double* f2, const double* f3, double* g0,
-                     double* g1, double* g2, double* g3) const;
+                     double* g1, double* g2, double* g3, int threads = 1) const;
     // Reductions of `jobs` into d_out (serial: the reference's order).
     void sums(const std::vector<SumJob>& jobs, double* d_out, bool serial);
     // FFT cross-correlation lags of `jobs` (envelope sums already in

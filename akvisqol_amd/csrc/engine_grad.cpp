@@ -48,7 +48,7 @@ bool column_nonzero(const double* g, int nb, int stride) {
 }  // namespace
 
 void Engine::mapper_grad(int nb, const double* f0, const double* f1, const double* f2, const double* f3, double* g0,
-                         double* g1, double* g2, double* g3) const {
+                         double* g1, double* g2, double* g3, int threads) const {
     double* g[4] = {g0, g1, g2, g3};
     for (double* x : g) std::fill(x, x + nb, 0.0);
     if (opts_.mode == 0) {
@@ -58,22 +58,33 @@ void Engine::mapper_grad(int nb, const double* f0, const double* f1, const doubl
         svr_->gradient(f0, g0);
     } else if (opts_.lattice) {
         // piecewise linear and evaluated in f32: central differences with
-        // steps of many f32 ulps (visqol.zig:132-146)
-        double x[4][kMaxBands];
+        // steps of many f32 ulps (visqol.zig:132-146), one feature at a
+        // time, spread over `threads` host threads
         const double* f[4] = {f0, f1, f2, f3};
-        for (int k = 0; k < 4; k++) std::copy(f[k], f[k] + nb, x[k]);
-        for (int k = 0; k < 4; k++)
-            for (int b = 0; b < nb; b++) {
-                const double v = x[k][b];
-                const double h = 1e-3 * std::max(1.0, fabs(v));
-                double up = 0.0, down = 0.0;
-                x[k][b] = v + h;
-                const bool ok_up = lattice_predict(*lattice_, x[0], x[1], x[2], x[3], up);
-                x[k][b] = v - h;
-                const bool ok_down = lattice_predict(*lattice_, x[0], x[1], x[2], x[3], down);
-                x[k][b] = v;
-                g[k][b] = ok_up && ok_down ? (up - down) / (2.0 * h) : 0.0;
-            }
+        auto slope = [&](int idx) {
+            double x[4][kMaxBands];
+            for (int k = 0; k < 4; k++) std::copy(f[k], f[k] + nb, x[k]);
+            const int k = idx / nb, b = idx % nb;
+            const double v = x[k][b];
+            const double h = 1e-3 * std::max(1.0, fabs(v));
+            double up = 0.0, down = 0.0;
+            x[k][b] = v + h;
+            const bool ok_up = lattice_predict(*lattice_, x[0], x[1], x[2], x[3], up);
+            x[k][b] = v - h;
+            const bool ok_down = lattice_predict(*lattice_, x[0], x[1], x[2], x[3], down);
+            g[k][b] = ok_up && ok_down ? (up - down) / (2.0 * h) : 0.0;
+        };
+        const int total = 4 * nb;
+        if (threads > 1) {
+            std::vector<std::thread> pool;
+            for (int t = 0; t < threads; t++)
+                pool.emplace_back([&, t] {
+                    for (int idx = t; idx < total; idx += threads) slope(idx);
+                });
+            for (auto& th : pool) th.join();
+        } else {
+            for (int idx = 0; idx < total; idx++) slope(idx);
+        }
     } else {
         // exponential speech mapping (visqol.zig:117-131)
         double sum = 0.0;
@@ -162,6 +173,12 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
     for (auto& sv : staged) memcpy(sv.second->data(), sv.first, sizeof(double) * sv.second->size());
 
     // ---------------------------------------------------------------- host: stats → raw spectrograms
+    // Pairs spread over host threads; with fewer pairs than threads, each
+    // pair's lattice slopes (4·nb pairs of predictions) take the rest.
+    const int nthreads = host_threads(todo.size(), 1);
+    const int inner_threads = opts_.mode == 1 && opts_.lattice
+                                  ? std::max(1, host_threads(16, 1) / std::max<int>(1, (int)todo.size()))
+                                  : 1;
     auto host_pass = [&](size_t q) {
         const int i = todo[q];
         PairState& s = st[i];
@@ -170,7 +187,9 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
         std::vector<double> gst;
         result_grad(r.matches, nb, rp.frame_duration,
                     [&](const double* f0, const double* f1, const double* f2, const double* f3, double* g0,
-                        double* g1, double* g2, double* g3) { mapper_grad(nb, f0, f1, f2, f3, g0, g1, g2, g3); },
+                        double* g1, double* g2, double* g3) {
+                        mapper_grad(nb, f0, f1, f2, f3, g0, g1, g2, g3, inner_threads);
+                    },
                     gst);
         std::vector<int> row_of(s.np, -1);
         for (size_t k = 0; k < s.rows.size(); k++)
@@ -232,7 +251,6 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
                     h.g_raw_deg[(size_t)b * s.nf_deg + tr.g0 + f] += rg.second[(size_t)b * tr.nf + f];
         }
     };
-    const int nthreads = host_threads(todo.size(), 1);
     if (nthreads > 1) {
         std::vector<std::thread> pool;
         for (int t = 0; t < nthreads; t++)
