@@ -183,7 +183,7 @@ __device__ inline double sim_cell_fast(double mr, double rms, double ssr, double
     const double mrd = mr * md;
     const double num = __builtin_fma(2.0, mrd, c1) * ((crd - mrd) + c3);
     const double t = ssr * ssd;
-    const double den = (t < 0.0 ? 0.0 : t) + c3;
+    const double den = fmax(t, 0.0) + c3;  // the same value as (t < 0 ? 0 : t) + c3 for every non-NaN t
     const double d = __builtin_fma(md, md, rms + c1) * den;
     double q = __builtin_amdgcn_rcp(d);
     double err = __builtin_fma(-d, q, 1.0);
