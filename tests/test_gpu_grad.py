@@ -239,3 +239,21 @@ def test_batch_prepared_and_failures(native, synth):
     for k in range(3):
         assert pres[k]["moslqo"] == res[k]["moslqo"]
         assert np.array_equal(pgrads[k], grads[k])
+
+
+def test_speech_prepared_gradient_equals_batch(native, synth):
+    # prepared speech references carry their VAD patch indices
+    pairs = [synth.speech_pair(s, duration=3.0, snr_db=35.0) for s in (51, 52)]
+    refs = [p[0] for p in pairs]
+    degs = [p[1] for p in pairs]
+    ctx = native.Context(1, use_lattice_model=False, precision="fast")
+    say("speech gradients: 2 pairs, plain and prepared")
+    rc, res, st, grads = ctx.measure_batch_grad(refs, degs, 16000)
+    assert rc == 0 and st == [0, 0]
+    prep = ctx.prepare(refs, 16000)
+    rc2, pres, pst, pgrads = ctx.measure_prepared_grad(prep, degs)
+    assert rc2 == 0 and pst == [0, 0]
+    for k in range(2):
+        assert pres[k]["moslqo"] == res[k]["moslqo"]
+        assert np.linalg.norm(grads[k]) > 0
+        assert np.array_equal(pgrads[k], grads[k])
