@@ -108,3 +108,28 @@ def test_invalid_arguments_before_device():
     assert L.visqol_measure_batch(None, 0, None, None, None, None, 48000, 0, None, None) == native.E_INVALID_ARGUMENT
     L.visqol_result_free(None)
     L.visqol_destroy(None)
+
+
+def test_gradient_and_probe_arguments_before_device():
+    # the gradient entry points check their context and arrays first, as
+    # visqol_measure_batch does (capi.zig:340-350); the host probes check
+    # theirs without touching a device
+    from akvisqol_amd import native
+
+    L = native.lib
+    assert L.visqol_hip_measure_batch_grad(None, 1, None, None, None, None, 48000, None, None, None) == \
+        native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_measure_prepared_grad(None, None, 1, None, None, None, None, None, None) == \
+        native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_patch_similarity_grad(None, None, 2, 2, None, None, None, None, None) == \
+        native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_prepare_grad(None, None, 2, 2, 2, None, None, None, None) == native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_lattice_predict(None, 0, 21, 1, None, None, None, None, None) == native.E_INVALID_ARGUMENT
+    frame = np.zeros(3840)
+    g = np.zeros(32)
+    out = np.zeros(3840)
+    p = native._dptr
+    # wrong window length, lo > hi, hi beyond the window: rejected before any device use
+    assert L.visqol_hip_frame_grad(0, 48000, p(frame), 100, p(g), 0, 100, p(out)) == native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_frame_grad(0, 48000, p(frame), 3840, p(g), 10, 5, p(out)) == native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_frame_grad(0, 48000, p(frame), 3840, p(g), 0, 4000, p(out)) == native.E_INVALID_ARGUMENT
