@@ -781,7 +781,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             int pair, match;
             Layout rl, dl;
             long long rlen, dlen;
-            double *sref, *sdeg;
+            const double *sref, *sdeg;
             int lagjob;
             Geometry geo;
             int nf = 0;
@@ -804,11 +804,17 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                 rj.rlen = rj.rl.pad + rj.rl.len;
                 rj.dlen = rj.dl.pad + rj.dl.len;
                 if (rj.rlen < W || rj.dlen < W) continue;
-                rj.sref = (double*)arena_.alloc(sizeof(double) * rj.rlen);
-                rj.sdeg = (double*)arena_.alloc(sizeof(double) * rj.dlen);
-                slj.push_back(SliceJob{s.dref, rj.sref, rj.rl.pad, rj.rl.base, rj.rl.len, rj.rlen});
-                slj.push_back(SliceJob{s.degs, rj.sdeg, rj.dl.pad, rj.dl.base, rj.dl.len, rj.dlen});
-                max_slice = std::max(max_slice, std::max(rj.rlen, rj.dlen));
+                // A slice with no padding is a window of its signal, read in
+                // place; a padded one is copied with its zeros.
+                auto slice = [&](const double* src, const Layout& l, long long total) -> const double* {
+                    if (l.pad == 0) return src + l.base;
+                    double* dst = (double*)arena_.alloc(sizeof(double) * total);
+                    slj.push_back(SliceJob{src, dst, l.pad, l.base, l.len, total});
+                    max_slice = std::max(max_slice, total);
+                    return dst;
+                };
+                rj.sref = slice(s.dref, rj.rl, rj.rlen);
+                rj.sdeg = slice(s.degs, rj.dl, rj.dlen);
                 rjs.push_back(rj);
             }
         }
