@@ -7,6 +7,7 @@
 // are sequential f32 fused multiply-adds, the TFLite builtin order the
 // reference reproduces (tflite.zig:665-671).
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <cstdlib>
@@ -330,10 +331,394 @@ class LatticeModel {
             if (set[t] && (last[t] < 0 || (int)t == output)) unreleased.push_back((int)t);
     }
 
-    void run(std::vector<Val>& v) const {
+    // keep: leave every tensor's value in v (the recording call that the
+    // execution plan is built from)
+    void run(std::vector<Val>& v, bool keep = false) const {
         for (size_t i = 0; i < nodes.size(); i++) {
             exec(nodes[i], v);
-            for (int t : release[i]) v[t] = Val{};
+            if (!keep)
+                for (int t : release[i]) v[t] = Val{};
+        }
+    }
+
+    // ---------------------------------------------------------------- execution plan
+    // The graph's shapes do not depend on the features, so after a recording
+    // call every tensor that does not depend on them (constants, shape
+    // arithmetic, tau) is folded to its recorded value, and the rest runs
+    // as a list of steps over one float arena: no allocation, no shape or
+    // broadcast work per call, reshapes as aliases, slots reused once a
+    // tensor's last reader has run.  Each step does the interpreter's
+    // arithmetic on the same elements in the same order, so predictions are
+    // bit-identical; a graph the plan cannot express (integer tensors that
+    // depend on the features, say) keeps the interpreter.
+    struct Operand {
+        const float* p = nullptr;  // folded value, or
+        int off = -1;              // arena offset
+    };
+    struct Step {
+        Op op = Op::Unknown;
+        int act = 0;
+        int out = 0;
+        size_t n = 0;  // output elements
+        Operand x, y, bias;
+        const size_t* ix = nullptr;  // broadcast maps (nullptr: dense, or scalar when the count is 1)
+        const size_t* iy = nullptr;
+        size_t xc = 0, yc = 0;       // operand element counts
+        size_t m = 0, kk = 0, nn = 0, batch = 0, nout = 0, nin = 0;
+        bool adj_x = false, adj_y = false;
+        std::vector<Operand> ins;    // concat inputs
+        std::vector<size_t> chunks;  // concat: per input chunk; split: chunk
+        std::vector<int> outs;       // split outputs
+        size_t outer = 1;
+    };
+    std::vector<Step> plan;
+    std::vector<std::vector<float>> plan_values;  // folded tensors the plan reads
+    std::vector<std::vector<size_t>> plan_maps;
+    std::vector<int> plan_feat;  // arena offsets of the feature inputs
+    size_t plan_size = 0;
+    int plan_out = -1;
+    bool plan_ok = false;
+
+    void build_plan(const std::vector<Val>& rec) {
+        plan_ok = false;
+        const size_t T = consts.size();
+        std::vector<char> dyn(T, 0);
+        for (int t : feat) dyn[t] = 1;
+        for (const Node& n : nodes) {
+            bool d = false;
+            for (int t : n.in)
+                if (t >= 0 && (size_t)t < T && dyn[t]) d = true;
+            if (n.op == Op::Shape) d = false;
+            for (int t : n.out)
+                if (t >= 0 && (size_t)t < T) dyn[t] = d;
+        }
+        if (output < 0 || !dyn[output]) return;
+        auto value = [&](int t) -> const Val& { return rec[t].set ? rec[t] : consts[t]; };
+        // storage: aliases share their source's slot; a slot is free after
+        // the last reader of any tensor on it
+        std::vector<int> owner(T, -1);
+        std::vector<int> last(T, -1);
+        for (int t : feat) owner[t] = t;
+        for (size_t i = 0; i < nodes.size(); i++) {
+            const Node& n = nodes[i];
+            for (int t : n.out)
+                if (t >= 0 && dyn[t]) owner[t] = t;
+            if ((n.op == Op::Reshape || n.op == Op::ExpandDims) && dyn[n.in[0]] && !n.out.empty())
+                owner[n.out[0]] = owner[n.in[0]];
+        }
+        for (size_t i = 0; i < nodes.size(); i++)
+            for (int t : nodes[i].in)
+                if (t >= 0 && (size_t)t < T && dyn[t]) last[owner[t]] = std::max(last[owner[t]], (int)i);
+        last[owner[output]] = (int)nodes.size();  // never freed
+        std::vector<int> off(T, -1);
+        std::vector<char> freed(T, 0);
+        std::vector<std::pair<size_t, size_t>> free_list;  // (offset, size)
+        size_t top = 0;
+        auto alloc = [&](size_t cnt) -> int {
+            const size_t sz = (cnt + 15) & ~(size_t)15;
+            for (auto it = free_list.begin(); it != free_list.end(); ++it)
+                if (it->second >= sz) {
+                    const size_t o = it->first;
+                    it->first += sz;
+                    it->second -= sz;
+                    if (it->second == 0) free_list.erase(it);
+                    return (int)o;
+                }
+            const size_t o = top;
+            top += sz;
+            return (int)o;
+        };
+        auto release_slot = [&](int t) {
+            const size_t sz = (value(t).count() + 15) & ~(size_t)15;
+            free_list.emplace_back((size_t)off[t], sz);
+        };
+        auto operand = [&](int t, size_t& count) -> Operand {
+            const Val& v = value(t);
+            count = v.count();
+            Operand o;
+            if (dyn[t]) {
+                o.off = off[owner[t]];
+                return o;
+            }
+            if (v.dt != Dt::F32 || !v.f) throw GraphError{};
+            if (consts[t].set) {
+                o.p = consts[t].f->data();  // the model's own constant
+                return o;
+            }
+            plan_values.push_back(*v.f);
+            o.p = plan_values.back().data();
+            return o;
+        };
+        try {
+            plan.clear();
+            plan_values.clear();
+            plan_maps.clear();
+            plan_feat.clear();
+            plan_values.reserve(nodes.size() * 3 + 4);  // pointers into it stay valid
+            for (int t : feat) {
+                if (value(t).count() != 1) return;
+                off[t] = alloc(1);
+                plan_feat.push_back(off[t]);
+            }
+            for (size_t i = 0; i < nodes.size(); i++) {
+                const Node& n = nodes[i];
+                bool d = false;
+                for (int t : n.out)
+                    if (t >= 0 && dyn[t]) d = true;
+                if (d) {
+                    for (int t : n.out)
+                        if (t >= 0 && owner[t] == t) {
+                            if (value(t).dt != Dt::F32 && !(n.op == Op::Equal && value(t).dt == Dt::Bool)) return;
+                            off[t] = alloc(value(t).count());
+                        }
+                    if (!plan_step(n, rec, dyn, operand, off, owner)) return;
+                }
+                for (int t : n.in)
+                    if (t >= 0 && (size_t)t < T && dyn[t]) {
+                        const int r = owner[t];
+                        if (last[r] == (int)i && !freed[r]) {
+                            release_slot(r);
+                            freed[r] = 1;
+                        }
+                    }
+                for (int t : n.out)  // written but never read
+                    if (t >= 0 && dyn[t] && owner[t] == t && last[t] < 0 && !freed[t]) {
+                        release_slot(t);
+                        freed[t] = 1;
+                    }
+            }
+        } catch (const GraphError&) {
+            plan.clear();
+            return;
+        }
+        plan_size = top;
+        plan_out = off[owner[output]];
+        plan_ok = true;
+    }
+
+    template <typename OperandFn>
+    bool plan_step(const Node& n, const std::vector<Val>& rec, const std::vector<char>& dyn, OperandFn& operand,
+                   const std::vector<int>& off, const std::vector<int>& owner) {
+        auto value = [&](int t) -> const Val& { return rec[t].set ? rec[t] : consts[t]; };
+        Step st;
+        st.op = n.op;
+        st.act = n.act;
+        if (n.out.empty()) return false;
+        const Val& outv = value(n.out[0]);
+        st.out = off[owner[n.out[0]]];
+        st.n = outv.count();
+        auto map_of = [&](const Val& src) -> const size_t* {
+            if (src.count() == st.n && src.shape == outv.shape) return nullptr;
+            if (src.count() == 1) return nullptr;
+            plan_maps.push_back(bindex(src.shape, outv.shape));
+            return plan_maps.back().data();
+        };
+        switch (n.op) {
+            case Op::Add: case Op::Sub: case Op::Mul: case Op::Min: case Op::Max: case Op::Equal: {
+                const Val &x = value(n.in[0]), &y = value(n.in[1]);
+                if (x.dt != Dt::F32 || y.dt != Dt::F32) return false;
+                st.x = operand(n.in[0], st.xc);
+                st.y = operand(n.in[1], st.yc);
+                st.ix = map_of(x);
+                st.iy = map_of(y);
+                break;
+            }
+            case Op::Relu: case Op::Relu6: case Op::Abs: {
+                if (value(n.in[0]).dt != Dt::F32) return false;
+                st.x = operand(n.in[0], st.xc);
+                break;
+            }
+            case Op::Cast: {
+                const Val& x = value(n.in[0]);
+                if (outv.dt != Dt::F32 || (x.dt != Dt::F32 && x.dt != Dt::Bool) || !dyn[n.in[0]]) return false;
+                st.x = operand(n.in[0], st.xc);
+                break;
+            }
+            case Op::Reshape: case Op::ExpandDims:
+                return true;  // an alias of its input's slot: nothing to run
+            case Op::Concat: {
+                const int ax = norm_axis(n.axis, outv.shape.size());
+                for (int d = 0; d < ax; d++) st.outer *= outv.shape[d];
+                for (size_t k = 0; k < n.in.size(); k++) {
+                    const Val& p = value(n.in[k]);
+                    if (p.dt != Dt::F32) return false;
+                    size_t inner = 1;
+                    for (size_t d = ax + 1; d < p.shape.size(); d++) inner *= p.shape[d];
+                    size_t c = 0;
+                    st.ins.push_back(operand(n.in[k], c));
+                    st.chunks.push_back(p.shape[ax] * inner);
+                }
+                break;
+            }
+            case Op::FullyConnected: {
+                const Val &x = value(n.in[0]), &w = value(n.in[1]);
+                if (x.dt != Dt::F32 || w.dt != Dt::F32 || w.shape.size() != 2) return false;
+                st.nout = w.shape[0];
+                st.nin = w.shape[1];
+                st.batch = x.count() / st.nin;
+                st.x = operand(n.in[0], st.xc);
+                st.y = operand(n.in[1], st.yc);
+                if (n.in.size() > 2 && n.in[2] >= 0) {
+                    size_t c = 0;
+                    st.bias = operand(n.in[2], c);
+                    st.kk = 1;  // has bias
+                }
+                break;
+            }
+            case Op::BatchMatMul: {
+                const Val &x = value(n.in[0]), &y = value(n.in[1]);
+                if (x.dt != Dt::F32 || y.dt != Dt::F32) return false;
+                const size_t xr = x.shape.size(), yr = y.shape.size();
+                st.adj_x = n.adj_x;
+                st.adj_y = n.adj_y;
+                st.m = n.adj_x ? x.shape[xr - 1] : x.shape[xr - 2];
+                st.kk = n.adj_x ? x.shape[xr - 2] : x.shape[xr - 1];
+                st.nn = n.adj_y ? y.shape[yr - 2] : y.shape[yr - 1];
+                std::vector<size_t> xb(x.shape.begin(), x.shape.end() - 2), yb(y.shape.begin(), y.shape.end() - 2);
+                auto ob = bshape(xb, yb);
+                std::vector<size_t> ix{0}, iy{0};
+                if (!ob.empty()) {
+                    ix = bindex(xb, ob);
+                    iy = bindex(yb, ob);
+                }
+                st.batch = ix.size();
+                plan_maps.push_back(std::move(ix));
+                st.ix = plan_maps.back().data();
+                plan_maps.push_back(std::move(iy));
+                st.iy = plan_maps.back().data();
+                st.x = operand(n.in[0], st.xc);
+                st.y = operand(n.in[1], st.yc);
+                break;
+            }
+            case Op::Split: {
+                const Val &a = value(n.in[0]), &x = value(n.in[1]);
+                if (a.dt != Dt::I32 || dyn[n.in[0]] || x.dt != Dt::F32) return false;
+                const int ax = norm_axis((*a.i)[0], x.shape.size());
+                const size_t parts = n.out.size();
+                size_t inner = 1;
+                for (int d = 0; d < ax; d++) st.outer *= x.shape[d];
+                for (size_t d = ax + 1; d < x.shape.size(); d++) inner *= x.shape[d];
+                st.chunks.push_back(x.shape[ax] / parts * inner);
+                st.x = operand(n.in[1], st.xc);
+                for (int t : n.out) st.outs.push_back(t >= 0 && owner[t] >= 0 ? off[owner[t]] : -1);
+                break;
+            }
+            default:
+                return false;
+        }
+        plan.push_back(std::move(st));
+        return true;
+    }
+
+    void run_plan(float* A) const {
+        auto get = [A](const Operand& o) -> const float* { return o.p ? o.p : A + o.off; };
+        for (const Step& st : plan) {
+            float* o = A + st.out;
+            switch (st.op) {
+                case Op::Add: case Op::Sub: case Op::Mul: case Op::Min: case Op::Max: case Op::Equal: {
+                    const float* xf = get(st.x);
+                    const float* yf = get(st.y);
+                    const size_t* ix = st.ix;
+                    const size_t* iy = st.iy;
+                    const bool xs = !ix && st.xc == 1 && st.n != 1, ys = !iy && st.yc == 1 && st.n != 1;
+                    for (size_t e = 0; e < st.n; e++) {
+                        const float p = ix ? xf[ix[e]] : xs ? xf[0] : xf[e];
+                        const float q = iy ? yf[iy[e]] : ys ? yf[0] : yf[e];
+                        float z;
+                        switch (st.op) {
+                            case Op::Add: z = p + q; break;
+                            case Op::Sub: z = p - q; break;
+                            case Op::Mul: z = p * q; break;
+                            case Op::Min: z = fminf(p, q); break;
+                            case Op::Max: z = fmaxf(p, q); break;
+                            default: o[e] = p == q ? 1.0f : 0.0f; continue;
+                        }
+                        o[e] = act_apply(st.act, z);
+                    }
+                    break;
+                }
+                case Op::Relu: case Op::Relu6: case Op::Abs: {
+                    const float* xf = get(st.x);
+                    for (size_t e = 0; e < st.n; e++) {
+                        const float a = xf[e];
+                        o[e] = st.op == Op::Relu ? fmaxf(a, 0.0f) : st.op == Op::Relu6 ? fmaxf(0.0f, fminf(a, 6.0f)) : fabsf(a);
+                    }
+                    break;
+                }
+                case Op::Cast: {
+                    const float* xf = get(st.x);
+                    for (size_t e = 0; e < st.n; e++) o[e] = xf[e];
+                    break;
+                }
+                case Op::Concat: {
+                    size_t w = 0;
+                    for (size_t oi = 0; oi < st.outer; oi++)
+                        for (size_t k = 0; k < st.ins.size(); k++) {
+                            const float* src = get(st.ins[k]) + oi * st.chunks[k];
+                            for (size_t e = 0; e < st.chunks[k]; e++) o[w++] = act_apply(st.act, src[e]);
+                        }
+                    break;
+                }
+                case Op::FullyConnected: {
+                    const float* xf = get(st.x);
+                    const float* wf = get(st.y);
+                    const float* bf = st.kk ? get(st.bias) : nullptr;
+                    for (size_t rr = 0; rr < st.batch; rr++)
+                        for (size_t oo = 0; oo < st.nout; oo++) {
+                            float acc = dot_fma(xf + rr * st.nin, wf + oo * st.nin, st.nin);
+                            if (bf) acc += bf[oo];
+                            o[rr * st.nout + oo] = act_apply(st.act, acc);
+                        }
+                    break;
+                }
+                case Op::BatchMatMul: {
+                    const float* xf = get(st.x);
+                    const float* yf = get(st.y);
+                    const size_t m = st.m, kk = st.kk, nn = st.nn;
+                    for (size_t bi = 0; bi < st.batch; bi++) {
+                        const float* __restrict__ xm = xf + st.ix[bi] * m * kk;
+                        const float* ym = yf + st.iy[bi] * kk * nn;
+                        float* __restrict__ ob = o + bi * m * nn;
+                        if (kk == 1) {
+                            if (nn == 2) {
+                                const float y0 = ym[0], y1 = ym[1];
+                                for (size_t a = 0; a < m; a++) {
+                                    ob[2 * a] = xm[a] * y0 + 0.0f;
+                                    ob[2 * a + 1] = xm[a] * y1 + 0.0f;
+                                }
+                            } else {
+                                for (size_t a = 0; a < m; a++)
+                                    for (size_t c = 0; c < nn; c++) ob[a * nn + c] = xm[a] * ym[c] + 0.0f;
+                            }
+                            continue;
+                        }
+                        for (size_t a = 0; a < m; a++)
+                            for (size_t c = 0; c < nn; c++) {
+                                float acc = 0.0f;
+                                for (size_t l = 0; l < kk; l++) {
+                                    const float p = st.adj_x ? xm[l * m + a] : xm[a * kk + l];
+                                    const float q = st.adj_y ? ym[c * kk + l] : ym[l * nn + c];
+                                    acc = fmaf(p, q, acc);
+                                }
+                                ob[a * nn + c] = acc;
+                            }
+                    }
+                    break;
+                }
+                case Op::Split: {
+                    const float* xf = get(st.x);
+                    const size_t parts = st.outs.size(), chunk = st.chunks[0];
+                    for (size_t p = 0; p < parts; p++) {
+                        if (st.outs[p] < 0) continue;
+                        float* dst = A + st.outs[p];
+                        for (size_t oi = 0; oi < st.outer; oi++)
+                            memcpy(dst + oi * chunk, xf + (oi * parts + p) * chunk, sizeof(float) * chunk);
+                    }
+                    break;
+                }
+                default:
+                    break;
+            }
         }
     }
 
@@ -856,20 +1241,51 @@ std::unique_ptr<LatticeModel, LatticeDeleter> load_lattice(const std::string& by
     m->output = m->find(m->outputs, "predictions");
     if (m->tau < 0 || m->output < 0) { err = "MissingInput"; return nullptr; }
     m->plan_release();
-    // one call on zero features fills the broadcast caches (shapes here do
-    // not depend on the feature values; each use checks its shapes anyway)
+    // One recording call on zero features fills the interpreter's broadcast
+    // caches (shapes here do not depend on the feature values; each use
+    // checks its shapes anyway) and gives the values the execution plan is
+    // built from.
     {
-        std::vector<double> z(nb, 0.0);
-        double y = 0.0;
+        std::vector<Val> rec(m->consts.size());
+        for (int t : m->feat) {
+            rec[t].set = true;
+            rec[t].dt = Dt::F32;
+            rec[t].shape = {1};
+            rec[t].f = std::make_shared<std::vector<float>>(1, 0.0f);
+        }
+        rec[m->tau].set = true;
+        rec[m->tau].dt = Dt::F32;
+        rec[m->tau].shape = {1};
+        rec[m->tau].f = std::make_shared<std::vector<float>>(1, 0.5f);
         m->warming = true;
-        lattice_predict(*m, z.data(), z.data(), z.data(), z.data(), y);
+        bool ok = true;
+        try {
+            m->run(rec, true);
+        } catch (const GraphError&) {
+            ok = false;
+        } catch (const FbError&) {
+            ok = false;
+        }
         m->warming = false;
+        const char* e = getenv("VISQOL_HIP_LATTICE_PLAN");
+        if (ok && !(e && atoi(e) == 0)) m->build_plan(rec);
     }
     return m;
 }
 
 bool lattice_predict(const LatticeModel& m, const double* f0, const double* f1, const double* f2, const double* f3,
                      double& out) {
+    if (m.plan_ok) {
+        thread_local std::vector<float> arena;
+        if (arena.size() < m.plan_size) arena.resize(m.plan_size);
+        float* A = arena.data();
+        const double* fs[4] = {f0, f1, f2, f3};
+        for (int f = 0; f < 4; f++)
+            for (int b = 0; b < m.nb; b++) A[m.plan_feat[f * m.nb + b]] = (float)fs[f][b];
+        m.run_plan(A);
+        out = (double)A[m.plan_out];
+        return true;
+    }
     // Per-call values (constants are read in place) in a per-thread table
     // that every call leaves empty: released tensors are cleared as their
     // last reader finishes, the rest below.

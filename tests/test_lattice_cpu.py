@@ -67,3 +67,13 @@ def test_lattice_rejects_truncated_model(native):
     a, b, c, d = feature_sets(13, 4)
     with pytest.raises(native.NativeError):
         native.lattice_predict(MODEL[: len(MODEL) // 2], a, b, c, d)
+
+
+def test_plan_equals_interpreter(native, monkeypatch):
+    # the execution plan built at load (folded constants, one arena) and the
+    # graph interpreter it replaces give the same bits
+    a, b, c, d = feature_sets(14, 96)
+    planned = native.lattice_predict(MODEL, a, b, c, d)
+    monkeypatch.setenv("VISQOL_HIP_LATTICE_PLAN", "0")
+    interpreted = native.lattice_predict(MODEL, a, b, c, d)
+    assert np.array_equal(planned, interpreted)
