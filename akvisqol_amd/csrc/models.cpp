@@ -171,6 +171,34 @@ float dot_fma(const float* x, const float* w, size_t n) {
     return hw ? dot_fma_hw(x, w, n) : dot_fma_lib(x, w, n);
 }
 
+// Eight such dot products side by side: each chain keeps its own order (the
+// same bits as eight dot_fma calls), and the chains overlap in the FMA
+// pipeline instead of each waiting on its own latency.
+__attribute__((target("fma"))) void dot8_fma_hw(const float* const* x, const float* const* w, size_t n, float* out) {
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f, a5 = 0.0f, a6 = 0.0f, a7 = 0.0f;
+    const float *x0 = x[0], *x1 = x[1], *x2 = x[2], *x3 = x[3], *x4 = x[4], *x5 = x[5], *x6 = x[6], *x7 = x[7];
+    const float *w0 = w[0], *w1 = w[1], *w2 = w[2], *w3 = w[3], *w4 = w[4], *w5 = w[5], *w6 = w[6], *w7 = w[7];
+    for (size_t k = 0; k < n; k++) {
+        a0 = __builtin_fmaf(x0[k], w0[k], a0);
+        a1 = __builtin_fmaf(x1[k], w1[k], a1);
+        a2 = __builtin_fmaf(x2[k], w2[k], a2);
+        a3 = __builtin_fmaf(x3[k], w3[k], a3);
+        a4 = __builtin_fmaf(x4[k], w4[k], a4);
+        a5 = __builtin_fmaf(x5[k], w5[k], a5);
+        a6 = __builtin_fmaf(x6[k], w6[k], a6);
+        a7 = __builtin_fmaf(x7[k], w7[k], a7);
+    }
+    out[0] = a0; out[1] = a1; out[2] = a2; out[3] = a3; out[4] = a4; out[5] = a5; out[6] = a6; out[7] = a7;
+}
+void dot8_fma(const float* const* x, const float* const* w, size_t n, float* out) {
+    static const bool hw = __builtin_cpu_supports("fma");
+    if (hw) {
+        dot8_fma_hw(x, w, n, out);
+        return;
+    }
+    for (int j = 0; j < 8; j++) out[j] = dot_fma_lib(x[j], w[j], n);
+}
+
 enum class Op { Add, Sub, Mul, Min, Max, Equal, Relu, Relu6, Abs, Cast, Concat, FullyConnected, BatchMatMul, Reshape,
                 ExpandDims, Split, Shape, Fill, Unknown };
 enum class Dt { None, F32, I32, Bool };
@@ -370,6 +398,9 @@ class LatticeModel {
         std::vector<size_t> chunks;  // concat: per input chunk; split: chunk
         std::vector<int> outs;       // split outputs
         size_t outer = 1;
+        // fused single-output fully-connected steps: operands per product
+        std::vector<Operand> fx, fw, fb;
+        std::vector<int> fout, fact;
     };
     std::vector<Step> plan;
     std::vector<std::vector<float>> plan_values;  // folded tensors the plan reads
@@ -394,6 +425,32 @@ class LatticeModel {
         }
         if (output < 0 || !dyn[output]) return;
         auto value = [&](int t) -> const Val& { return rec[t].set ? rec[t] : consts[t]; };
+        // Execution order: the graph's, except that each long single-output
+        // fully-connected node moves to just before its first reader (its
+        // inputs are ready earlier, and nothing between reads it), so that
+        // such nodes end up side by side and run together.
+        const int N = (int)nodes.size();
+        std::vector<int> first_reader(T, N);
+        for (int i = N - 1; i >= 0; i--)
+            for (int t : nodes[i].in)
+                if (t >= 0 && (size_t)t < T) first_reader[t] = i;
+        std::vector<std::pair<double, int>> keyed(N);
+        for (int i = 0; i < N; i++) {
+            const Node& n = nodes[i];
+            double key = i;
+            if (n.op == Op::FullyConnected && n.in.size() >= 2 && n.out.size() == 1 && n.out[0] >= 0 &&
+                dyn[n.out[0]]) {
+                const Val& w = value(n.in[1]);
+                if (w.dt == Dt::F32 && w.shape.size() == 2 && w.shape[0] == 1 && w.shape[1] >= 256 &&
+                    value(n.in[0]).count() == w.shape[1] && first_reader[n.out[0]] < N)
+                    key = first_reader[n.out[0]] - 0.5;
+            }
+            keyed[i] = {key, i};
+        }
+        std::stable_sort(keyed.begin(), keyed.end(),
+                         [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first < b.first; });
+        std::vector<int> ord(N);
+        for (int i = 0; i < N; i++) ord[i] = keyed[i].second;
         // storage: aliases share their source's slot; a slot is free after
         // the last reader of any tensor on it
         std::vector<int> owner(T, -1);
@@ -406,10 +463,10 @@ class LatticeModel {
             if ((n.op == Op::Reshape || n.op == Op::ExpandDims) && dyn[n.in[0]] && !n.out.empty())
                 owner[n.out[0]] = owner[n.in[0]];
         }
-        for (size_t i = 0; i < nodes.size(); i++)
-            for (int t : nodes[i].in)
-                if (t >= 0 && (size_t)t < T && dyn[t]) last[owner[t]] = std::max(last[owner[t]], (int)i);
-        last[owner[output]] = (int)nodes.size();  // never freed
+        for (int pos = 0; pos < N; pos++)
+            for (int t : nodes[ord[pos]].in)
+                if (t >= 0 && (size_t)t < T && dyn[t]) last[owner[t]] = std::max(last[owner[t]], pos);
+        last[owner[output]] = N;  // never freed
         std::vector<int> off(T, -1);
         std::vector<char> freed(T, 0);
         std::vector<std::pair<size_t, size_t>> free_list;  // (offset, size)
@@ -460,8 +517,8 @@ class LatticeModel {
                 off[t] = alloc(1);
                 plan_feat.push_back(off[t]);
             }
-            for (size_t i = 0; i < nodes.size(); i++) {
-                const Node& n = nodes[i];
+            for (int i = 0; i < N; i++) {
+                const Node& n = nodes[ord[i]];
                 bool d = false;
                 for (int t : n.out)
                     if (t >= 0 && dyn[t]) d = true;
@@ -491,6 +548,43 @@ class LatticeModel {
             plan.clear();
             return;
         }
+        // consecutive single-output products of one length become one step
+        std::vector<Step> fused;
+        for (size_t i = 0; i < plan.size(); i++) {
+            Step& st = plan[i];
+            const bool single = st.op == Op::FullyConnected && st.batch == 1 && st.nout == 1 && st.nin >= 256;
+            auto reads_group = [&](const Step& g) {  // a product of the group feeds this one
+                for (int o : g.fout)
+                    if ((!st.x.p && st.x.off == o) || (!st.y.p && st.y.off == o) ||
+                        (st.kk && !st.bias.p && st.bias.off == o))
+                        return true;
+                return false;
+            };
+            if (single && !fused.empty() && fused.back().op == Op::FullyConnected && !fused.back().fx.empty() &&
+                fused.back().nin == st.nin && !reads_group(fused.back())) {
+                Step& g = fused.back();
+                g.fx.push_back(st.x);
+                g.fw.push_back(st.y);
+                g.fb.push_back(st.kk ? st.bias : Operand{});
+                g.fout.push_back(st.out);
+                g.fact.push_back(st.act);
+                continue;
+            }
+            if (single) {
+                Step g;
+                g.op = Op::FullyConnected;
+                g.nin = st.nin;
+                g.fx.push_back(st.x);
+                g.fw.push_back(st.y);
+                g.fb.push_back(st.kk ? st.bias : Operand{});
+                g.fout.push_back(st.out);
+                g.fact.push_back(st.act);
+                fused.push_back(std::move(g));
+                continue;
+            }
+            fused.push_back(std::move(st));
+        }
+        plan = std::move(fused);
         plan_size = top;
         plan_out = off[owner[output]];
         plan_ok = true;
@@ -610,6 +704,33 @@ class LatticeModel {
         return true;
     }
 
+    // One elementwise step: z = f(x, y) per output element (the activation
+    // after it unless `act` is false), with the operands' element maps
+    // resolved once per step rather than per element.
+    template <typename F>
+    static void binary(const Step& st, const float* xf, const float* yf, float* o, F f, bool act = true) {
+        const size_t n = st.n;
+        const size_t* ix = st.ix;
+        const size_t* iy = st.iy;
+        const bool xs = !ix && st.xc == 1 && n != 1, ys = !iy && st.yc == 1 && n != 1;
+        auto put = [&](size_t e, float z) { o[e] = act ? act_apply(st.act, z) : z; };
+        if (!ix && !iy && !xs && !ys) {
+            for (size_t e = 0; e < n; e++) put(e, f(xf[e], yf[e]));
+        } else if (!ix && !xs && ys) {
+            const float q = yf[0];
+            for (size_t e = 0; e < n; e++) put(e, f(xf[e], q));
+        } else if (xs && !iy && !ys) {
+            const float p = xf[0];
+            for (size_t e = 0; e < n; e++) put(e, f(p, yf[e]));
+        } else {
+            for (size_t e = 0; e < n; e++) {
+                const float p = ix ? xf[ix[e]] : xs ? xf[0] : xf[e];
+                const float q = iy ? yf[iy[e]] : ys ? yf[0] : yf[e];
+                put(e, f(p, q));
+            }
+        }
+    }
+
     void run_plan(float* A) const {
         auto get = [A](const Operand& o) -> const float* { return o.p ? o.p : A + o.off; };
         for (const Step& st : plan) {
@@ -618,22 +739,13 @@ class LatticeModel {
                 case Op::Add: case Op::Sub: case Op::Mul: case Op::Min: case Op::Max: case Op::Equal: {
                     const float* xf = get(st.x);
                     const float* yf = get(st.y);
-                    const size_t* ix = st.ix;
-                    const size_t* iy = st.iy;
-                    const bool xs = !ix && st.xc == 1 && st.n != 1, ys = !iy && st.yc == 1 && st.n != 1;
-                    for (size_t e = 0; e < st.n; e++) {
-                        const float p = ix ? xf[ix[e]] : xs ? xf[0] : xf[e];
-                        const float q = iy ? yf[iy[e]] : ys ? yf[0] : yf[e];
-                        float z;
-                        switch (st.op) {
-                            case Op::Add: z = p + q; break;
-                            case Op::Sub: z = p - q; break;
-                            case Op::Mul: z = p * q; break;
-                            case Op::Min: z = fminf(p, q); break;
-                            case Op::Max: z = fmaxf(p, q); break;
-                            default: o[e] = p == q ? 1.0f : 0.0f; continue;
-                        }
-                        o[e] = act_apply(st.act, z);
+                    switch (st.op) {
+                        case Op::Add: binary(st, xf, yf, o, [](float p, float q) { return p + q; }); break;
+                        case Op::Sub: binary(st, xf, yf, o, [](float p, float q) { return p - q; }); break;
+                        case Op::Mul: binary(st, xf, yf, o, [](float p, float q) { return p * q; }); break;
+                        case Op::Min: binary(st, xf, yf, o, [](float p, float q) { return fminf(p, q); }); break;
+                        case Op::Max: binary(st, xf, yf, o, [](float p, float q) { return fmaxf(p, q); }); break;
+                        default: binary(st, xf, yf, o, [](float p, float q) { return p == q ? 1.0f : 0.0f; }, false);
                     }
                     break;
                 }
@@ -660,6 +772,33 @@ class LatticeModel {
                     break;
                 }
                 case Op::FullyConnected: {
+                    if (!st.fx.empty()) {
+                        const size_t k = st.fx.size();
+                        size_t g = 0;
+                        for (; g + 8 <= k; g += 8) {
+                            const float* xs[8];
+                            const float* ws[8];
+                            float acc[8];
+                            for (int j = 0; j < 8; j++) {
+                                xs[j] = get(st.fx[g + j]);
+                                ws[j] = get(st.fw[g + j]);
+                            }
+                            dot8_fma(xs, ws, st.nin, acc);
+                            for (int j = 0; j < 8; j++) {
+                                const Operand& b = st.fb[g + j];
+                                float a = acc[j];
+                                if (b.p || b.off >= 0) a += get(b)[0];
+                                A[st.fout[g + j]] = act_apply(st.fact[g + j], a);
+                            }
+                        }
+                        for (; g < k; g++) {
+                            float a = dot_fma(get(st.fx[g]), get(st.fw[g]), st.nin);
+                            const Operand& b = st.fb[g];
+                            if (b.p || b.off >= 0) a += get(b)[0];
+                            A[st.fout[g]] = act_apply(st.fact[g], a);
+                        }
+                        break;
+                    }
                     const float* xf = get(st.x);
                     const float* wf = get(st.y);
                     const float* bf = st.kk ? get(st.bias) : nullptr;
