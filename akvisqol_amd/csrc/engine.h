@@ -209,6 +209,11 @@ class Engine {
     struct HelperTag {};
     Engine(HelperTag, Engine& parent);
     std::unique_ptr<Engine> helper_;
+    // Uploads host-resident chunks ahead of scoring into slots_ (device
+    // buffers of slot_len_ samples each, kept between calls).
+    std::unique_ptr<Engine> uploader_;
+    std::vector<double*> slots_;
+    size_t slot_len_ = 0;
     std::recursive_mutex own_enqueue_mu_;
     std::recursive_mutex* enqueue_mu_ = &own_enqueue_mu_;  // held around timed launches on a shared stream
     bool owns_stream_ = true;
@@ -231,6 +236,11 @@ class Engine {
     template <typename T>
     T* upload(const T* v, size_t count);
     void readback(void* host_dst, const void* dev_src, size_t bytes);
+    // Host signals src[i] (len[i] samples each) to device memory dst[i]
+    // through pinned staging, on this engine's copy stream; returns once
+    // every sample is on the device.
+    void upload_signals(int n, const double* const* src, const long long* len, double* const* dst);
+    hipStream_t copy_stream_ = nullptr;  // created on the first upload_signals
     void sync();
     void check(hipError_t e, const char* what);
     // One chunk of pairs (bounded HBM footprint).

@@ -322,18 +322,23 @@ def test_tuned_batch_equals_first_shape(native, synth):
     """A batch large enough to tune the spectrogram shape (whole signals and
     realignment slices) scores exactly as the same batch on shape 0."""
     # 384 pairs x 2 signals x 3 blocks of 64 frames x 2 groups of 16 bands
-    # = 4608 units of work, above the 4096 the engine needs before it times
-    # the shapes
+    # = 4608 units of work in one chunk (signals on the device: chunks of
+    # 2^28 samples), above the 4096 the engine needs before it times the
+    # shapes
     pairs = [synth.audio_pair(40 + s, duration=2.7, delay=50 * s) for s in range(8)] * 48
-    refs, degs = [p[0] for p in pairs], [p[1] for p in pairs]
+    bufs = [(native.DeviceBuffer(r), native.DeviceBuffer(d)) for r, d in pairs[:8]]
+    refs, degs = [bufs[i % 8][0] for i in range(len(pairs))], [bufs[i % 8][1] for i in range(len(pairs))]
     say(f"tuned batch: {len(pairs)} pairs of 2.7 s")
     ctx = native.Context(native.MODE_AUDIO, precision="fast")
-    rc, tuned, st = ctx.measure_batch(refs, degs, 48000)
+    rc, tuned, st = native.measure_batch_device(ctx, refs, degs, 48000)
     assert rc == 0
     t = ctx.gammatone_tuning(48000, 0)
     say("tuning", t)
     assert t["chosen"] >= 0 and len(t["ms"]) == len(t["shapes"])
-    rc, first, _ = _ctx_with_shape(native, native.MODE_AUDIO, "fast", 0).measure_batch(refs, degs, 48000)
+    rc, first, _ = native.measure_batch_device(_ctx_with_shape(native, native.MODE_AUDIO, "fast", 0), refs, degs, 48000)
     assert rc == 0
     for a, b in zip(tuned, first):
         _bitsame(a, b)
+    for r, d in bufs:
+        r.free()
+        d.free()

@@ -201,3 +201,28 @@ def test_multi_chunk_batch_equals_single_chunk(native, synth, precision):
     assert rc == 0
     for i, g in enumerate(got):
         _same(g, want[i % len(base)])
+
+
+def test_prefetched_chunks_equal_single_chunk(native, synth):
+    """Host arrays of a batch of several chunks go up one chunk ahead of
+    scoring, into a ring of three device buffers: with five chunks the ring
+    is reused, and each pair's result still equals the one-chunk batch's,
+    bit for bit, for plain and for prepared references (whose degraded
+    signals alone are uploaded)."""
+    base = [synth.audio_pair(70 + s, duration=1.2, snr_db=15.0 + 2 * s, delay=10 * s) for s in range(8)]
+    pairs = base * 257  # 2056 pairs: chunks of 512 x 4 and 8
+    ctx = native.Context(0, precision="fast")
+    say(f"prefetched chunks: {len(pairs)} pairs")
+    rc, want, _ = ctx.measure_batch([p[0] for p in base], [p[1] for p in base], 48000)
+    assert rc == 0
+    rc, got, st = ctx.measure_batch([p[0] for p in pairs], [p[1] for p in pairs], 48000)
+    assert rc == 0 and set(st) == {0}
+    for i, g in enumerate(got):
+        _same(g, want[i % len(base)])
+    say("prefetched chunks: prepared references")
+    prep = ctx.prepare([p[0] for p in base], 48000)
+    rc, pgot, pst = ctx.measure_prepared(prep, [p[1] for p in pairs], ref_index=[i % len(base) for i in range(len(pairs))])
+    assert rc == 0 and set(pst) == {0}
+    for i, g in enumerate(pgot):
+        _same(g, want[i % len(base)])
+    prep.free()
