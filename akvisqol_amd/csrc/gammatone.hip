@@ -249,8 +249,9 @@ void launch_shape(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_
                   const double* d_hann, int nb, int window, int hop, hipStream_t stream) {
     constexpr GtShape sh = kGtShapes[V];
     constexpr int frames = PAIR ? kFramesPerWave / 2 : kFramesPerWave;
-    dim3 grid((max_nf + frames - 1) / frames, (nb + sh.waves * sh.bands_per_lane - 1) / (sh.waves * sh.bands_per_lane),
-              PAIR ? (njobs + 1) / 2 : njobs);
+    // blocks past a job's last frame exit before any work
+    dim3 grid(xcd_spread((max_nf + frames - 1) / frames),
+              (nb + sh.waves * sh.bands_per_lane - 1) / (sh.waves * sh.bands_per_lane), PAIR ? (njobs + 1) / 2 : njobs);
     hipLaunchKernelGGL((k_gammatone<FAST, sh.waves, sh.bands_per_lane, sh.chunk, PAIR>), grid, dim3(64 * sh.waves), 0,
                        stream, d_jobs, njobs, d_coef, d_fir, d_hann, nb, window, hop);
 }

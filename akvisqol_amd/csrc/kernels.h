@@ -29,6 +29,14 @@ GtShape gammatone_shape(int v);
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const FirCoef* d_fir,
                       const double* d_hann, int nb, int window, int hop, bool fast, int shape, hipStream_t stream);
 
+// Grid extent for a dimension whose trailing blocks exit at once when a job
+// is shorter than the longest (jobs of uneven size).  Workgroups go to the
+// eight XCDs round-robin by linear block index, so with an even extent the
+// busy leading blocks of every row land on the same few XCDs (a 48-wide
+// spectrogram grid over 1-60 s signals put 1.35x the mean load on XCD 0);
+// an odd extent rotates them from row to row.
+inline unsigned xcd_spread(unsigned n) { return n > 1 && n % 2 == 0 ? n + 1 : n; }
+
 // ---------------------------------------------------------------- reductions
 // Serial (reference-order) or tree reductions of one signal each.
 enum SumKind : int {

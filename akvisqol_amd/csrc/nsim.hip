@@ -341,8 +341,9 @@ void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int ma
                       bool fast, hipStream_t stream) {
     if (njobs == 0 || max_np == 0 || max_window == 0) return;
     const dim3 gt((nb * patch + 255) / 256, max_np, njobs);
-    const dim3 grid((max_window + kSimThreads - 1) / kSimThreads, (max_np + kSimPatchGroup - 1) / kSimPatchGroup,
-                    njobs);
+    // blocks past a job's patches or search windows exit before any work
+    const dim3 grid(xcd_spread((max_window + kSimThreads - 1) / kSimThreads),
+                    xcd_spread((max_np + kSimPatchGroup - 1) / kSimPatchGroup), njobs);
     const size_t lds = sizeof(double) * 4 * nb * patch * kSimPatchGroup;
     if (fast) {
         hipLaunchKernelGGL(k_ref_terms<true>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
