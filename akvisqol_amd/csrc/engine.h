@@ -123,6 +123,10 @@ struct PairState;  // engine_state.h
 // most 16 and the machine's hardware threads, 1 below 2·per pairs.
 int host_threads(size_t n, size_t per);
 
+// Samples a signal of len samples takes in upload staging: at least one,
+// rounded up to 256 bytes.
+size_t upload_span(long long len);
+
 struct BackendError {
     int code;
     std::string msg;
@@ -239,8 +243,10 @@ class Engine {
     // Host signals src[i] (len[i] samples each) to device memory dst[i]
     // through pinned staging, on this engine's copy stream; returns once
     // every sample is on the device.
+    // A destination laid out as upload_span of each length in turn from
+    // dst[0] gets one DMA per staged piece.
     void upload_signals(int n, const double* const* src, const long long* len, double* const* dst);
-    hipStream_t copy_stream_ = nullptr;  // created on the first upload_signals
+    hipStream_t copy_streams_[2] = {nullptr, nullptr};  // created on the first upload_signals
     void sync();
     void check(hipError_t e, const char* what);
     // One chunk of pairs (bounded HBM footprint).
