@@ -48,6 +48,25 @@ int invalid(const char* msg) {
     return VISQOL_E_INVALID_ARGUMENT;
 }
 
+int exception_code() {
+    try {
+        throw;
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("OutOfMemory");
+        return VISQOL_E_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        // std::system_error from thread creation, length errors, ...
+        set_error("%s", e.what());
+        return VISQOL_E_BACKEND;
+    } catch (...) {
+        set_error("unknown exception");
+        return VISQOL_E_INTERNAL;
+    }
+}
+
 // One allocation holding the visqol_result, its arrays and its patches
 // (capi.zig:65-70, 236-278).
 visqol_result* box(const PairResult& r) {
@@ -143,12 +162,8 @@ int run_pairs(visqol_ctx* c, const PairInput* pairs, size_t n, uint32_t sr, std:
     }
     try {
         c->engine->run(pairs, (int)n, sr, res.data(), false, grads);
-    } catch (const BackendError& e) {
-        set_error("%s", e.msg.c_str());
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_error("OutOfMemory");
-        return VISQOL_E_OUT_OF_MEMORY;
+    } catch (...) {
+        return exception_code();
     }
     return VISQOL_OK;
 }
@@ -238,9 +253,10 @@ const char* visqol_version(void) { return "0.1.0"; }
 
 const char* visqol_last_error(void) { return g_error; }
 
+// Both slots run on the device: VISQOL_BACKEND_CPU as the exact f64 path.
 uint32_t visqol_available_backends(void) {
     std::string why;
-    return device_ok(why) ? (1u << VISQOL_BACKEND_HIP) : 0u;
+    return device_ok(why) ? (1u << VISQOL_BACKEND_CPU) | (1u << VISQOL_BACKEND_HIP) : 0u;
 }
 
 void visqol_options_default(visqol_options* o) {
@@ -252,11 +268,11 @@ void visqol_options_default(visqol_options* o) {
     o->disable_realignment = 0;
     o->use_lattice_model = 1;
     o->model_path = nullptr;
-    o->backend = VISQOL_BACKEND_HIP;
+    o->backend = VISQOL_BACKEND_CPU;
     o->precision = VISQOL_PRECISION_AUTO;
 }
 
-int visqol_create(const visqol_options* in, visqol_ctx** out) {
+int visqol_create(const visqol_options* in, visqol_ctx** out) try {
     if (!out) return invalid("out is NULL");
     *out = nullptr;
     visqol_options o;
@@ -274,28 +290,28 @@ int visqol_create(const visqol_options* in, visqol_ctx** out) {
         set_error("invalid precision %u", o.precision);
         return VISQOL_E_INVALID_ARGUMENT;
     }
-    if (o.backend != VISQOL_BACKEND_HIP) {
-        set_error("backend %u is not provided by this library; it implements VISQOL_BACKEND_HIP (4) on gfx950",
+    if (o.backend != VISQOL_BACKEND_HIP && o.backend != VISQOL_BACKEND_CPU) {
+        set_error("backend %u is not provided by this library; it implements VISQOL_BACKEND_HIP (4) on gfx950 "
+                  "and serves VISQOL_BACKEND_CPU (0) with its exact f64 device path",
                   o.backend);
         return VISQOL_E_BACKEND;
     }
+    // The reference's CPU backend computes in f64 whatever the precision
+    // (backend.zig:163 reads it only for GPU prepared references); the exact
+    // device path gives that backend's scores bit for bit.
+    if (o.backend == VISQOL_BACKEND_CPU) o.precision = VISQOL_PRECISION_EXACT;
     std::string why;
     if (!device_ok(why)) {
         set_error("%s", why.c_str());
         return VISQOL_E_BACKEND;
     }
-    visqol_ctx* c = new (std::nothrow) visqol_ctx();
-    if (!c) {
-        set_error("OutOfMemory");
-        return VISQOL_E_OUT_OF_MEMORY;
-    }
+    std::unique_ptr<visqol_ctx> c(new visqol_ctx());
     const int rc = build_engine(o, c->opts, c->engine);
-    if (rc != VISQOL_OK) {
-        delete c;
-        return rc;
-    }
-    *out = c;
+    if (rc != VISQOL_OK) return rc;
+    *out = c.release();
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 void visqol_destroy(visqol_ctx* c) { delete c; }
@@ -303,7 +319,7 @@ void visqol_destroy(visqol_ctx* c) { delete c; }
 void visqol_result_free(visqol_result* r) { free(r); }
 
 int visqol_measure(visqol_ctx* c, const double* ref, size_t ref_len, const double* deg, size_t deg_len,
-                   uint32_t sample_rate, visqol_result** out) {
+                   uint32_t sample_rate, visqol_result** out) try {
     if (!c) return invalid("ctx is NULL");
     if (!out) return invalid("out is NULL");
     *out = nullptr;
@@ -313,11 +329,13 @@ int visqol_measure(visqol_ctx* c, const double* ref, size_t ref_len, const doubl
     const int rc = run_pairs(c, &p, 1, sample_rate, res);
     if (rc != VISQOL_OK) return rc;
     return finish_batch(res, out, nullptr);
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_measure_batch(visqol_ctx* c, size_t n, const double* const* refs, const size_t* ref_lens,
                          const double* const* degs, const size_t* deg_lens, uint32_t sample_rate, uint32_t num_threads,
-                         visqol_result** out, int32_t* status) {
+                         visqol_result** out, int32_t* status) try {
     (void)num_threads;  // the device packs the whole batch
     if (!c) return invalid("ctx is NULL");
     if (n == 0) return VISQOL_OK;
@@ -338,11 +356,13 @@ int visqol_measure_batch(visqol_ctx* c, size_t n, const double* const* refs, con
         return rc;
     }
     return finish_batch(res, out, status);
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_measure_batch_grad(visqol_ctx* c, size_t n, const double* const* refs, const size_t* ref_lens,
                                   const double* const* degs, const size_t* deg_lens, uint32_t sample_rate,
-                                  visqol_result** out, int32_t* status, double* const* grads) {
+                                  visqol_result** out, int32_t* status, double* const* grads) try {
     if (!c) return invalid("ctx is NULL");
     if (n == 0) return VISQOL_OK;
     if (!out || !refs || !ref_lens || !degs || !deg_lens || !grads) return invalid("NULL argument");
@@ -361,9 +381,11 @@ int visqol_hip_measure_batch_grad(visqol_ctx* c, size_t n, const double* const* 
         return rc;
     }
     return finish_batch(res, out, status);
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_measure_files(visqol_ctx* c, const char* ref_path, const char* deg_path, visqol_result** out) {
+int visqol_measure_files(visqol_ctx* c, const char* ref_path, const char* deg_path, visqol_result** out) try {
     if (!c) return invalid("ctx is NULL");
     if (!out) return invalid("out is NULL");
     *out = nullptr;
@@ -379,10 +401,12 @@ int visqol_measure_files(visqol_ctx* c, const char* ref_path, const char* deg_pa
         return VISQOL_E_SIGNAL;
     }
     return visqol_measure(c, ra.samples.data(), ra.samples.size(), da.samples.data(), da.samples.size(), ra.sample_rate, out);
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_decode_wav(const uint8_t* data, size_t len, double* samples, size_t cap, size_t* n_samples,
-                          uint32_t* sample_rate) {
+                          uint32_t* sample_rate) try {
     if (!data && len) return invalid("data is NULL");
     if (!n_samples) return invalid("n_samples is NULL");
     *n_samples = 0;
@@ -399,10 +423,12 @@ int visqol_hip_decode_wav(const uint8_t* data, size_t len, double* samples, size
         memcpy(samples, a.samples.data(), a.samples.size() * sizeof(double));
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_measure_files_batch(visqol_ctx* c, size_t n, const char* const* ref_paths, const char* const* deg_paths,
-                               uint32_t num_threads, visqol_result** out, int32_t* status) {
+                               uint32_t num_threads, visqol_result** out, int32_t* status) try {
     (void)num_threads;
     if (!c) return invalid("ctx is NULL");
     if (n == 0) return VISQOL_OK;
@@ -464,12 +490,14 @@ int visqol_measure_files_batch(visqol_ctx* c, size_t n, const char* const* ref_p
         if (codes[i] != VISQOL_OK && first_err == VISQOL_OK) first_err = codes[i];
     }
     return first_err;
+} catch (...) {
+    return exception_code();
 }
 
 // ---------------------------------------------------------------- extensions
 int visqol_hip_measure_trace(visqol_ctx* c, const double* ref, size_t ref_len, const double* deg, size_t deg_len,
                              uint32_t sample_rate, visqol_result** out, visqol_hip_trace* tr, double* ref_spec_out,
-                             double* deg_spec_out, double* sims_out) {
+                             double* deg_spec_out, double* sims_out) try {
     if (!c) return invalid("ctx is NULL");
     if (!out) return invalid("out is NULL");
     *out = nullptr;
@@ -505,11 +533,13 @@ int visqol_hip_measure_trace(visqol_ctx* c, const double* ref, size_t ref_len, c
     if (deg_spec_out && !r.deg_spec.empty()) memcpy(deg_spec_out, r.deg_spec.data(), sizeof(double) * r.deg_spec.size());
     if (sims_out && !r.sims.empty()) memcpy(sims_out, r.sims.data(), sizeof(double) * r.sims.size());
     return finish_batch(res, out, nullptr);
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_measure_batch_device(visqol_ctx* c, size_t n, const double* const* dev_refs, const size_t* ref_lens,
                                     const double* const* dev_degs, const size_t* deg_lens, uint32_t sample_rate,
-                                    visqol_result** out, int32_t* status) {
+                                    visqol_result** out, int32_t* status) try {
     if (!c) return invalid("ctx is NULL");
     if (n == 0) return VISQOL_OK;
     if (!out || !dev_refs || !ref_lens || !dev_degs || !deg_lens) return invalid("NULL argument");
@@ -526,10 +556,12 @@ int visqol_hip_measure_batch_device(visqol_ctx* c, size_t n, const double* const
         return rc;
     }
     return finish_batch(res, out, status);
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_prepare(visqol_ctx* c, size_t n, const double* const* refs, const size_t* ref_lens,
-                       uint32_t sample_rate, visqol_hip_prepared** out) {
+                       uint32_t sample_rate, visqol_hip_prepared** out) try {
     if (!c || !out) return invalid("NULL argument");
     *out = nullptr;
     if (n == 0 || !refs || !ref_lens) return invalid("no references");
@@ -556,17 +588,21 @@ int visqol_hip_prepare(visqol_ctx* c, size_t n, const double* const* refs, const
     }
     *out = p;
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_measure_prepared(visqol_ctx* c, const visqol_hip_prepared* prep, size_t n, const size_t* ref_index,
                                 const double* const* degs, const size_t* deg_lens, visqol_result** out,
-                                int32_t* status) {
+                                int32_t* status) try {
     return visqol_hip_measure_prepared_grad(c, prep, n, ref_index, degs, deg_lens, out, status, nullptr);
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_measure_prepared_grad(visqol_ctx* c, const visqol_hip_prepared* prep, size_t n,
                                      const size_t* ref_index, const double* const* degs, const size_t* deg_lens,
-                                     visqol_result** out, int32_t* status, double* const* grads) {
+                                     visqol_result** out, int32_t* status, double* const* grads) try {
     if (!c || !prep || !prep->set) return invalid("NULL argument");
     if (n == 0) return VISQOL_OK;
     if (!out || !degs || !deg_lens) return invalid("NULL argument");
@@ -590,18 +626,22 @@ int visqol_hip_measure_prepared_grad(visqol_ctx* c, const visqol_hip_prepared* p
         return rc;
     }
     return finish_batch(res, out, status);
+} catch (...) {
+    return exception_code();
 }
 
 void visqol_hip_prepared_free(visqol_hip_prepared* p) { delete p; }
 
-int visqol_hip_profile(visqol_ctx* c, int enable, int reset) {
+int visqol_hip_profile(visqol_ctx* c, int enable, int reset) try {
     if (!c) return invalid("ctx is NULL");
     c->engine->set_profiling(enable != 0);
     if (reset) c->engine->reset_profile();
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_profile_read(visqol_ctx* c, double* ms, double* flop, double* bytes, int64_t* launches) {
+int visqol_hip_profile_read(visqol_ctx* c, double* ms, double* flop, double* bytes, int64_t* launches) try {
     if (!c) return invalid("ctx is NULL");
     const Engine::Profile p = c->engine->profile();
     if (ms) *ms = p.gammatone_ms;
@@ -609,45 +649,55 @@ int visqol_hip_profile_read(visqol_ctx* c, double* ms, double* flop, double* byt
     if (bytes) *bytes = p.gammatone_bytes;
     if (launches) *launches = p.launches;
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_profile_sim(visqol_ctx* c, double* ms, double* cells, int64_t* launches) {
+int visqol_hip_profile_sim(visqol_ctx* c, double* ms, double* cells, int64_t* launches) try {
     if (!c) return invalid("ctx is NULL");
     const Engine::Profile p = c->engine->profile();
     if (ms) *ms = p.sim_ms;
     if (cells) *cells = p.sim_cells;
     if (launches) *launches = p.sim_launches;
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_profile_phases(visqol_ctx* c, double* ms, int n) {
+int visqol_hip_profile_phases(visqol_ctx* c, double* ms, int n) try {
     if (!c) return invalid("ctx is NULL");
     if (n > 0 && !ms) return invalid("ms is NULL");
     const Engine::Profile p = c->engine->profile();
     for (int i = 0; i < n && i < Engine::Profile::kPhases; i++) ms[i] = p.phase_ms[i];
     return Engine::Profile::kPhases;
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_patch_similarity_grad(const double* ref, const double* deg, size_t nrows, size_t ncols,
                                      const double* g_means, const double* g_stddevs, const double* g_energy,
-                                     double* g_ref, double* g_deg) {
+                                     double* g_ref, double* g_deg) try {
     if (!ref || !deg || !g_ref || !g_deg || nrows == 0 || ncols == 0) return invalid("invalid argument");
     memset(g_ref, 0, sizeof(double) * nrows * ncols);
     memset(g_deg, 0, sizeof(double) * nrows * ncols);
     patch_similarity_grad(ref, deg, (int)nrows, (int)ncols, g_means, g_stddevs, g_energy, g_ref, g_deg);
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_prepare_grad(const double* raw_ref, const double* raw_deg, size_t nb, size_t nfr, size_t nfd,
-                            const double* g_ref_prep, const double* g_deg_prep, double* g_raw_ref, double* g_raw_deg) {
+                            const double* g_ref_prep, const double* g_deg_prep, double* g_raw_ref, double* g_raw_deg) try {
     if (!raw_ref || !raw_deg || !g_ref_prep || !g_deg_prep || !g_raw_ref || !g_raw_deg || nb == 0)
         return invalid("invalid argument");
     prepare_grad(raw_ref, raw_deg, (int)nb, (int)nfr, (int)nfd, g_ref_prep, g_deg_prep, g_raw_ref, g_raw_deg);
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_lattice_predict(const uint8_t* model, size_t len, size_t nb, size_t count, const double* fvnsim,
-                               const double* fvnsim10, const double* fstdnsim, const double* fvdegenergy, double* y) {
+                               const double* fvnsim10, const double* fstdnsim, const double* fvdegenergy, double* y) try {
     if (!model || !y || nb == 0 || nb > kMaxBands || (count && (!fvnsim || !fvnsim10 || !fstdnsim || !fvdegenergy)))
         return invalid("invalid argument");
     std::string err;
@@ -662,21 +712,26 @@ int visqol_hip_lattice_predict(const uint8_t* model, size_t len, size_t nb, size
             return VISQOL_E_MODEL;
         }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_gammatone_shape(int shape, int* waves, int* bands_per_lane, int* chunk) {
+int visqol_hip_gammatone_shape(int shape, int* waves, int* bands_per_lane, int* chunk) try {
     if (shape < 0 || shape >= gammatone_shape_count()) return invalid("no such spectrogram block shape");
     const GtShape s = gammatone_shape(shape);
     if (waves) *waves = s.waves;
     if (bands_per_lane) *bands_per_lane = s.bands_per_lane;
     if (chunk) *chunk = s.chunk;
     return gammatone_shape_count();
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_gammatone_tuning(visqol_ctx* c, uint32_t sample_rate, int kind, int* chosen, float* ms, int cap) {
+int visqol_hip_gammatone_tuning(visqol_ctx* c, uint32_t sample_rate, int kind, int* chosen, float* ms, int cap) try {
     if (!c) return invalid("ctx is NULL");
     if (!chosen) return invalid("chosen is NULL");
     if (kind < 0 || kind > 1 || sample_rate == 0) return invalid("invalid argument");
+    if (cap > 0 && !ms) return invalid("ms is NULL with cap > 0");
     try {
         const Engine::GtTuning t = c->engine->gammatone_tuning(sample_rate, kind);
         *chosen = t.chosen;
@@ -686,9 +741,11 @@ int visqol_hip_gammatone_tuning(visqol_ctx* c, uint32_t sample_rate, int kind, i
         return e.code;
     }
     return gammatone_shape_count();
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_alloc(size_t bytes, void** p) {
+int visqol_hip_alloc(size_t bytes, void** p) try {
     if (!p) return invalid("NULL argument");
     if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
         (void)hipGetLastError();
@@ -696,27 +753,33 @@ int visqol_hip_alloc(size_t bytes, void** p) {
         return VISQOL_E_OUT_OF_MEMORY;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_free(void* p) {
+int visqol_hip_free(void* p) try {
     if (p && hipFree(p) != hipSuccess) {
         set_error("hipFree failed");
         return VISQOL_E_BACKEND;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_upload(void* dst, const void* src, size_t bytes) {
+int visqol_hip_upload(void* dst, const void* src, size_t bytes) try {
     if (!dst || (!src && bytes)) return invalid("NULL argument");
     if (hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
         set_error("hipMemcpy failed");
         return VISQOL_E_BACKEND;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_spectrogram(uint32_t mode, uint32_t sample_rate, uint32_t precision, const double* signal, size_t len,
-                           double* out, size_t* num_frames_out) {
+                           double* out, size_t* num_frames_out) try {
     if (!signal || !out || mode > 1 || sample_rate == 0) return invalid("invalid argument");
     std::string why;
     if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
@@ -733,10 +796,12 @@ int visqol_hip_spectrogram(uint32_t mode, uint32_t sample_rate, uint32_t precisi
         return e.code;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_frame_grad(uint32_t mode, uint32_t sample_rate, const double* frame, size_t window, const double* g,
-                          size_t lo, size_t hi, double* out) {
+                          size_t lo, size_t hi, double* out) try {
     if (!frame || !g || !out || mode > 1 || sample_rate == 0) return invalid("invalid argument");
     const RateParams p = make_rate_params((int)mode, sample_rate);
     if (window != (size_t)p.window) return invalid("window is not the sample rate's frame length");
@@ -751,17 +816,21 @@ int visqol_hip_frame_grad(uint32_t mode, uint32_t sample_rate, const double* fra
         return e.code;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_filters(uint32_t mode, uint32_t sample_rate, double* bands8, double* cf) {
+int visqol_hip_filters(uint32_t mode, uint32_t sample_rate, double* bands8, double* cf) try {
     if (mode > 1 || sample_rate == 0) return invalid("invalid argument");
     const RateParams p = make_rate_params((int)mode, sample_rate);
     if (bands8) memcpy(bands8, p.raw_band.data(), sizeof(double) * p.raw_band.size());
     if (cf) memcpy(cf, p.cf.data(), sizeof(double) * p.cf.size());
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_best_lag(const double* a, size_t la, const double* b, size_t lb, int64_t* lag) {
+int visqol_hip_best_lag(const double* a, size_t la, const double* b, size_t lb, int64_t* lag) try {
     if ((!a && la) || (!b && lb) || !lag) return invalid("invalid argument");
     std::string why;
     if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
@@ -773,9 +842,11 @@ int visqol_hip_best_lag(const double* a, size_t la, const double* b, size_t lb, 
         return e.code;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_xcorr(const double* a, size_t la, const double* b, size_t lb, double* out, size_t cap, size_t* n_out) {
+int visqol_hip_xcorr(const double* a, size_t la, const double* b, size_t lb, double* out, size_t cap, size_t* n_out) try {
     if ((!a && la) || (!b && lb) || (!out && cap) || !n_out) return invalid("invalid argument");
     if (std::max(la, lb) < 2) return invalid("xcorr needs a signal of at least 2 samples");
     std::string why;
@@ -791,9 +862,11 @@ int visqol_hip_xcorr(const double* a, size_t la, const double* b, size_t lb, dou
         return e.code;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_fft(double* z, size_t n, int dit) {
+int visqol_hip_fft(double* z, size_t n, int dit) try {
     if (!z || n == 0 || (n & (n - 1))) return invalid("n must be a power of two");
     std::string why;
     if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
@@ -805,9 +878,11 @@ int visqol_hip_fft(double* z, size_t n, int dit) {
         return e.code;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_math_probe(int op, const double* a, const double* b, double* out, size_t n) {
+int visqol_hip_math_probe(int op, const double* a, const double* b, double* out, size_t n) try {
     if (!a || !out || op < 0 || op > 2 || (op == 2 && !b)) return invalid("invalid argument");
     std::string why;
     if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
@@ -819,6 +894,8 @@ int visqol_hip_math_probe(int op, const double* a, const double* b, double* out,
         return e.code;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 }  // extern "C"

@@ -95,7 +95,10 @@ struct Request {
     uint32_t sr = 0;
     size_t ref_len = 0, deg_len = 0;
     size_t slab_off = 0, slab_bytes = 0, footprint = 0;
-    Clock::time_point t_submit, t_done;
+    // submit called, admitted (footprint and slab space granted), signals
+    // on the device and queued, batch started, batch finished
+    Clock::time_point t_submit, t_admit, t_queued, t_run, t_done;
+    uint32_t batch_size = 0;
     PairResult res;
     bool done = false;
 };
@@ -158,21 +161,25 @@ void visqol_hip_server::work() {
         }
         const int mode = batch.front()->mode;
         int rc = VISQOL_OK;
-        if (!eng[mode]) {
-            visqol_options o = opts;
-            o.mode = (uint32_t)mode;
-            rc = build_engine(o, eo[mode], eng[mode], false);
-        }
-        std::vector<PairInput> in(batch.size());
-        std::vector<PairResult> out(batch.size());
-        for (size_t i = 0; i < batch.size(); i++) {
-            const Request* r = batch[i];
-            const double* ref = (const double*)slab.at(r->slab_off);
-            in[i] = PairInput{ref, (long long)r->ref_len, ref + r->ref_len, (long long)r->deg_len, true};
-        }
         std::string err;
-        if (rc == VISQOL_OK) {
+        std::vector<PairResult> out;
+        {
             try {
+                if (!eng[mode]) {
+                    visqol_options o = opts;
+                    o.mode = (uint32_t)mode;
+                    rc = build_engine(o, eo[mode], eng[mode], false);
+                    if (rc != VISQOL_OK) throw BackendError{rc, "cannot build the scoring engine"};
+                }
+                std::vector<PairInput> in(batch.size());
+                out.resize(batch.size());
+                for (size_t i = 0; i < batch.size(); i++) {
+                    const Request* r = batch[i];
+                    const double* ref = (const double*)slab.at(r->slab_off);
+                    in[i] = PairInput{ref, (long long)r->ref_len, ref + r->ref_len, (long long)r->deg_len, true};
+                }
+                const Clock::time_point t_run = Clock::now();
+                for (Request* r : batch) r->t_run = t_run;
                 eng[mode]->run(in.data(), (int)in.size(), batch.front()->sr, out.data(), false);
             } catch (const BackendError& e) {
                 rc = e.code;
@@ -180,16 +187,20 @@ void visqol_hip_server::work() {
             } catch (const std::bad_alloc&) {
                 rc = VISQOL_E_OUT_OF_MEMORY;
                 err = "OutOfMemory";
+            } catch (const std::exception& e) {
+                // e.g. std::system_error from the engine's host threads: the
+                // batch fails, the worker and its waiters carry on
+                rc = VISQOL_E_BACKEND;
+                err = e.what();
             }
-        } else {
-            err = "cannot build the scoring engine";
         }
         const Clock::time_point now = Clock::now();
         {
             std::lock_guard<std::mutex> lk(mu);
             for (size_t i = 0; i < batch.size(); i++) {
                 Request* r = batch[i];
-                r->res = std::move(out[i]);
+                if (rc == VISQOL_OK) r->res = std::move(out[i]);
+                r->batch_size = (uint32_t)batch.size();
                 if (rc != VISQOL_OK) {
                     r->res = PairResult();
                     r->res.status = rc;
@@ -219,7 +230,7 @@ void visqol_hip_server_config_default(visqol_hip_server_config* c) {
 }
 
 int visqol_hip_server_create(const visqol_options* opts, const visqol_hip_server_config* cfg,
-                             visqol_hip_server** out) {
+                             visqol_hip_server** out) try {
     if (!out) return invalid("out is NULL");
     *out = nullptr;
     visqol_options o;
@@ -271,10 +282,12 @@ int visqol_hip_server_create(const visqol_options* opts, const visqol_hip_server
     for (uint32_t w = 0; w < c.workers; w++) s->workers.emplace_back([s] { s->work(); });
     *out = s;
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int visqol_hip_server_submit(visqol_hip_server* s, uint32_t mode, const double* ref, size_t ref_len,
-                             const double* deg, size_t deg_len, uint32_t sample_rate, uint64_t* ticket) {
+                             const double* deg, size_t deg_len, uint32_t sample_rate, uint64_t* ticket) try {
     const Clock::time_point t0 = Clock::now();
     if (!s || !ticket) return invalid("NULL argument");
     if (mode > 1 || sample_rate == 0) return invalid("invalid mode or sample rate");
@@ -304,6 +317,7 @@ int visqol_hip_server_submit(visqol_hip_server* s, uint32_t mode, const double* 
         s->inflight += rp->footprint;
         rp->id = s->next_id++;
     }
+    rp->t_admit = Clock::now();
     // Upload outside the lock; the request is queued once its signals are
     // on the device.
     (void)hipSetDevice(s->device);
@@ -323,6 +337,7 @@ int visqol_hip_server_submit(visqol_hip_server* s, uint32_t mode, const double* 
         set_error("upload failed: %s", hipGetErrorString(e));
         return VISQOL_E_BACKEND;
     }
+    rp->t_queued = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
         *ticket = rp->id;
@@ -331,9 +346,22 @@ int visqol_hip_server_submit(visqol_hip_server* s, uint32_t mode, const double* 
     }
     s->cv_work.notify_one();
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_server_wait(visqol_hip_server* s, uint64_t ticket, visqol_result** out, double* latency_ms) {
+int visqol_hip_server_wait(visqol_hip_server* s, uint64_t ticket, visqol_result** out, double* latency_ms) try {
+    visqol_hip_request_timing t;
+    const int rc = visqol_hip_server_wait_timed(s, ticket, out, &t);
+    if (latency_ms) *latency_ms = t.total_ms;
+    return rc;
+} catch (...) {
+    return exception_code();
+}
+
+int visqol_hip_server_wait_timed(visqol_hip_server* s, uint64_t ticket, visqol_result** out,
+                                 visqol_hip_request_timing* timing) try {
+    if (timing) memset(timing, 0, sizeof *timing);
     if (!s || !out) return invalid("NULL argument");
     *out = nullptr;
     std::unique_ptr<Request> r;
@@ -346,7 +374,19 @@ int visqol_hip_server_wait(visqol_hip_server* s, uint64_t ticket, visqol_result*
         r = std::move(it->second);
         s->all.erase(it);
     }
-    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (timing) {
+        auto ms = [](Clock::time_point a, Clock::time_point b) {
+            return std::chrono::duration<double, std::milli>(b - a).count();
+        };
+        timing->admit_ms = ms(r->t_submit, r->t_admit);
+        timing->upload_ms = ms(r->t_admit, r->t_queued);
+        // a batch that failed before it started has no start time
+        const Clock::time_point run = r->t_run < r->t_queued ? r->t_done : r->t_run;
+        timing->queue_ms = ms(r->t_queued, run);
+        timing->run_ms = ms(run, r->t_done);
+        timing->total_ms = ms(r->t_submit, r->t_done);
+        timing->batch_size = r->batch_size;
+    }
     if (r->res.status != VISQOL_OK) {
         set_error("%s", r->res.error.c_str());
         return r->res.status;
@@ -357,15 +397,19 @@ int visqol_hip_server_wait(visqol_hip_server* s, uint64_t ticket, visqol_result*
         return VISQOL_E_OUT_OF_MEMORY;
     }
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
-int visqol_hip_server_stats(visqol_hip_server* s, uint64_t* batches, uint64_t* pairs, uint64_t* inflight_bytes) {
+int visqol_hip_server_stats(visqol_hip_server* s, uint64_t* batches, uint64_t* pairs, uint64_t* inflight_bytes) try {
     if (!s) return invalid("NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     if (batches) *batches = s->batches;
     if (pairs) *pairs = s->batched_pairs;
     if (inflight_bytes) *inflight_bytes = s->inflight;
     return VISQOL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 void visqol_hip_server_destroy(visqol_hip_server* s) {

@@ -85,6 +85,17 @@ class ServerConfig(ctypes.Structure):
     ]
 
 
+class RequestTiming(ctypes.Structure):
+    _fields_ = [
+        ("admit_ms", c_double),
+        ("upload_ms", c_double),
+        ("queue_ms", c_double),
+        ("run_ms", c_double),
+        ("total_ms", c_double),
+        ("batch_size", c_uint32),
+    ]
+
+
 class Trace(ctypes.Structure):
     _fields_ = [
         ("lag", c_int64),
@@ -163,6 +174,8 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_server_submit": (c_int, [c_void_p, c_uint32, D, c_size_t, D, c_size_t, c_uint32,
                                              POINTER(ctypes.c_uint64)]),
         "visqol_hip_server_wait": (c_int, [c_void_p, ctypes.c_uint64, POINTER(POINTER(Result)), D]),
+        "visqol_hip_server_wait_timed": (c_int, [c_void_p, ctypes.c_uint64, POINTER(POINTER(Result)),
+                                                 POINTER(RequestTiming)]),
         "visqol_hip_server_stats": (c_int, [c_void_p, POINTER(ctypes.c_uint64), POINTER(ctypes.c_uint64),
                                             POINTER(ctypes.c_uint64)]),
         "visqol_hip_server_destroy": (None, [c_void_p]),
@@ -616,6 +629,20 @@ class Server:
             res = result_dict(out.contents) if convert else out.contents.moslqo
             lib.visqol_result_free(out)
         return rc, res, lat.value
+
+    def wait_timed(self, ticket: int, convert: bool = True):
+        """(status, result dict or MOS, timing dict): where the request's
+        time went (admission, upload, queue, batch run, total ms) and the
+        size of the batch it ran in."""
+        out = POINTER(Result)()
+        t = RequestTiming()
+        rc = lib.visqol_hip_server_wait_timed(self._h, ticket, ctypes.byref(out), ctypes.byref(t))
+        res = None
+        if out:
+            res = result_dict(out.contents) if convert else out.contents.moslqo
+            lib.visqol_result_free(out)
+        timing = {k: getattr(t, k) for k, _ in RequestTiming._fields_}
+        return rc, res, timing
 
     def stats(self) -> dict:
         b, p, f = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()

@@ -10,7 +10,11 @@
  * One additive value: VISQOL_BACKEND_HIP (4) names this library's backend.
  * The reference's four backends have no AMD slot, and reusing "cuda" would
  * make this a CUDA shim; ABI version 1 is kept because no existing value,
- * layout or signature changes.  visqol_options_default() selects it.
+ * layout or signature changes.  visqol_options_default() keeps the
+ * reference's default, VISQOL_BACKEND_CPU, which this library serves with
+ * its exact f64 device path: that path's scores are the reference CPU
+ * path's bit for bit, so callers that never name a backend (the reference's
+ * Python and C++ wrappers with VISQOL_BACKEND unset) run unchanged.
  *
  * Threading: a visqol_ctx is used by one thread at a time;
  * visqol_measure_batch packs its pairs onto the GPU itself.
@@ -54,7 +58,9 @@ enum {
 /* visqol.h:43.  On this library: EXACT = f64 with the reference CPU path's
  * operation order (bit-compatible scores); FAST and BALANCED = the
  * throughput mode (f64 with fused multiply-add and parallel reductions,
- * |dMOS| <= 1e-6); AUTO = EXACT. */
+ * lags nominated from an f32 cross-correlation and rescored in f64,
+ * |dMOS| <= 1e-6); AUTO = EXACT.  VISQOL_BACKEND_CPU always runs EXACT,
+ * as the reference's CPU backend computes in f64 whatever the precision. */
 enum {
     VISQOL_PRECISION_EXACT = 0,
     VISQOL_PRECISION_FAST = 1,
@@ -110,10 +116,11 @@ uint32_t visqol_abi_version(void);
 const char *visqol_version(void);
 /* visqol.h:88; capi.zig:119-122: thread-local, <= 511 bytes. */
 const char *visqol_last_error(void);
-/* visqol.h:90; capi.zig:132-134: 1 << VISQOL_BACKEND_HIP when a gfx950
- * device is present. */
+/* visqol.h:90; capi.zig:132-134: (1 << VISQOL_BACKEND_CPU) |
+ * (1 << VISQOL_BACKEND_HIP) when a gfx950 device is present (both run on
+ * it), else 0. */
 uint32_t visqol_available_backends(void);
-/* visqol.h:92; capi.zig:136-149 */
+/* visqol.h:92; capi.zig:136-149: the reference's defaults, backend CPU. */
 void visqol_options_default(visqol_options *opts);
 /* visqol.h:94; capi.zig:155-223 */
 int visqol_create(const visqol_options *opts, visqol_ctx **out);

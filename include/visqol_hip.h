@@ -128,6 +128,17 @@ int visqol_hip_server_create(const visqol_options *opts, const visqol_hip_server
 int visqol_hip_server_submit(visqol_hip_server *s, uint32_t mode, const double *ref, size_t ref_len,
                              const double *deg, size_t deg_len, uint32_t sample_rate, uint64_t *ticket);
 int visqol_hip_server_wait(visqol_hip_server *s, uint64_t ticket, visqol_result **out, double *latency_ms);
+/* visqol_hip_server_wait with where the request's time went, each on the
+ * steady clock in ms: waiting for admission (footprint budget and slab
+ * space), uploading its signals, queued until a worker took it into a
+ * batch, that batch's run, and the total (submit call to completion), plus
+ * the size of the batch it ran in. */
+typedef struct {
+    double admit_ms, upload_ms, queue_ms, run_ms, total_ms;
+    uint32_t batch_size;
+} visqol_hip_request_timing;
+int visqol_hip_server_wait_timed(visqol_hip_server *s, uint64_t ticket, visqol_result **out,
+                                 visqol_hip_request_timing *timing);
 /* Batches run, pairs scored, and the footprint now admitted. */
 int visqol_hip_server_stats(visqol_hip_server *s, uint64_t *batches, uint64_t *pairs, uint64_t *inflight_bytes);
 void visqol_hip_server_destroy(visqol_hip_server *s);

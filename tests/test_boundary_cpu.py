@@ -50,7 +50,9 @@ def test_abi_version_and_defaults():
     assert native.__version__ == "0.1.0"
     o = native.default_options()
     assert (o.mode, o.search_window_radius, o.use_lattice_model, o.precision) == (0, 60, 1, 3)
-    assert o.backend == native.BACKENDS["hip"]
+    # the reference's default (capi.zig:136-149): backend CPU, served here
+    # by the exact f64 device path
+    assert o.backend == native.BACKENDS["cpu"] == 0
 
 
 def test_header_constants():
@@ -100,9 +102,14 @@ def test_invalid_arguments_before_device():
     o = native.default_options()
     o.precision = 9
     assert L.visqol_create(ctypes.byref(o), ctypes.byref(h)) == native.E_INVALID_ARGUMENT
+    for backend in (1, 2, 3):  # Metal, CUDA, Vulkan: not provided here
+        o = native.default_options()
+        o.backend = backend
+        assert L.visqol_create(ctypes.byref(o), ctypes.byref(h)) == native.E_BACKEND
+        assert b"VISQOL_BACKEND_HIP" in L.visqol_last_error()
     o = native.default_options()
-    o.backend = 0  # the reference's CPU backend is not this library
-    assert L.visqol_create(ctypes.byref(o), ctypes.byref(h)) == native.E_BACKEND
+    o.backend = 5  # past the enum: an invalid argument, as capi.zig:171-174
+    assert L.visqol_create(ctypes.byref(o), ctypes.byref(h)) == native.E_INVALID_ARGUMENT
     assert L.visqol_create(ctypes.byref(o), None) == native.E_INVALID_ARGUMENT
     assert L.visqol_measure(None, None, 0, None, 0, 48000, None) == native.E_INVALID_ARGUMENT
     assert L.visqol_measure_batch(None, 0, None, None, None, None, 48000, 0, None, None) == native.E_INVALID_ARGUMENT

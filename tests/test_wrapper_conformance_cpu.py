@@ -128,3 +128,31 @@ def test_reference_header_program_links_and_runs(tmp_path):
     assert got["batch_null_ctx"] == "-1"  # ctx is checked before n == 0 (capi.zig:344)
     assert got["last_error"] == "ctx is NULL"
     assert got["have"] == "1 1 1"
+
+
+DEFAULT_FLOW = ROOT / "tests" / "native" / "default_flow.c"
+
+
+def test_default_flow_program_builds_against_reference_header(tmp_path):
+    """The wrappers' default flow (options default, backend CPU from an
+    unset VISQOL_BACKEND) builds against the reference header unchanged; with
+    no GPU here visqol_create fails loudly with E_BACKEND, as it would for any
+    backend the machine lacks.  The GPU test runs the same program and
+    compares its scores with the oracle bit for bit."""
+    import torch
+
+    exe = tmp_path / "default_flow"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{REF_INCLUDE}", str(DEFAULT_FLOW), "-o", str(exe),
+                    f"-L{LIB.parent}", "-lvisqol", f"-Wl,-rpath,{LIB.parent}"], check=True)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: the GPU test scores with this program")
+    import numpy as np
+
+    a = tmp_path / "a.f64"
+    np.zeros(48000).tofile(a)
+    env = {k: v for k, v in os.environ.items() if k != "VISQOL_BACKEND"}
+    out = subprocess.run([str(exe), "0", "0", "48000", str(a), str(a)], check=True, capture_output=True, text=True,
+                         env=env).stdout
+    got = dict(line.split(" ", 1) for line in out.strip().splitlines())
+    assert got["backends"] == "0"
+    assert got["create"] == "-6" and "no HIP device" in got["error"]
