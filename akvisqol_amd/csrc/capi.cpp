@@ -846,6 +846,45 @@ int visqol_hip_best_lag(const double* a, size_t la, const double* b, size_t lb, 
     return exception_code();
 }
 
+int visqol_hip_best_lag_fast(const double* a, size_t la, const double* b, size_t lb, int cap, int64_t* lag,
+                             int* nominated) try {
+    if ((!a && la) || (!b && lb) || !lag) return invalid("invalid argument");
+    if (cap > kMaxLagCands) return invalid("cap exceeds 64");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    Engine* e = stage_engine(0, true);
+    const int keep = e->options().lag_cap;
+    if (cap >= 0) e->set_lag_cap(cap);
+    bool nom = false;
+    try {
+        *lag = e->best_lag_fast(a, (long long)la, b, (long long)lb, &nom);
+    } catch (...) {
+        e->set_lag_cap(keep);
+        throw;
+    }
+    e->set_lag_cap(keep);
+    if (nominated) *nominated = nom ? 1 : 0;
+    return VISQOL_OK;
+} catch (...) {
+    return exception_code();
+}
+
+int visqol_hip_xcorr32(const double* a, size_t la, const double* b, size_t lb, float* out, size_t cap,
+                       size_t* n_out) try {
+    if ((!a && la) || (!b && lb) || (!out && cap) || !n_out) return invalid("invalid argument");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    std::vector<float> v;
+    stage_engine(0, true)->xcorr32(a, (long long)la, b, (long long)lb, v);
+    *n_out = v.size();
+    if (!v.empty()) memcpy(out, v.data(), sizeof(float) * std::min(cap, v.size()));
+    return VISQOL_OK;
+} catch (...) {
+    return exception_code();
+}
+
 int visqol_hip_xcorr(const double* a, size_t la, const double* b, size_t lb, double* out, size_t cap, size_t* n_out) try {
     if ((!a && la) || (!b && lb) || (!out && cap) || !n_out) return invalid("invalid argument");
     if (std::max(la, lb) < 2) return invalid("xcorr needs a signal of at least 2 samples");

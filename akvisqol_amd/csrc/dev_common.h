@@ -37,6 +37,18 @@ __device__ inline void store2(double2* p, long long i, double2 v) {
     w.y = v.y;
     ((__attribute__((address_space(1))) vq_d2*)p)[i] = w;
 }
+// float2 element i, one 8-byte global access.
+typedef float vq_f2 __attribute__((ext_vector_type(2)));
+__device__ inline float2 load2(const float2* p, long long i) {
+    const vq_f2 v = ((const __attribute__((address_space(1))) vq_f2*)p)[i];
+    return make_float2(v.x, v.y);
+}
+__device__ inline void store2(float2* p, long long i, float2 v) {
+    vq_f2 w;
+    w.x = v.x;
+    w.y = v.y;
+    ((__attribute__((address_space(1))) vq_f2*)p)[i] = w;
+}
 
 // 2 × 8 coefficients per band, precomputed on the host exactly as
 // gammatone.zig:268-278 derives them from Filters: s1n0, s1n1, s2n1, s3n1,

@@ -114,6 +114,11 @@ Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
                std::unique_ptr<LatticeModel, LatticeDeleter> lattice)
     : opts_(o), svr_(std::move(svr)), lattice_(lattice.release(), LatticeDeleter()) {
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    if (const char* e = getenv("VISQOL_HIP_LAG_CAP")) {
+        char* end = nullptr;
+        const long v = strtol(e, &end, 10);
+        if (end != e && *end == 0 && v >= 0 && v <= kMaxLagCands) opts_.lag_cap = (int)v;
+    }
     if (const char* e = getenv("VISQOL_HIP_GAMMATONE_SHAPE")) {
         char* end = nullptr;
         const long v = strtol(e, &end, 10);
@@ -155,6 +160,7 @@ Engine::~Engine() {
         (void)hipFree(kv.second->hann);
     }
     for (auto& kv : twiddles_) (void)hipFree(kv.second);
+    for (auto& kv : twiddles32_) (void)hipFree(kv.second);
     for (hipStream_t cs : copy_streams_)
         if (cs) (void)hipStreamDestroy(cs);
     if (stream_ && owns_stream_) (void)hipStreamDestroy(stream_);
@@ -317,6 +323,18 @@ const double2* Engine::twiddles(long long n) {
     return d;
 }
 
+const float2* Engine::twiddles32(long long n) {
+    auto it = twiddles32_.find(n);
+    if (it != twiddles32_.end()) return it->second;
+    const std::vector<double> tw = make_level_twiddles(n);
+    std::vector<float> t32(tw.begin(), tw.end());
+    float2* d = nullptr;
+    check(hipMalloc(&d, sizeof(float) * t32.size()), "hipMalloc twiddles");
+    check(hipMemcpy(d, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice), "upload twiddles");
+    twiddles32_[n] = d;
+    return d;
+}
+
 void Engine::lap(int phase) {
     const auto now = std::chrono::steady_clock::now();
     if (profiling_ && phase >= 0)
@@ -335,7 +353,9 @@ void Engine::sums(const std::vector<SumJob>& jobs, double* d_out, bool serial) {
 
 // Lag searches grouped by transform size; pos[i] is job i's slot in the
 // returned device array of lags.
-long long* Engine::lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos) {
+long long* Engine::lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos,
+                              int** d_fallback, bool exact) {
+    if (d_fallback) *d_fallback = nullptr;
     pos.assign(jobs.size(), 0);
     long long* d_lags = (long long*)arena_.alloc(sizeof(long long) * std::max<size_t>(1, jobs.size()));
     if (jobs.empty()) return d_lags;
@@ -352,16 +372,29 @@ long long* Engine::lag_search(const std::vector<LagJob>& jobs, const double* d_s
     size_t parts = 0;
     for (const LagJob& j : sorted) parts += 2 * (size_t)lag_partials_per_job(j.n);
     auto* d_parts = (unsigned long long*)arena_.alloc(sizeof(unsigned long long) * parts);
+    const bool nominate = opts_.fast && !exact && d_fallback;
+    int* d_fb = nullptr;
     size_t start = 0, poff = 0;
     while (start < sorted.size()) {
         size_t end = start;
         while (end < sorted.size() && sorted[end].n == sorted[start].n) end++;
         const long long n = sorted[start].n;
-        launch_lag_search(d_jobs + start, (int)(end - start), n, twiddles(n), d_sums, d_parts + poff, d_lags + start,
-                          stream_);
+        const int rows = (int)(end - start);
+        if (nominate && lag32_handles(n)) {
+            if (!d_fb) {
+                d_fb = (int*)arena_.alloc(sizeof(int) * sorted.size());
+                check(hipMemsetAsync(d_fb, 0, sizeof(int) * sorted.size(), stream_), "memset");
+            }
+            void* scratch = arena_.alloc(lag32_scratch_bytes(rows, n));
+            launch_lag_search32(d_jobs + start, rows, n, twiddles32(n), d_sums, scratch, opts_.lag_cap, d_lags + start,
+                                d_fb + start, stream_);
+        } else {
+            launch_lag_search(d_jobs + start, rows, n, twiddles(n), d_sums, d_parts + poff, d_lags + start, stream_);
+        }
         poff += 2 * (size_t)lag_partials_per_job(n) * (end - start);
         start = end;
     }
+    if (d_fallback) *d_fallback = d_fb;
     return d_lags;
 }
 
@@ -649,7 +682,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     long long* d_zero = (long long*)arena_.alloc(sizeof(long long));
     check(hipMemsetAsync(d_zero, 0, sizeof(long long), stream_), "memset");
     std::vector<int> lag_pos;
-    long long* d_lags = lag_search(lj, d_sums, lag_pos);
+    int* d_fb = nullptr;
+    long long* d_lags = lag_search(lj, d_sums, lag_pos, &d_fb);
     for (int i = 0; i < n; i++) {
         PairState& s = st[i];
         const long long* lp = s.lagjob >= 0 ? d_lags + lag_pos[s.lagjob] : d_zero;
@@ -682,17 +716,53 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     std::vector<long long> h_lags(lj.size());
     std::vector<double> h_sums(6 * n);
     std::vector<unsigned char> h_active(vad_bytes);
+    std::vector<int> h_fb(d_fb ? lj.size() : 0);
     {
         long long* pl = (long long*)pinned_.alloc(sizeof(long long) * (lj.size() + 1));
         double* ps = (double*)pinned_.alloc(sizeof(double) * 6 * n);
         unsigned char* pa = (unsigned char*)pinned_.alloc(vad_bytes + 1);
+        int* pf = (int*)pinned_.alloc(sizeof(int) * (h_fb.size() + 1));
         readback(pl, d_lags, sizeof(long long) * lj.size());
         readback(ps, d_sums, sizeof(double) * 6 * n);
         readback(pa, d_active, vad_bytes);
+        readback(pf, d_fb, sizeof(int) * h_fb.size());
         sync();
         memcpy(h_lags.data(), pl, sizeof(long long) * lj.size());
         memcpy(h_sums.data(), ps, sizeof(double) * 6 * n);
         memcpy(h_active.data(), pa, vad_bytes);
+        memcpy(h_fb.data(), pf, sizeof(int) * h_fb.size());
+    }
+    // Rows the nominating search could not settle (more near-maximal lags
+    // than the cap, xcorr.zig:292-307): the f64 search, then the SPL energy
+    // of those pairs' aligned degraded signals again.
+    if (!h_fb.empty()) {
+        std::vector<LagJob> redo;
+        std::vector<int> redo_pair;
+        for (int i = 0; i < n; i++)
+            if (st[i].lagjob >= 0 && h_fb[lag_pos[st[i].lagjob]]) {
+                redo.push_back(lj[st[i].lagjob]);
+                redo_pair.push_back(i);
+            }
+        if (!redo.empty()) {
+            std::vector<int> rpos;
+            long long* d_l2 = lag_search(redo, d_sums, rpos, nullptr, true);
+            std::vector<SumJob> j4;
+            for (size_t k = 0; k < redo.size(); k++) {
+                const PairState& s = st[redo_pair[k]];
+                j4.push_back(SumJob{s.ddeg, s.dl, kSumSqFrom, 6 * redo_pair[k] + 5, 0, 0, d_l2 + rpos[k], s.rl / 2});
+            }
+            sums(j4, d_sums, serial);
+            long long* pl = (long long*)pinned_.alloc(sizeof(long long) * redo.size());
+            double* ps = (double*)pinned_.alloc(sizeof(double) * 6 * n);
+            readback(pl, d_l2, sizeof(long long) * redo.size());
+            readback(ps, d_sums, sizeof(double) * 6 * n);
+            sync();
+            for (size_t k = 0; k < redo.size(); k++) {
+                const int i = redo_pair[k];
+                h_lags[lag_pos[st[i].lagjob]] = pl[rpos[k]];
+                h_sums[6 * i + 5] = ps[6 * i + 5];
+            }
+        }
     }
     lap(0);
 
@@ -1077,13 +1147,34 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             sums(a1, d_rs, serial);
             sums(a2, d_rs, serial);
             std::vector<int> rpos;
-            long long* d_rlags = lag_search(rlj, d_rs, rpos);
+            int* d_rfb = nullptr;
+            long long* d_rlags = lag_search(rlj, d_rs, rpos, &d_rfb);
             std::vector<long long> h_rl(rlj.size());
             long long* pl = (long long*)pinned_.alloc(sizeof(long long) * rlj.size());
+            int* pf = (int*)pinned_.alloc(sizeof(int) * (rlj.size() + 1));
             readback(pl, d_rlags, sizeof(long long) * rlj.size());
+            if (d_rfb) readback(pf, d_rfb, sizeof(int) * rlj.size());
             sync();
-            lap(5);
             for (size_t q = 0; q < rlj.size(); q++) h_rl[q] = pl[rpos[q]];
+            if (d_rfb) {
+                // rows the nominating search could not settle: the f64 search
+                std::vector<LagJob> redo;
+                std::vector<int> redo_row;
+                for (size_t q = 0; q < rlj.size(); q++)
+                    if (pf[rpos[q]]) {
+                        redo.push_back(rlj[q]);
+                        redo_row.push_back((int)q);
+                    }
+                if (!redo.empty()) {
+                    std::vector<int> p2;
+                    long long* d_l2 = lag_search(redo, d_rs, p2, nullptr, true);
+                    long long* pl2 = (long long*)pinned_.alloc(sizeof(long long) * redo.size());
+                    readback(pl2, d_l2, sizeof(long long) * redo.size());
+                    sync();
+                    for (size_t k = 0; k < redo.size(); k++) h_rl[redo_row[k]] = pl2[p2[k]];
+                }
+            }
+            lap(5);
 
             // Trim geometry, frame reuse, spectrograms of what is kept.
             std::vector<GtJob> rg;
@@ -1571,6 +1662,67 @@ void Engine::xcorr(const double* a, long long la, const double* b, long long lb,
     launch_lag_search(d_lj, 1, nfft, twiddles(nfft), d_s, nullptr, nullptr, stream_, true);
     double* p = (double*)pinned_.alloc(sizeof(double2) * nfft);
     readback(p, z, sizeof(double2) * nfft);
+    sync();
+    out.resize(nfft);
+    for (long long i = 0; i < nfft; i++) out[i] = p[2 * i];
+}
+
+// Both probes form the sums (Σx, Nyquist sum) with the throughput mode's
+// one-pass reduction, as the pipeline's phase A does.
+long long Engine::best_lag_fast(const double* a, long long la, const double* b, long long lb, bool* nominated) {
+    arena_.reset();
+    pinned_.reset();
+    if (nominated) *nominated = false;
+    if (std::max(la, lb) == 0) return 0;
+    double* da = (double*)arena_.alloc(sizeof(double) * std::max(1LL, la));
+    double* db = (double*)arena_.alloc(sizeof(double) * std::max(1LL, lb));
+    if (la) check(hipMemcpyAsync(da, a, sizeof(double) * la, hipMemcpyHostToDevice, stream_), "upload");
+    if (lb) check(hipMemcpyAsync(db, b, sizeof(double) * lb, hipMemcpyHostToDevice, stream_), "upload");
+    double* d_s = (double*)arena_.alloc(sizeof(double) * 4);
+    check(hipMemsetAsync(d_s, 0, sizeof(double) * 4, stream_), "memset");
+    sums({SumJob{da, la, kMoments, 0, 2, -1, nullptr, 0}, SumJob{db, lb, kMoments, 1, 3, -1, nullptr, 0}}, d_s, false);
+    const long long nfft = fft_size_for(std::max(la, lb));
+    double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
+    std::vector<LagJob> lj{LagJob{da, db, la, lb, z, nfft, 0, 1, 2, 3}};
+    std::vector<int> pos;
+    int* d_fb = nullptr;
+    long long* d_lag = lag_search(lj, d_s, pos, &d_fb);
+    long long* p = (long long*)pinned_.alloc(sizeof(long long));
+    int* pf = (int*)pinned_.alloc(sizeof(int));
+    *pf = 1;
+    readback(p, d_lag, sizeof(long long));
+    if (d_fb) readback(pf, d_fb, sizeof(int));
+    sync();
+    if (!*pf) {
+        if (nominated) *nominated = true;
+        return *p;
+    }
+    d_lag = lag_search(lj, d_s, pos, nullptr, true);
+    readback(p, d_lag, sizeof(long long));
+    sync();
+    return *p;
+}
+
+void Engine::xcorr32(const double* a, long long la, const double* b, long long lb, std::vector<float>& out) {
+    arena_.reset();
+    pinned_.reset();
+    out.clear();
+    const long long nfft = fft_size_for(std::max(la, lb));
+    if (std::max(la, lb) == 0 || !lag32_handles(nfft)) return;
+    double* da = (double*)arena_.alloc(sizeof(double) * std::max(1LL, la));
+    double* db = (double*)arena_.alloc(sizeof(double) * std::max(1LL, lb));
+    if (la) check(hipMemcpyAsync(da, a, sizeof(double) * la, hipMemcpyHostToDevice, stream_), "upload");
+    if (lb) check(hipMemcpyAsync(db, b, sizeof(double) * lb, hipMemcpyHostToDevice, stream_), "upload");
+    double* d_s = (double*)arena_.alloc(sizeof(double) * 4);
+    check(hipMemsetAsync(d_s, 0, sizeof(double) * 4, stream_), "memset");
+    sums({SumJob{da, la, kMoments, 0, 2, -1, nullptr, 0}, SumJob{db, lb, kMoments, 1, 3, -1, nullptr, 0}}, d_s, false);
+    double2* z = (double2*)arena_.alloc(sizeof(double2) * nfft);
+    std::vector<LagJob> lj{LagJob{da, db, la, lb, z, nfft, 0, 1, 2, 3}};
+    void* scratch = arena_.alloc(lag32_scratch_bytes(1, nfft));
+    launch_lag_search32(upload(lj), 1, nfft, twiddles32(nfft), d_s, scratch, opts_.lag_cap, nullptr, nullptr, stream_,
+                        true);
+    float* p = (float*)pinned_.alloc(sizeof(float2) * nfft);
+    readback(p, z, sizeof(float2) * nfft);
     sync();
     out.resize(nfft);
     for (long long i = 0; i < nfft; i++) out[i] = p[2 * i];

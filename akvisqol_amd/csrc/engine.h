@@ -30,6 +30,10 @@ struct EngineOptions {
     // every other chunk from its own host thread, so one chunk's host work
     // (planning, decoding, aggregation) overlaps the other's kernels.
     bool interleave_chunks = true;
+    // Throughput mode: rows of the lag search with more nominated lags than
+    // this (at most kMaxLagCands) take the f64 search instead; 0 sends every
+    // row there (VISQOL_HIP_LAG_CAP, read when an engine is created).
+    int lag_cap = kMaxLagCands;
 };
 
 // A reference prepared for scoring many degraded signals against it
@@ -146,6 +150,7 @@ class Engine {
     void run(const PairInput* pairs, int n, uint32_t sample_rate, PairResult* out, bool want_stages,
              double* const* grads = nullptr);
     const EngineOptions& options() const { return opts_; }
+    void set_lag_cap(int cap) { opts_.lag_cap = cap; }
 
     // References of one sample rate prepared for run() (PairInput::prep);
     // throws BackendError on device failure.
@@ -154,6 +159,14 @@ class Engine {
     // Stage access for checks.
     void spectrogram(uint32_t sample_rate, const double* host_sig, long long len, std::vector<double>& out, int& nf);
     long long best_lag(const double* a, long long la, const double* b, long long lb);
+    // The throughput mode's lag (nominated in f32, rescored in f64) with
+    // the engine's cap; *nominated is false when the row fell back to the
+    // f64 search (or its size is outside the nominating search's range).
+    long long best_lag_fast(const double* a, long long la, const double* b, long long lb, bool* nominated);
+    // The nominating search's f32 correlation of the two envelopes (Re,
+    // times n), fft_size_for(max(la, lb)) values; empty when that size is
+    // outside its range.
+    void xcorr32(const double* a, long long la, const double* b, long long lb, std::vector<float>& out);
     // Re of the circular cross-correlation of the two signals' envelopes
     // (the values best_lag maximises), fft_size_for(max(la, lb)) of them.
     void xcorr(const double* a, long long la, const double* b, long long lb, std::vector<double>& out);
@@ -235,6 +248,7 @@ class Engine {
     };
     DevRate& dev_rate(uint32_t sample_rate);
     const double2* twiddles(long long n);
+    const float2* twiddles32(long long n);  // the f64 table rounded to f32
     template <typename T>
     T* upload(const std::vector<T>& v);
     template <typename T>
@@ -263,8 +277,13 @@ class Engine {
     // Reductions of `jobs` into d_out (serial: the reference's order).
     void sums(const std::vector<SumJob>& jobs, double* d_out, bool serial);
     // FFT cross-correlation lags of `jobs` (envelope sums already in
-    // d_sums); pos[i] is job i's slot in the returned device lag array.
-    long long* lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos);
+    // d_sums); pos[i] is job i's slot in the returned device lag array.  In
+    // the throughput mode rows of a size the nominating search handles take
+    // it (unless `exact`), and *d_fallback (one int per slot; nullptr when
+    // no row took it) flags the rows it could not nominate, whose lags are
+    // left for lag_search(..., exact = true) on those rows.
+    long long* lag_search(const std::vector<LagJob>& jobs, const double* d_sums, std::vector<int>& pos,
+                          int** d_fallback = nullptr, bool exact = false);
     void finish(PairResult& r, const RateParams& rp);
     // Spectrograms of d_jobs with the tuned block shape (tuning on the
     // first large launch of this kind); kind as in gammatone_tuning.
@@ -279,6 +298,7 @@ class Engine {
     PinnedArena pinned_;
     std::map<uint32_t, std::unique_ptr<DevRate>> rates_;
     std::map<long long, double2*> twiddles_;
+    std::map<long long, float2*> twiddles32_;
 };
 
 }  // namespace vq

@@ -92,6 +92,24 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
 void launch_fft_dif(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, hipStream_t stream);
 void launch_fft_dit(const LagJob* d_jobs, int njobs, long long n, const double2* d_tw, hipStream_t stream);
 
+// Throughput-mode lag search (xcorr.zig:269-340 refineLag / rescore,
+// engine.zig:800-822): an f32 correlation of the same packed envelopes
+// nominates the lags within its error bound of the row's maximum, and f64
+// dot products of the f64 envelopes pick the first maximum among them.  A
+// row is nominated when at most `cap` (<= kMaxLagCands,
+// MAX_REFINE_CANDIDATES) lags qualify; lags[j] is then written and
+// fallback[j] = 0, else fallback[j] = 1 and lags[j] is left for the f64
+// search.  d_tw32: the f32 twiddle table (W_S^j at S/2 + j, rounded from
+// the f64 one); each job's z holds at least n float2.  keep_xcorr: leave
+// the f32 circular correlation (times n, as float2 with Re first) in each
+// job's z instead, for checks.
+constexpr int kMaxLagCands = 64;
+bool lag32_handles(long long n);  // n a power of two in [2^14, 2^30]
+size_t lag32_scratch_bytes(int njobs, long long n);
+void launch_lag_search32(const LagJob* d_jobs, int njobs, long long n, const float2* d_tw32, const double* d_sums,
+                         void* d_scratch, int cap, long long* d_lags, int* d_fallback, hipStream_t stream,
+                         bool keep_xcorr = false);
+
 // ---------------------------------------------------------------- signal prep
 // dst[i] = src-aligned sample i times factor: negative lag drops -lag
 // leading samples, positive lag prepends zeros (realign.zig:91-110).

@@ -15,20 +15,37 @@
 // the stage-by-stage transform bit for bit.  The first DIF pass builds its
 // input from the two signals (the envelope is never stored); the last DIT
 // pass keeps no output, only each workgroup's first maximum in scan order.
+#include <float.h>
+
 #include "dev_common.h"
 #include "kernels.h"
 
 namespace vq {
 
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// Complex arithmetic on double2 (the exact transform) and float2 (the
+// throughput mode's nominating transform).
+template <typename C> struct Cx;
+template <> struct Cx<double2> {
+    using R = double;
+    __device__ static double2 make(double x, double y) { return make_double2(x, y); }
+};
+template <> struct Cx<float2> {
+    using R = float;
+    __device__ static float2 make(float x, float y) { return make_float2(x, y); }
+};
+template <typename C>
+__device__ inline C cadd(C a, C b) { return Cx<C>::make(a.x + b.x, a.y + b.y); }
+template <typename C>
+__device__ inline C csub(C a, C b) { return Cx<C>::make(a.x - b.x, a.y - b.y); }
 // fft.zig:181-188: cmul and the conjugate form, in the reference's order.
-__device__ inline double2 cmul(double2 x, double2 w) {
-    return make_double2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+template <typename C>
+__device__ inline C cmul(C x, C w) {
+    return Cx<C>::make(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
 }
-__device__ inline double2 cmul_conj(double2 x, double2 w) {
-    const double wi = -w.y;
-    return make_double2(x.x * w.x - x.y * wi, x.x * wi + x.y * w.x);
+template <typename C>
+__device__ inline C cmul_conj(C x, C w) {
+    const typename Cx<C>::R wi = -w.y;
+    return Cx<C>::make(x.x * w.x - x.y * wi, x.x * wi + x.y * w.x);
 }
 
 __device__ inline long long env_fft_size(long long l) {
@@ -126,14 +143,16 @@ __device__ inline void tile_argmax(double2* sm, int count, GIDX gidx, long long 
 }
 
 // DIF butterfly (fft.zig:45-90): (a, b) -> (a + b, (a - b)·w).
-__device__ inline void bf_dif(double2& a, double2& b, double2 w) {
-    const double2 t = csub(a, b);
+template <typename C>
+__device__ inline void bf_dif(C& a, C& b, C w) {
+    const C t = csub(a, b);
     a = cadd(a, b);
     b = cmul(t, w);
 }
 // DIT butterfly with the conjugate twiddle (fft.zig:92-166).
-__device__ inline void bf_dit(double2& a, double2& b, double2 w) {
-    const double2 bw = cmul_conj(b, w);
+template <typename C>
+__device__ inline void bf_dit(C& a, C& b, C w) {
+    const C bw = cmul_conj(b, w);
     b = csub(a, bw);
     a = cadd(a, bw);
 }
@@ -245,8 +264,8 @@ struct TileStep {
 // one radix-2 stage.  Twiddles of the next step load while the current one
 // computes.  Twiddle of stage l (span S_l = 2^(lgq + K - l)) at position
 // j = p·q + r0 + rr: tw[S_l/2 + j].  Starts and ends with a barrier.
-template <int K, bool DIT, int THREADS>
-__device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, const double2* __restrict__ tw) {
+template <int K, bool DIT, int THREADS, typename C>
+__device__ inline void tile_stages(C* sm, int lgR, int lgq, long long r0, const C* __restrict__ tw) {
     const int lgGR = K + lgR;
     // Radix-4 group g (two per thread): residue rr, quarter index mq, block bb.
     // lgH = log2 of the smaller distance.  Element x_c sits at m0 + c·h.
@@ -259,15 +278,15 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
         i0 = (bb << lgGR) + (m0 << lgR) + rr;
     };
     // Twiddle of stage l at position p (< distance) for residue r0 + rr.
-    auto twi = [&](int l, int p, int rr) -> double2 {
+    auto twi = [&](int l, int p, int rr) -> C {
         const long long j = ((long long)p << lgq) + r0 + rr;
         return tw[(1LL << (lgq + K - l - 1)) + j];
     };
     constexpr int kSteps = (K + 1) / 2;
     // w[u][0..1]: the larger-distance stage at lo and lo + h; w[u][2]: the
     // smaller-distance stage at lo.  Radix-2 step: w[u][0] for butterfly u.
-    double2 w[4][3];
-    auto load_tw = [&](int st, double2 (&dst)[4][3]) {
+    C w[4][3];
+    auto load_tw = [&](int st, C (&dst)[4][3]) {
         const int li = 2 * st;
         if (li + 1 < K) {
             const int lgH = DIT ? li : K - 2 - li;  // smaller distance 2^lgH
@@ -297,10 +316,10 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
 #pragma unroll
     for (int st = 0; st < kSteps; st++) {
         const int li = 2 * st;
-        double2 wn[4][3];
+        C wn[4][3];
         if (li + 1 < K) {
             const int lgH = DIT ? li : K - 2 - li;
-            double2 x[2][4];
+            C x[2][4];
             int ib[2];
 #pragma unroll
             for (int u = 0; u < 2; u++) {
@@ -329,7 +348,7 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
         } else {
             const int l = TileStep<K, DIT>::stage(li);
             const int lgHm = K - 1 - l;
-            double2 x[4][2];
+            C x[4][2];
             int ia[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -360,17 +379,106 @@ __device__ inline void tile_stages(double2* sm, int lgR, int lgq, long long r0, 
 }
 
 // Full-tile pass (every tile holds TILE elements, eight per thread) with K
-// stages: load the tile, run tile_stages, store it (or, on the inverse
-// transform's last pass, keep only the tile's first maximum).
+// stages: load the tile, run tile_stages, then store it, keep its first
+// maximum of |Re| in scan order (the exact inverse transform's last pass),
+// or keep its lag candidates (the nominating transform's last pass).
+enum : int { kEpiStore = 0, kEpiArgmax = 1, kEpiCandidates = 2 };
 
-// At least two workgroups per CU (TILE / 1024 waves per SIMD): for
-// 4096-element tiles (8 waves each) at most 128 VGPRs.
-template <int K, bool DIT, int SRC, bool ARGMAX, int TILE>
-__global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TILE / 1024))) void k_fft_tile(const LagJob* __restrict__ jobs, int lgn, int lgS, int lgR, int lgBn,
-                                                       const double2* __restrict__ tw, const double* __restrict__ sums,
-                                                       unsigned long long* __restrict__ partials, int njobs) {
+// Lag candidates of one workgroup of the nominating transform's last pass:
+// the tile's largest in-range |Re|, how many in-range elements lie within
+// the row's slack of it, and the first kTileCands of them (scan position
+// and |Re|).  The rows' candidates are a subset of the union of these.
+constexpr int kTileCands = 14;
+struct TileCands {
+    float top;
+    int count;
+    int pos[kTileCands];
+    float val[kTileCands];
+};
+
+// Block-wide sum of one double per thread in a fixed order (the same
+// bits on every run): a shuffle tree per wave, then the waves in order.
+// red holds THREADS / 64 doubles; the sum is valid in thread 0.
+template <int THREADS>
+__device__ inline double block_sum(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < THREADS / 64; w++) r += red[w];
+    return r;
+}
+
+// The candidates epilogue (kEpiCandidates): in-range |Re| against the
+// tile's own top.  The row's floor (row top - slack) is at least this
+// tile's floor, so every row candidate in this tile is among those counted
+// here; their order in out->pos is arbitrary (the row merge sorts them).
+template <int THREADS, typename GIDX>
+__device__ inline void tile_candidates(const float2* sm, GIDX gidx, long long n, const LagJob& job, float slack,
+                                       TileCands* out) {
+    constexpr int kPer = 8;
+    const long long big = job.la > job.lb ? job.la : job.lb;
+    const long long max_lag = big - 1;
+    unsigned top_bits = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const int e = threadIdx.x + u * THREADS;
+        if (scan_order(gidx(e), n, max_lag) >= 0) top_bits = max(top_bits, __float_as_uint(fabsf(sm[e].x)));
+    }
+    __shared__ unsigned s_top;
+    __shared__ int s_count;
+    if (threadIdx.x == 0) {
+        s_top = 0;
+        s_count = 0;
+    }
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) top_bits = max(top_bits, (unsigned)__shfl_down((int)top_bits, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&s_top, top_bits);  // |Re| >= 0: bit order is value order
+    __syncthreads();
+    const float top = __uint_as_float(s_top);
+    const float floor = top - slack;
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const int e = threadIdx.x + u * THREADS;
+        const long long so = scan_order(gidx(e), n, max_lag);
+        const float v = fabsf(sm[e].x);
+        if (so < 0 || !(v >= floor)) continue;
+        const int slot = atomicAdd(&s_count, 1);
+        if (slot < kTileCands) {
+            out->pos[slot] = (int)so;
+            out->val[slot] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out->top = top;
+        out->count = s_count;
+    }
+}
+
+// Workgroups per CU: f64 tiles at least two (TILE / 1024 waves per SIMD;
+// 4096-element tiles of 8 waves at most 128 VGPRs); f32 tiles of 8192
+// elements (64 KiB, 16 waves) one, within 128 VGPRs: held to 64 VGPRs for
+// two per CU, the envelope and candidate passes spill 70-160 bytes per
+// lane and run 1.3-2.3x slower (measured on the 279 ten-second pairs).
+template <typename C, int TILE>
+struct TileWaves {
+    static constexpr int value = sizeof(C) == 16 ? TILE / 1024 : TILE / 2048;
+};
+
+// SRC == kSrcEnvelope builds the tile from the two signals' envelopes
+// (computed in f64; a float2 tile rounds them to f32 and adds the tile's
+// share of ‖env_a‖² + ‖env_b‖² to aux[job·nwg + wg], as doubles).
+// EPI == kEpiArgmax writes the (bits, position) pair of the tile's first
+// maximum to aux; kEpiCandidates writes a TileCands to aux against the
+// row's slack slack[job].
+template <int K, bool DIT, int SRC, int EPI, int TILE, typename C>
+__global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TileWaves<C, TILE>::value))) void k_fft_tile(
+    const LagJob* __restrict__ jobs, int lgn, int lgS, int lgR, int lgBn, const C* __restrict__ tw,
+    const double* __restrict__ sums, void* __restrict__ aux, const float* __restrict__ slack, int njobs) {
     constexpr int THREADS = TILE / 8;
-    __shared__ double2 sm[TILE];
+    __shared__ C sm[TILE];
     const int jb = (int)(blockIdx.x % (unsigned)njobs);
     const long long wg = blockIdx.x / (unsigned)njobs;
     const long long nwg = gridDim.x / (unsigned)njobs;
@@ -382,7 +490,7 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TILE /
     const long long blk0 = (wg >> lgPer) << lgBn;
     const long long r0 = (wg & ((1LL << lgPer) - 1)) << lgR;
     const long long base = (blk0 << lgS) + r0;
-    double2* z = job.z;
+    C* z = reinterpret_cast<C*>(job.z);
     auto gidx = [&](int e) -> long long {
         const int bb = e >> lgGR;
         const int m = (e >> lgR) & ((1 << K) - 1);
@@ -394,21 +502,35 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TILE /
         // both signals' samples for the tile, loads first, envelope after
         const EnvSide ea = env_side(job.a, job.la, sums[job.sum_a], sums[job.nyq_a]);
         const EnvSide eb = env_side(job.b, job.lb, sums[job.sum_b], sums[job.nyq_b]);
-        double xa[kPer], xb[kPer];
+        // f64 tiles load all eight pairs first; f32 tiles (16 waves per
+        // workgroup, two workgroups per CU) four at a time, within 64 VGPRs
+        constexpr int kGroup = sizeof(C) == 16 ? kPer : kPer / 2;
+        double energy = 0.0;
 #pragma unroll
-        for (int u = 0; u < kPer; u++) {
-            const long long i = gidx(threadIdx.x + u * THREADS);
-            xa[u] = env_load(ea, i);
-            xb[u] = env_load(eb, i);
+        for (int g = 0; g < kPer; g += kGroup) {
+            double xa[kGroup], xb[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) {
+                const long long i = gidx(threadIdx.x + (g + u) * THREADS);
+                xa[u] = env_load(ea, i);
+                xb[u] = env_load(eb, i);
+            }
+#pragma unroll
+            for (int u = 0; u < kGroup; u++) {
+                const long long i = gidx(threadIdx.x + (g + u) * THREADS);
+                const double va = i < ea.len ? env_value(ea, xa[u], i) : 0.0;
+                const double vb = i < eb.len ? env_value(eb, xb[u], i) : 0.0;
+                sm[threadIdx.x + (g + u) * THREADS] = Cx<C>::make((typename Cx<C>::R)va, (typename Cx<C>::R)vb);
+                if constexpr (sizeof(C) == 8) energy = __builtin_fma(va, va, __builtin_fma(vb, vb, energy));
+            }
         }
-#pragma unroll
-        for (int u = 0; u < kPer; u++) {
-            const long long i = gidx(threadIdx.x + u * THREADS);
-            sm[threadIdx.x + u * THREADS] = make_double2(i < ea.len ? env_value(ea, xa[u], i) : 0.0,
-                                                          i < eb.len ? env_value(eb, xb[u], i) : 0.0);
+        if constexpr (sizeof(C) == 8) {
+            __shared__ double red[THREADS / 64];
+            const double e = block_sum<THREADS>(energy, red);
+            if (threadIdx.x == 0) static_cast<double*>(aux)[jb * nwg + wg] = e;
         }
     } else {
-        double2 v[kPer];
+        C v[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; u++) v[u] = load2(z, gidx(threadIdx.x + u * THREADS));
 #pragma unroll
@@ -416,18 +538,29 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TILE /
     }
 
     tile_stages<K, DIT, THREADS>(sm, lgR, lgq, r0, tw);
-    if (!ARGMAX) {
+    if constexpr (EPI == kEpiStore) {
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
             const int e = threadIdx.x + u * THREADS;
             store2(z, gidx(e), sm[e]);
         }
         return;
+    } else if constexpr (EPI == kEpiArgmax) {
+        tile_argmax<THREADS>(sm, TILE, gidx, n, job, static_cast<unsigned long long*>(aux) + 2 * (jb * nwg + wg));
+        return;
+    } else {
+        tile_candidates<THREADS>(sm, gidx, n, job, slack[jb], static_cast<TileCands*>(aux) + (jb * nwg + wg));
     }
-    tile_argmax<THREADS>(sm, TILE, gidx, n, job, partials + 2 * (jb * nwg + wg));
 }
 
 // xcorr.zig:211-229 cross terms, in place in bit-reversed order.
+__device__ inline float2 cross_term(float2 x, float2 zy) {
+    const float2 y = make_float2(zy.x, -zy.y);
+    const float2 sum = make_float2(x.x + y.x, x.y + y.y);
+    const float2 dif_c = make_float2(x.x - y.x, -(x.y - y.y));
+    const float2 idc = make_float2(-dif_c.y, dif_c.x);
+    return make_float2(0.25f * (sum.x * idc.x - sum.y * idc.y), 0.25f * (sum.x * idc.y + sum.y * idc.x));
+}
 __device__ inline double2 cross_term(double2 x, double2 zy) {
     const double2 y = make_double2(zy.x, -zy.y);
     const double2 sum = make_double2(x.x + y.x, x.y + y.y);
@@ -472,11 +605,11 @@ __global__ __launch_bounds__(256) void k_cross_terms(const LagJob* __restrict__ 
 // (d << lgH) + lo + ((c mod d) << lgH).  DIF runs the stages largest
 // distance first, DIT smallest first: the radix-2 butterflies of the
 // stage-by-stage transform.
-template <int B, bool DIT>
-__device__ inline void radix_step(double2 (&x)[8], const int (&lo)[8 >> B], int lgH, const double2* __restrict__ tw) {
+template <int B, bool DIT, typename C>
+__device__ inline void radix_step(C (&x)[8], const int (&lo)[8 >> B], int lgH, const C* __restrict__ tw) {
     constexpr int G = 1 << B;
     constexpr int NG = 8 >> B;
-    double2 w[NG][G];  // w[gi][d + t]: stage distance d, position t (t < d)
+    C w[NG][G];  // w[gi][d + t]: stage distance d, position t (t < d)
 #pragma unroll
     for (int gi = 0; gi < NG; gi++)
 #pragma unroll
@@ -494,8 +627,8 @@ __device__ inline void radix_step(double2 (&x)[8], const int (&lo)[8 >> B], int 
 #pragma unroll
             for (int c = 0; c < G; c++) {
                 if (c & d) continue;
-                double2& a = x[gi * G + c];
-                double2& b = x[gi * G + c + d];
+                C& a = x[gi * G + c];
+                C& b = x[gi * G + c + d];
                 if (DIT) bf_dit(a, b, w[gi][d + (c & (d - 1))]);
                 else bf_dif(a, b, w[gi][d + (c & (d - 1))]);
             }
@@ -551,17 +684,17 @@ struct MidPlan {
 // Four waves per SIMD (at most 128 VGPRs): 2048-element tiles then fit four
 // workgroups per CU instead of three (a few spilled registers cost less
 // than the lost occupancy); 4096-element tiles are held to two by LDS.
-template <int K, int TILE>
-__global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_mid(const LagJob* __restrict__ jobs, const double2* __restrict__ tw,
+template <int K, int TILE, typename C>
+__global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_mid(const LagJob* __restrict__ jobs, const C* __restrict__ tw,
                                                       int njobs) {
     constexpr int THREADS = TILE / 8;
     constexpr int kHalfTile = TILE / 2;
     static_assert((1 << K) <= kHalfTile, "blocks of 2^K elements must fit a chunk");
-    __shared__ double2 sm[TILE];
+    __shared__ C sm[TILE];
     using P = MidPlan<K>;
     const int jb = (int)(blockIdx.x % (unsigned)njobs);
     const long long wg = blockIdx.x / (unsigned)njobs;
-    double2* z = jobs[jb].z;
+    C* z = reinterpret_cast<C*>(jobs[jb].z);
     long long ca = 0, cb = 1;
     if (wg > 0) {
         const int k = 63 - __builtin_clzll((unsigned long long)wg);
@@ -579,7 +712,7 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
     // stride 8 (the central step and its mirror reads) — then falls on
     // distinct banks.
     auto swz = [](int e) { return e ^ (((e >> 3) & 7) | ((((e >> 5) ^ (e >> 6)) & 1) << 3)); };
-    double2 x[8];
+    C x[8];
     int idx[8];
     // LDS hand-over between steps: store at the current layout, barrier,
     // load at the next.
@@ -630,7 +763,7 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
         radix_step<3, false>(x, lo, 0, tw);
         put();
         __syncthreads();
-        double2 mirror[8];
+        C mirror[8];
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const long long g = gidx(idx[c]);
@@ -768,8 +901,9 @@ PassShape pass_shape(long long n, long long S_hi, int k, int tile) {
 template <int K, bool DIT, int SRC, bool ARGMAX, int TILE>
 void tile_pass(const PassShape& s, const LagJob* d_jobs, int njobs, long long n, long long S_hi, const double2* tw,
                const double* sums, unsigned long long* partials, hipStream_t stream) {
-    hipLaunchKernelGGL((k_fft_tile<K, DIT, SRC, ARGMAX, TILE>), dim3((unsigned)(s.wgs * njobs)), dim3(TILE / 8), 0,
-                       stream, d_jobs, log2i(n), log2i(S_hi), s.lgR, s.lgBn, tw, sums, partials, njobs);
+    hipLaunchKernelGGL((k_fft_tile<K, DIT, SRC, ARGMAX ? kEpiArgmax : kEpiStore, TILE, double2>),
+                       dim3((unsigned)(s.wgs * njobs)), dim3(TILE / 8), 0, stream, d_jobs, log2i(n), log2i(S_hi), s.lgR,
+                       s.lgBn, tw, sums, (void*)partials, (const float*)nullptr, njobs);
 }
 
 template <bool DIT, int SRC, bool ARGMAX>
@@ -831,12 +965,12 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
     if (fuse) {
         const dim3 g((unsigned)((n / kTile) * njobs));
         switch (ks[np - 1]) {
-            case 7: hipLaunchKernelGGL((k_fft_mid<7, kTile>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            case 8: hipLaunchKernelGGL((k_fft_mid<8, kTile>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            case 9: hipLaunchKernelGGL((k_fft_mid<9, kTile>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            case 10: hipLaunchKernelGGL((k_fft_mid<10, kTile>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 7: hipLaunchKernelGGL((k_fft_mid<7, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 8: hipLaunchKernelGGL((k_fft_mid<8, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 9: hipLaunchKernelGGL((k_fft_mid<9, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 10: hipLaunchKernelGGL((k_fft_mid<10, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
             default:
-                hipLaunchKernelGGL((k_fft_mid<11, kBigTile>), dim3((unsigned)((n / kBigTile) * njobs)),
+                hipLaunchKernelGGL((k_fft_mid<11, kBigTile, double2>), dim3((unsigned)((n / kBigTile) * njobs)),
                                    dim3(kBigTile / 8), 0, stream, d_jobs, d_tw, njobs);
                 break;
         }
@@ -875,6 +1009,301 @@ void launch_fft_dit(const LagJob* d_jobs, int njobs, long long n, const double2*
         S <<= ks[p];
         pass<true, kSrcZ, false>(d_jobs, njobs, n, S, ks[p], d_tw, nullptr, nullptr, stream);
     }
+}
+
+// ---------------------------------------------------------------- nominating (f32) lag search
+// The throughput mode's lag search (xcorr.zig:269-340 refineLag and rescore,
+// engine.zig:800-822): the same packed-envelope correlation computed in f32
+// nominates every lag whose |Re| lies within the f32 error bound of the
+// row's maximum; those are rescored by f64 dot products of the f64
+// envelopes and the first maximum in scan order wins.  The transform is not
+// bound to the exact path's operation order, so its passes use 8192-element
+// f32 tiles: one strided DIF pass (two past 2^21 points) from the signals,
+// the fused middle on 4096-element blocks, and the mirrored DIT passes, the
+// last one keeping only each tile's candidates.
+namespace {
+constexpr int kTile32 = 8192;   // float2 per workgroup (64 KiB LDS, 1024 threads)
+constexpr int kMid32 = 12;      // stages of the fused middle (blocks of 4096)
+constexpr int kMaxStrided32 = 9;
+constexpr int kRescoreThreads = 256;
+constexpr int kRescorePer = 16;
+constexpr long long kRescoreSeg = (long long)kRescoreThreads * kRescorePer;  // t values per rescoring block
+
+// Strided stage counts (over the largest spans first) of an n-point
+// nominating transform; returns the number of strided passes (1 or 2), or
+// 0 when n is outside the range this path handles.
+int plan32(long long n, int* ks) {
+    const int L = log2i(n);
+    if ((1LL << L) != n || L < kMid32 + 2 || L > kMid32 + 2 * kMaxStrided32) return 0;
+    const int T = L - kMid32;
+    if (T <= kMaxStrided32) {
+        ks[0] = T;
+        return 1;
+    }
+    ks[0] = (T + 1) / 2;
+    ks[1] = T - ks[0];
+    return 2;
+}
+
+template <int K, bool DIT, int SRC, int EPI>
+void tile32(const LagJob* d_jobs, int njobs, long long n, long long S_hi, const float2* tw, const double* sums, void* aux,
+            const float* slack, hipStream_t stream) {
+    const PassShape s = pass_shape(n, S_hi, K, kTile32);
+    hipLaunchKernelGGL((k_fft_tile<K, DIT, SRC, EPI, kTile32, float2>), dim3((unsigned)(s.wgs * njobs)),
+                       dim3(kTile32 / 8), 0, stream, d_jobs, log2i(n), log2i(S_hi), s.lgR, s.lgBn, tw, sums, aux, slack,
+                       njobs);
+}
+
+template <bool DIT, int SRC, int EPI>
+void pass32(int k, const LagJob* d_jobs, int njobs, long long n, long long S_hi, const float2* tw, const double* sums,
+            void* aux, const float* slack, hipStream_t stream) {
+    switch (k) {
+        case 2: tile32<2, DIT, SRC, EPI>(d_jobs, njobs, n, S_hi, tw, sums, aux, slack, stream); return;
+        case 3: tile32<3, DIT, SRC, EPI>(d_jobs, njobs, n, S_hi, tw, sums, aux, slack, stream); return;
+        case 4: tile32<4, DIT, SRC, EPI>(d_jobs, njobs, n, S_hi, tw, sums, aux, slack, stream); return;
+        case 5: tile32<5, DIT, SRC, EPI>(d_jobs, njobs, n, S_hi, tw, sums, aux, slack, stream); return;
+        case 6: tile32<6, DIT, SRC, EPI>(d_jobs, njobs, n, S_hi, tw, sums, aux, slack, stream); return;
+        case 7: tile32<7, DIT, SRC, EPI>(d_jobs, njobs, n, S_hi, tw, sums, aux, slack, stream); return;
+        case 8: tile32<8, DIT, SRC, EPI>(d_jobs, njobs, n, S_hi, tw, sums, aux, slack, stream); return;
+        default: tile32<9, DIT, SRC, EPI>(d_jobs, njobs, n, S_hi, tw, sums, aux, slack, stream); return;
+    }
+}
+
+// Scratch of one launch, carved in this order (each part 256-byte aligned).
+struct Lag32Layout {
+    long long wg_first, wg_last, nseg;
+    size_t energy, slack, tiles, count, cands, dots, total;
+};
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+Lag32Layout lag32_layout(int njobs, long long n) {
+    int ks[2];
+    const int np = plan32(n, ks);
+    Lag32Layout l{};
+    if (np == 0) return l;
+    l.wg_first = pass_shape(n, n, ks[0], kTile32).wgs;
+    l.wg_last = l.wg_first;
+    l.nseg = (n / 2 + kRescoreSeg - 1) / kRescoreSeg;
+    size_t at = 0;
+    l.energy = at;
+    at += align256(sizeof(double) * njobs * l.wg_first);
+    l.slack = at;
+    at += align256(sizeof(float) * njobs);
+    l.tiles = at;
+    at += align256(sizeof(TileCands) * njobs * l.wg_last);
+    l.count = at;
+    at += align256(sizeof(int) * njobs);
+    l.cands = at;
+    at += align256(sizeof(long long) * njobs * kMaxLagCands);
+    l.dots = at;
+    at += align256(sizeof(double) * njobs * kMaxLagCands * l.nseg);
+    l.total = at;
+    return l;
+}
+}  // namespace
+
+// Per row: the f32 slack 2·tol·(‖env_a‖² + ‖env_b‖²)·n with tol =
+// 8·eps_f32·log2 n (engine.zig:819-822 lagTolerance, xcorr.zig:320-326
+// refineSlack; the correlation here carries the factor n of the unscaled
+// inverse transform), the energy summed from the first pass's tile
+// partials in tile order.
+__global__ __launch_bounds__(64) void k_lag32_slack(const double* __restrict__ energy, int nparts, int lgn,
+                                                    float* __restrict__ slack) {
+    if (threadIdx.x != 0) return;
+    const double* e = energy + (long long)blockIdx.x * nparts;
+    double acc = 0.0;
+    for (int p = 0; p < nparts; p++) acc += e[p];
+    const double tol = 8.0 * (double)FLT_EPSILON * (double)lgn;
+    slack[blockIdx.x] = (float)(2.0 * tol * acc * (double)(1LL << lgn));
+}
+
+// Per row: the row's top (the largest tile top), floor = top - slack in
+// f32, and every in-range lag at or above it, gathered from the tiles whose
+// own top reaches the floor and sorted into scan order.  count[row] = -1
+// when the row is not nominated: a tile that reaches the floor overflowed
+// its list, more than cap lags qualify, or the top is not finite.
+__global__ __launch_bounds__(256) void k_lag32_select(const LagJob* __restrict__ jobs, const TileCands* __restrict__ tiles,
+                                                      int ntiles, const float* __restrict__ slack, int cap,
+                                                      int* __restrict__ count, long long* __restrict__ cands) {
+    const int row = blockIdx.x;
+    const TileCands* t = tiles + (long long)row * ntiles;
+    __shared__ unsigned s_top;
+    __shared__ int s_n, s_bad;
+    __shared__ int s_pos[kMaxLagCands];
+    if (threadIdx.x == 0) {
+        s_top = 0;
+        s_n = 0;
+        s_bad = 0;
+    }
+    __syncthreads();
+    unsigned top = 0;
+    for (int w = threadIdx.x; w < ntiles; w += blockDim.x) top = max(top, __float_as_uint(t[w].top));
+    atomicMax(&s_top, top);
+    __syncthreads();
+    const float row_top = __uint_as_float(s_top);
+    const float floor = row_top - slack[row];
+    if (!isfinite(row_top) || !isfinite(floor)) {
+        if (threadIdx.x == 0) count[row] = -1;
+        return;
+    }
+    for (int w = threadIdx.x; w < ntiles; w += blockDim.x) {
+        if (!(t[w].top >= floor)) continue;
+        if (t[w].count > kTileCands) {
+            atomicExch(&s_bad, 1);
+            continue;
+        }
+        for (int c = 0; c < t[w].count; c++) {
+            if (!(t[w].val[c] >= floor)) continue;
+            const int slot = atomicAdd(&s_n, 1);
+            if (slot < kMaxLagCands) s_pos[slot] = t[w].pos[c];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int nc = s_n;
+    if (s_bad || nc > cap || nc == 0) {
+        count[row] = -1;
+        return;
+    }
+    // insertion sort into scan order (at most kMaxLagCands entries)
+    for (int i = 1; i < nc; i++) {
+        const int v = s_pos[i];
+        int j = i - 1;
+        while (j >= 0 && s_pos[j] > v) {
+            s_pos[j + 1] = s_pos[j];
+            j--;
+        }
+        s_pos[j + 1] = v;
+    }
+    const LagJob j = jobs[row];
+    const long long max_lag = (j.la > j.lb ? j.la : j.lb) - 1;
+    for (int i = 0; i < nc; i++) cands[(long long)row * kMaxLagCands + i] = (long long)s_pos[i] - max_lag;
+    count[row] = nc;
+}
+
+// Partial f64 dot products Σ_t env_a[t + lag]·env_b[t] (xcorr.zig:328-340
+// dotAtLag) of every candidate of a row over one segment of t, with the
+// envelopes formed in f64 from the signals as the transform's first pass
+// forms them.  Fixed segments, per-thread order and reduction tree: the
+// same bits for a row whatever the batch.
+__global__ __launch_bounds__(kRescoreThreads) void k_lag32_rescore(const LagJob* __restrict__ jobs,
+                                                                   const double* __restrict__ sums,
+                                                                   const int* __restrict__ count,
+                                                                   const long long* __restrict__ cands, int nseg,
+                                                                   double* __restrict__ dots) {
+    const int row = blockIdx.y;
+    const int nc = count[row];
+    if (nc <= 1) return;  // no row, or a lone nominee (k_lag32_pick takes it as is)
+    const LagJob j = jobs[row];
+    const long long t0 = (long long)blockIdx.x * kRescoreSeg;
+    if (t0 >= j.lb) {
+        if (threadIdx.x < nc) dots[((long long)row * kMaxLagCands + threadIdx.x) * nseg + blockIdx.x] = 0.0;
+        return;
+    }
+    const EnvSide ea = env_side(j.a, j.la, sums[j.sum_a], sums[j.nyq_a]);
+    const EnvSide eb = env_side(j.b, j.lb, sums[j.sum_b], sums[j.nyq_b]);
+    double vb[kRescorePer];
+#pragma unroll
+    for (int u = 0; u < kRescorePer; u++) {
+        const long long t = t0 + u * kRescoreThreads + threadIdx.x;
+        vb[u] = t < eb.len ? env_value(eb, env_load(eb, t), t) : 0.0;
+    }
+    __shared__ double red[kRescoreThreads / 64];
+    for (int c = 0; c < nc; c++) {
+        const long long lag = cands[(long long)row * kMaxLagCands + c];
+        double acc = 0.0;
+#pragma unroll
+        for (int u = 0; u < kRescorePer; u++) {
+            const long long t = t0 + u * kRescoreThreads + threadIdx.x;
+            const long long i = t + lag;
+            const bool in = t < eb.len && i >= 0 && i < ea.len;
+            const double va = env_value(ea, env_load(ea, in ? i : 0), i);
+            acc = __builtin_fma(in ? va : 0.0, vb[u], acc);
+        }
+        const double d = block_sum<kRescoreThreads>(acc, red);
+        if (threadIdx.x == 0) dots[((long long)row * kMaxLagCands + c) * nseg + blockIdx.x] = d;
+        __syncthreads();
+    }
+}
+
+// Per row: each candidate's dot summed over the segments in order, then the
+// first maximum of |dot| in scan order (xcorr.zig:313-326 rescore).
+__global__ __launch_bounds__(kMaxLagCands) void k_lag32_pick(const int* __restrict__ count,
+                                                             const long long* __restrict__ cands,
+                                                             const double* __restrict__ dots, int nseg,
+                                                             long long* __restrict__ lags, int* __restrict__ fallback) {
+    const int row = blockIdx.x;
+    const int nc = count[row];
+    __shared__ double s_dot[kMaxLagCands];
+    if (nc <= 0) {
+        if (threadIdx.x == 0) fallback[row] = 1;
+        return;
+    }
+    if (nc == 1) {
+        // the row's argmax is always nominated (|error| <= slack / 2), so a
+        // lone nominee is it; rescoring one candidate picks it too
+        if (threadIdx.x == 0) {
+            lags[row] = cands[(long long)row * kMaxLagCands];
+            fallback[row] = 0;
+        }
+        return;
+    }
+    if ((int)threadIdx.x < nc) {
+        const double* d = dots + ((long long)row * kMaxLagCands + threadIdx.x) * nseg;
+        double acc = 0.0;
+        for (int g = 0; g < nseg; g++) acc += d[g];
+        s_dot[threadIdx.x] = fabs(acc);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int best = 0;
+    for (int c = 1; c < nc; c++)
+        if (s_dot[c] > s_dot[best]) best = c;
+    lags[row] = cands[(long long)row * kMaxLagCands + best];
+    fallback[row] = 0;
+}
+
+bool lag32_handles(long long n) {
+    int ks[2];
+    return plan32(n, ks) != 0;
+}
+
+size_t lag32_scratch_bytes(int njobs, long long n) { return lag32_layout(njobs, n).total; }
+
+void launch_lag_search32(const LagJob* d_jobs, int njobs, long long n, const float2* d_tw, const double* d_sums,
+                         void* d_scratch, int cap, long long* d_lags, int* d_fallback, hipStream_t stream,
+                         bool keep_xcorr) {
+    if (njobs == 0) return;
+    int ks[2];
+    const int np = plan32(n, ks);
+    const Lag32Layout l = lag32_layout(njobs, n);
+    char* base = static_cast<char*>(d_scratch);
+    double* energy = reinterpret_cast<double*>(base + l.energy);
+    float* slack = reinterpret_cast<float*>(base + l.slack);
+    TileCands* tiles = reinterpret_cast<TileCands*>(base + l.tiles);
+    int* count = reinterpret_cast<int*>(base + l.count);
+    long long* cands = reinterpret_cast<long long*>(base + l.cands);
+    double* dots = reinterpret_cast<double*>(base + l.dots);
+    const int lgn = log2i(n);
+    // forward strided pass(es), the first from the signals
+    pass32<false, kSrcEnvelope, kEpiStore>(ks[0], d_jobs, njobs, n, n, d_tw, d_sums, energy, nullptr, stream);
+    if (np == 2) pass32<false, kSrcZ, kEpiStore>(ks[1], d_jobs, njobs, n, n >> ks[0], d_tw, d_sums, nullptr, nullptr, stream);
+    hipLaunchKernelGGL(k_lag32_slack, dim3(njobs), dim3(64), 0, stream, (const double*)energy, (int)l.wg_first, lgn,
+                       slack);
+    hipLaunchKernelGGL((k_fft_mid<kMid32, kTile32, float2>), dim3((unsigned)((n / kTile32) * njobs)),
+                       dim3(kTile32 / 8), 0, stream, d_jobs, d_tw, njobs);
+    if (np == 2)
+        pass32<true, kSrcZ, kEpiStore>(ks[1], d_jobs, njobs, n, n >> ks[0], d_tw, d_sums, nullptr, nullptr, stream);
+    if (keep_xcorr) {
+        pass32<true, kSrcZ, kEpiStore>(ks[0], d_jobs, njobs, n, n, d_tw, d_sums, nullptr, nullptr, stream);
+        return;
+    }
+    pass32<true, kSrcZ, kEpiCandidates>(ks[0], d_jobs, njobs, n, n, d_tw, d_sums, tiles, slack, stream);
+    hipLaunchKernelGGL(k_lag32_select, dim3(njobs), dim3(256), 0, stream, d_jobs, (const TileCands*)tiles,
+                       (int)l.wg_last, (const float*)slack, cap < kMaxLagCands ? cap : kMaxLagCands, count, cands);
+    hipLaunchKernelGGL(k_lag32_rescore, dim3((unsigned)l.nseg, njobs), dim3(kRescoreThreads), 0, stream, d_jobs,
+                       d_sums, (const int*)count, (const long long*)cands, (int)l.nseg, dots);
+    hipLaunchKernelGGL(k_lag32_pick, dim3(njobs), dim3(kMaxLagCands), 0, stream, (const int*)count,
+                       (const long long*)cands, (const double*)dots, (int)l.nseg, d_lags, d_fallback);
 }
 
 }  // namespace vq

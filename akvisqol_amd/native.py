@@ -161,6 +161,9 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_math_probe": (c_int, [c_int, D, D, D, c_size_t]),
         "visqol_hip_fft": (c_int, [D, c_size_t, c_int]),
         "visqol_hip_xcorr": (c_int, [D, c_size_t, D, c_size_t, D, c_size_t, POINTER(c_size_t)]),
+        "visqol_hip_xcorr32": (c_int, [D, c_size_t, D, c_size_t, POINTER(ctypes.c_float), c_size_t,
+                                       POINTER(c_size_t)]),
+        "visqol_hip_best_lag_fast": (c_int, [D, c_size_t, D, c_size_t, c_int, POINTER(c_int64), POINTER(c_int)]),
         "visqol_hip_prepare": (c_int, [c_void_p, c_size_t, POINTER(D), S, c_uint32, POINTER(c_void_p)]),
         "visqol_hip_measure_prepared": (c_int, [c_void_p, c_void_p, c_size_t, S, POINTER(D), S,
                                                 POINTER(POINTER(Result)), POINTER(c_int32)]),
@@ -772,6 +775,29 @@ def xcorr(a, b) -> np.ndarray:
     got = c_size_t()
     _check(lib.visqol_hip_xcorr(_dptr(a), a.size, _dptr(b), b.size, _dptr(out), out.size, ctypes.byref(got)))
     return out[: got.value]
+
+
+def xcorr32(a, b) -> np.ndarray:
+    """The throughput mode's f32 correlation of the upper envelopes (Re,
+    times n); empty when n is outside 2^14..2^30."""
+    a, b = _f64(a), _f64(b)
+    biggest = max(a.size, b.size)
+    n = 1
+    while n < max(1, 2 * biggest - 1):
+        n <<= 1
+    out = np.zeros(n, dtype=np.float32)
+    got = c_size_t()
+    _check(lib.visqol_hip_xcorr32(_dptr(a), a.size, _dptr(b), b.size,
+                                  out.ctypes.data_as(POINTER(ctypes.c_float)), out.size, ctypes.byref(got)))
+    return out[: got.value]
+
+
+def best_lag_fast(a, b, cap: int = -1):
+    """(lag, nominated) of the throughput mode's lag search on raw signals."""
+    a, b = _f64(a), _f64(b)
+    lag, nom = c_int64(), c_int()
+    _check(lib.visqol_hip_best_lag_fast(_dptr(a), a.size, _dptr(b), b.size, cap, ctypes.byref(lag), ctypes.byref(nom)))
+    return lag.value, bool(nom.value)
 
 
 def fft(z: np.ndarray, dit: bool = False) -> np.ndarray:

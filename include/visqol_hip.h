@@ -213,6 +213,20 @@ int visqol_hip_best_lag(const double *a, size_t la, const double *b, size_t lb, 
  * >= 2·max(la, lb) - 1 (max(la, lb) >= 2).  Writes min(n, cap) values to
  * out and stores n in *n_out. */
 int visqol_hip_xcorr(const double *a, size_t la, const double *b, size_t lb, double *out, size_t cap, size_t *n_out);
+/* The throughput mode's lag search on one pair (xcorr.zig:269-340
+ * refineLag): the lags whose f32 correlation lies within the f32 error
+ * bound of the maximum are rescored by f64 dot products of the envelopes,
+ * in scan order, first maximum.  cap (0..64; negative: the default 64,
+ * MAX_REFINE_CANDIDATES) bounds the lags rescored: a pair with more, or of
+ * a transform size outside 2^14..2^30, takes the f64 search instead
+ * (*nominated = 0). */
+int visqol_hip_best_lag_fast(const double *a, size_t la, const double *b, size_t lb, int cap, int64_t *lag,
+                             int *nominated);
+/* The f32 correlation that search nominates from: Re of the circular
+ * cross-correlation of the two envelopes times n (n as visqol_hip_xcorr);
+ * *n_out = 0 when n is outside 2^14..2^30. */
+int visqol_hip_xcorr32(const double *a, size_t la, const double *b, size_t lb, float *out, size_t cap,
+                       size_t *n_out);
 /* The device radix-2 transform in place on n interleaved complex values
  * (n a power of two): dit = 0 runs the DIF forward transform (natural in,
  * bit-reversed out), dit = 1 the DIT inverse (bit-reversed in, natural out),
