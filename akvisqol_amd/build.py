@@ -22,7 +22,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["gammatone.hip", "signal.hip", "xcorr.hip", "nsim.hip", "adjoint.hip", "engine.cpp", "engine_grad.cpp",
-           "host_plan.cpp", "models.cpp", "wav.cpp", "capi.cpp", "server.cpp", "grad.cpp"]
+           "host_plan.cpp", "models.cpp", "wav.cpp", "capi.cpp", "server.cpp", "grad.cpp",
+           "noalloc.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -158,6 +158,7 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_spectrogram": (c_int, [c_uint32, c_uint32, c_uint32, D, c_size_t, D, S]),
         "visqol_hip_filters": (c_int, [c_uint32, c_uint32, D, D]),
         "visqol_hip_best_lag": (c_int, [D, c_size_t, D, c_size_t, POINTER(c_int64)]),
+        "visqol_hip_best_lag_noalloc": (c_int, [c_void_p, c_size_t, c_void_p, c_size_t, POINTER(c_int64)]),
         "visqol_hip_math_probe": (c_int, [c_int, D, D, D, c_size_t]),
         "visqol_hip_fft": (c_int, [D, c_size_t, c_int]),
         "visqol_hip_xcorr": (c_int, [D, c_size_t, D, c_size_t, D, c_size_t, POINTER(c_size_t)]),
@@ -761,6 +762,13 @@ def best_lag(a, b) -> int:
     a, b = _f64(a), _f64(b)
     lag = c_int64()
     _check(lib.visqol_hip_best_lag(_dptr(a), a.size, _dptr(b), b.size, ctypes.byref(lag)))
+    return lag.value
+
+
+def best_lag_noalloc(dev_a: "DeviceBuffer", dev_b: "DeviceBuffer") -> int:
+    """visqol_hip_best_lag_noalloc on two device-resident signals."""
+    lag = c_int64()
+    _check(lib.visqol_hip_best_lag_noalloc(dev_a.ptr, dev_a.size, dev_b.ptr, dev_b.size, ctypes.byref(lag)))
     return lag.value
 
 

@@ -208,6 +208,16 @@ int visqol_hip_frame_grad(uint32_t mode, uint32_t sample_rate, const double *fra
 int visqol_hip_filters(uint32_t mode, uint32_t sample_rate, double *bands8, double *center_freqs);
 /* FFT cross-correlation lag of the upper envelopes of a and b. */
 int visqol_hip_best_lag(const double *a, size_t la, const double *b, size_t lb, int64_t *lag);
+/* visqol_hip_best_lag on signals in device memory (dev_a, dev_b on the
+ * current device), allocating nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under graph
+ * capture, serving loops that forbid hipMalloc).  Its work buffers are a
+ * static device array of 16 MiB + 20 KiB per device (plus a 16 MiB twiddle
+ * table), held until the process ends; signals of more than 2^19 samples
+ * (a transform beyond 2^20 points) are rejected with
+ * VISQOL_E_INVALID_ARGUMENT.  Calls that overlap on one device race (they
+ * share the array); the lag equals visqol_hip_best_lag's. */
+int visqol_hip_best_lag_noalloc(const double *dev_a, size_t la, const double *dev_b, size_t lb, int64_t *lag);
 /* The values visqol_hip_best_lag maximises: Re of the circular
  * cross-correlation of the two upper envelopes, n = the power of two
  * >= 2·max(la, lb) - 1 (max(la, lb) >= 2).  Writes min(n, cap) values to

@@ -140,3 +140,16 @@ def test_gradient_and_probe_arguments_before_device():
     assert L.visqol_hip_frame_grad(0, 48000, p(frame), 100, p(g), 0, 100, p(out)) == native.E_INVALID_ARGUMENT
     assert L.visqol_hip_frame_grad(0, 48000, p(frame), 3840, p(g), 10, 5, p(out)) == native.E_INVALID_ARGUMENT
     assert L.visqol_hip_frame_grad(0, 48000, p(frame), 3840, p(g), 0, 4000, p(out)) == native.E_INVALID_ARGUMENT
+
+
+def test_noalloc_limit_checked_before_device():
+    # the size limit is checked before any device use (the pointer is never
+    # dereferenced): a transform beyond 2^20 points is refused
+    from akvisqol_amd import native
+
+    L = native.lib
+    lag = ctypes.c_int64()
+    fake = ctypes.c_void_p(256)
+    assert L.visqol_hip_best_lag_noalloc(fake, (1 << 19) + 1, fake, 10, ctypes.byref(lag)) == native.E_INVALID_ARGUMENT
+    assert b"limit" in L.visqol_last_error()
+    assert L.visqol_hip_best_lag_noalloc(None, 5, fake, 5, ctypes.byref(lag)) == native.E_INVALID_ARGUMENT
