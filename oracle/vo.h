@@ -88,6 +88,10 @@ int vo_measure_batch(vo_ctx *c, size_t n, const double *const *refs, const size_
                      const double *const *degs, const size_t *deg_lens, uint32_t sample_rate,
                      uint32_t num_threads, vo_result **out, int32_t *status);
 void vo_result_free(vo_result *r);
+/* buildResult and the mapping over given matches ([n][nb] per-band means,
+ * stddevs, deg energies; reference patch times): aggregation checks. */
+int vo_build_result_from(vo_ctx *c, size_t n, const double *means, const double *stddevs, const double *energy,
+                         const double *ref_start, const double *ref_end, double frame_duration, vo_result **out);
 
 /* ---- stage entry points (unit tests / per-stage parity) ---- */
 double vo_log10(double x);

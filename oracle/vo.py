@@ -106,6 +106,7 @@ def lib() -> ctypes.CDLL:
             "vo_measure_batch": (c_int, [c_void_p, c_size_t, POINTER(D), S, POINTER(D), S, c_uint32, c_uint32,
                                          POINTER(POINTER(Result)), POINTER(c_int32)]),
             "vo_result_free": (None, [POINTER(Result)]),
+            "vo_build_result_from": (c_int, [c_void_p, c_size_t, D, D, D, D, D, c_double, POINTER(POINTER(Result))]),
             "vo_log10": (c_double, [c_double]),
             "vo_center_freqs_desc": (None, [c_size_t, c_double, c_double, D]),
             "vo_filters": (None, [c_uint32, c_size_t, c_double, c_double, POINTER(Band), D]),
@@ -260,6 +261,20 @@ class Oracle:
             trace["deg_spec"] = ds[: nb * tr.nf_deg].reshape(nb, tr.nf_deg).copy()
             trace["sims"] = sims[: np_ * tr.nf_deg].reshape(np_, tr.nf_deg).copy()
         return res, trace
+
+    def build_result(self, means, stddevs, energy, ref_start, ref_end, frame_duration: float) -> dict:
+        """buildResult + mapping over given matches (arrays [n][nb] and [n])."""
+        L = lib()
+        m, s, e = (np.ascontiguousarray(x, dtype=np.float64) for x in (means, stddevs, energy))
+        r0, r1 = f64(ref_start), f64(ref_end)
+        out = POINTER(Result)()
+        rc = L.vo_build_result_from(self._h, m.shape[0], dptr(m), dptr(s), dptr(e), dptr(r0), dptr(r1),
+                                    frame_duration, ctypes.byref(out))
+        if rc:
+            raise OracleError(rc, L.vo_last_error().decode())
+        res = result_to_dict(out.contents)
+        L.vo_result_free(out)
+        return res
 
     def measure_batch(self, refs, degs, sample_rate: int, num_threads: int = 0):
         L = lib()

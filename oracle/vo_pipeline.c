@@ -241,6 +241,30 @@ static int build_result(const struct vo_ctx *c, const vo_match *items, size_t n,
 
 void vo_result_free(vo_result *r) { free(r); }
 
+/* buildResult (visqol.zig:595-709) over caller-given matches, for checks of
+ * the aggregation and the mappings: n matches of the context's band count,
+ * per-band means / stddevs / deg energies row-major [n][nb], each match's
+ * reference start and end times; centre frequencies and lag read zero. */
+int vo_build_result_from(vo_ctx *c, size_t n, const double *means, const double *stddevs, const double *energy,
+                         const double *ref_start, const double *ref_end, double frame_duration, vo_result **out) {
+    if (!c || !out || n == 0 || !means || !stddevs || !energy || !ref_start || !ref_end) return VO_E_INVALID_ARGUMENT;
+    const size_t nb = c->nb;
+    vo_match *items = (vo_match *)calloc(n, sizeof(vo_match));
+    if (!items) return VO_E_OUT_OF_MEMORY;
+    for (size_t i = 0; i < n; i++) {
+        memcpy(items[i].means, means + i * nb, sizeof(double) * nb);
+        memcpy(items[i].stddevs, stddevs + i * nb, sizeof(double) * nb);
+        memcpy(items[i].energy, energy + i * nb, sizeof(double) * nb);
+        items[i].ref_start = ref_start[i];
+        items[i].ref_end = ref_end[i];
+        items[i].valid = 1;
+    }
+    double cf[VO_MAX_BANDS] = {0};
+    const int rc = build_result(c, items, n, frame_duration, cf, 0.0, out);
+    free(items);
+    return rc;
+}
+
 /* realign.zig:262-281 (RawSpectrograms.reuse) */
 static int reuse_frames(size_t nb, const double *global, size_t nf_global, vo_layout l, int64_t off, size_t hop,
                         size_t nf_local, double *out) {
