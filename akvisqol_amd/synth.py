@@ -15,6 +15,10 @@ from a recorded PRNG (numpy PCG64) and a seed:
 * ``mixed_pair``: one pair of the uneven serving mix: 70 % audio / 30 %
   speech, log-uniform duration, noise 5-40 dB SNR, delay 0-4800 samples,
   0-3 % linear-resampling time stretch, 0-5 zeroed 20 ms dropouts.
+* ``mix_source`` / ``mix_request``: the same mix for workloads of thousands
+  of pairs: long source pairs carry the delay, low-pass and stretch, and
+  each request is a slice of one (log-uniform duration, random offset) with
+  its own noise, dropouts and quantisation.
 """
 from __future__ import annotations
 
@@ -151,6 +155,49 @@ def mixed_pair(seed: int, max_duration: float = 60.0):
     if not speech:
         deg = _quantise16(deg)
     return ref, deg, sr, 1 if speech else 0
+
+
+def mix_source(seed: int, duration: float):
+    """A long pair of the uneven mix before its per-request degradations:
+    (ref, deg, sample_rate, mode), 70 % 48 kHz audio / 30 % 16 kHz speech,
+    the degraded copy delayed 0-4800 samples, low-passed (speech) and
+    time-stretched 0-3 % by linear resampling.  mix_request cuts requests
+    from it."""
+    rng = np.random.default_rng(30_000_000 + seed)
+    speech = rng.uniform() < 0.3
+    delay = int(rng.integers(0, 4801))
+    stretch = 1.0 + rng.uniform(0.0, 0.03)
+    if speech:
+        sr = 16000
+        ref = speech_reference(40_000 + seed, duration, sr)
+        deg = _butter_lowpass(_delay(ref, delay), 4000.0, sr)
+    else:
+        sr = 48000
+        ref = audio_reference(40_000 + seed, duration, sr)
+        deg = _delay(ref, delay)
+    return ref, _stretch(deg, stretch), sr, 1 if speech else 0
+
+
+def mix_request(src, seed: int, lo: float = 1.0, hi: float = 60.0):
+    """One pair of the uneven mix cut from a mix_source pair: a duration
+    log-uniform in [lo, hi] s (at most the source's), a random offset, then
+    the pair's own degradations: white noise at 5-40 dB SNR, 0-5 zeroed
+    20 ms dropouts, 16-bit quantisation for audio.  The reference is a view
+    of the source; the degraded signal is new."""
+    ref, deg, sr, mode = src
+    rng = np.random.default_rng(50_000_000 + seed)
+    n = min(ref.size, deg.size)
+    d = min(n, max(2, int(np.exp(rng.uniform(np.log(lo), np.log(hi))) * sr)))
+    o = int(rng.integers(0, n - d + 1))
+    out = _add_noise(rng, deg[o:o + d], rng.uniform(5.0, 40.0))
+    dl = int(0.02 * sr)
+    for _ in range(int(rng.integers(0, 6))):
+        if out.size > dl:
+            s0 = int(rng.integers(0, out.size - dl))
+            out[s0: s0 + dl] = 0.0
+    if mode == 0:
+        out = _quantise16(out)
+    return ref[o:o + d], out, sr, mode
 
 
 def uniform_durations(seed: int, n: int, lo: float, hi: float) -> np.ndarray:
