@@ -1,4 +1,5 @@
-"""Builds akvisqol_amd/libvisqol.so for gfx950 with hipcc.
+"""Builds akvisqol_amd/libvisqol.so for gfx950 with hipcc, and the visqol
+command-line tool (akvisqol_amd/visqol) over it with the host compiler.
 
 Every source compiles in parallel into _build/, then links into one shared
 library next to this file, where the bundled models live.  Device code is
@@ -18,6 +19,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libvisqol.so"
+CLI = HERE / "visqol"  # the command-line tool (csrc/cli.cpp)
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -58,8 +61,15 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the command-line tool, host code over the C ABI, next to the library
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", f"-I{HERE.parent / 'include'}", str(CSRC / "cli.cpp"), "-o", str(CLI) + ".tmp",
+           str(LIB), "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building the visqol tool failed:\n{r.stderr}")
+    os.replace(str(CLI) + ".tmp", CLI)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
 
 
