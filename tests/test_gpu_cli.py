@@ -94,7 +94,7 @@ def test_batch_csv_and_results(files, tmp_path):
     keys = ["a0", "a1", "a2", "a0"]
     csv.write_text("reference,degraded\n" + "".join(f"{files[k][0]},{files[k][1]}\n" for k in keys))
     res = tmp_path / "out.csv"
-    r = _run("--batch_input_csv", csv, "--results_csv", res, "--precision", "fast")
+    r = _run("--batch_input_csv", csv, "--results_csv", res, "--precision", "fast", "--backend", "hip")
     assert r.returncode == 0, r.stderr
     rows = res.read_text().splitlines()
     assert rows[0] == "reference,degraded,moslqo" and len(rows) == 1 + len(keys)
@@ -105,6 +105,18 @@ def test_batch_csv_and_results(files, tmp_path):
         assert (ref, deg) == (str(files[k][0]), str(files[k][1]))
         assert float(mos) == want
         assert line == f"{ref} {deg} MOS-LQO: {want:.6f}"
+
+
+def test_cpu_backend_is_exact(files):
+    """--backend cpu (the default) scores in f64 whatever --precision says,
+    as the reference's CPU backend does."""
+    from akvisqol_amd import native
+
+    ref, deg = files["a1"]
+    r = _run("--reference_file", ref, "--degraded_file", deg, "--precision", "fast")
+    assert r.returncode == 0, r.stderr
+    want = native.Context(0, precision="exact").measure_files(ref, deg)
+    assert r.stdout.splitlines()[2] == f"MOS-LQO:\t\t{want['moslqo']:.6f}"
 
 
 def test_batch_rejects_mixed_sample_rates(files, tmp_path):
