@@ -159,8 +159,6 @@ Engine::~Engine() {
         (void)hipFree(kv.second->fir);
         (void)hipFree(kv.second->hann);
     }
-    for (auto& kv : twiddles_) (void)hipFree(kv.second);
-    for (auto& kv : twiddles32_) (void)hipFree(kv.second);
     for (hipStream_t cs : copy_streams_)
         if (cs) (void)hipStreamDestroy(cs);
     if (stream_ && owns_stream_) (void)hipStreamDestroy(stream_);
@@ -312,28 +310,84 @@ Engine::DevRate& Engine::dev_rate(uint32_t sr) {
     return ref;
 }
 
-const double2* Engine::twiddles(long long n) {
-    auto it = twiddles_.find(n);
-    if (it != twiddles_.end()) return it->second;
-    const std::vector<double> tw = make_level_twiddles(n);
-    double2* d = nullptr;
-    check(hipMalloc(&d, sizeof(double) * tw.size()), "hipMalloc twiddles");
-    check(hipMemcpy(d, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice), "upload twiddles");
-    twiddles_[n] = d;
-    return d;
-}
+namespace {
 
-const float2* Engine::twiddles32(long long n) {
-    auto it = twiddles32_.find(n);
-    if (it != twiddles32_.end()) return it->second;
-    const std::vector<double> tw = make_level_twiddles(n);
-    std::vector<float> t32(tw.begin(), tw.end());
-    float2* d = nullptr;
-    check(hipMalloc(&d, sizeof(float) * t32.size()), "hipMalloc twiddles");
-    check(hipMemcpy(d, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice), "upload twiddles");
-    twiddles32_[n] = d;
-    return d;
-}
+// Level twiddle tables shared by every engine of the process on one device,
+// one per precision, each grown to the largest transform asked for: the
+// table for n begins with the table of every smaller power of two, bit for
+// bit, so one table serves every size.  A table replaced by a larger one
+// stays allocated, since kernels of other engines may still read it.  The
+// first transform of a new largest size pays the host generation (16 host
+// threads: ~30 ms for 2^24 points) and one upload; later ones, in any
+// engine, pay nothing.
+class TwiddleStore {
+   public:
+    static TwiddleStore& of_current_device() {
+        static std::mutex mu;
+        static std::map<int, TwiddleStore*>* stores = new std::map<int, TwiddleStore*>();
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        std::lock_guard<std::mutex> lk(mu);
+        TwiddleStore*& s = (*stores)[dev];
+        if (!s) s = new TwiddleStore();  // lives until the process ends
+        return *s;
+    }
+    const double2* f64(long long n) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (n > cap64_) {
+            grow_host(n);
+            double2* d = nullptr;
+            put(d, host_.data(), host_.size());
+            d64_ = d;
+            cap64_ = host_cap_;
+        }
+        return d64_;
+    }
+    const float2* f32(long long n) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (n > cap32_) {
+            grow_host(n);
+            std::vector<float> t(host_.begin(), host_.end());
+            float2* d = nullptr;
+            put(d, t.data(), t.size());
+            d32_ = d;
+            cap32_ = host_cap_;
+        }
+        return d32_;
+    }
+
+   private:
+    static constexpr long long kMinCap = 1LL << 20;
+    void grow_host(long long n) {
+        if (n <= host_cap_) return;
+        long long cap = kMinCap;
+        while (cap < n) cap <<= 1;
+        host_ = make_level_twiddles(cap, 16);
+        host_cap_ = cap;
+    }
+    template <typename T, typename V>
+    static void put(T*& d, const V* src, size_t count) {
+        if (hipMalloc(&d, sizeof(V) * count) != hipSuccess) {
+            (void)hipGetLastError();
+            throw BackendError{-2, "device out of memory (twiddle table)"};
+        }
+        if (hipMemcpy(d, src, sizeof(V) * count, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            throw BackendError{-6, "twiddle table upload failed"};
+        }
+    }
+    std::mutex mu_;
+    std::vector<double> host_;
+    long long host_cap_ = 0, cap64_ = 0, cap32_ = 0;
+    double2* d64_ = nullptr;
+    float2* d32_ = nullptr;
+};
+
+}  // namespace
+
+const double2* Engine::twiddles(long long n) { return TwiddleStore::of_current_device().f64(n); }
+
+const float2* Engine::twiddles32(long long n) { return TwiddleStore::of_current_device().f32(n); }
 
 void Engine::lap(int phase) {
     const auto now = std::chrono::steady_clock::now();

@@ -247,6 +247,8 @@ class Engine {
         double* hann = nullptr;
     };
     DevRate& dev_rate(uint32_t sample_rate);
+    // Level twiddles for transforms of up to n points, shared by the
+    // process's engines on the current device.
     const double2* twiddles(long long n);
     const float2* twiddles32(long long n);  // the f64 table rounded to f32
     template <typename T>
@@ -297,8 +299,6 @@ class Engine {
     DeviceArena arena_;
     PinnedArena pinned_;
     std::map<uint32_t, std::unique_ptr<DevRate>> rates_;
-    std::map<long long, double2*> twiddles_;
-    std::map<long long, float2*> twiddles32_;
 };
 
 }  // namespace vq

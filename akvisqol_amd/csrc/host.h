@@ -37,8 +37,9 @@ RateParams make_rate_params(int mode, uint32_t sample_rate);
 
 // Twiddles W_n^k = (cos(-2πk/n), sin(-2πk/n)) for k < n/2, interleaved.
 std::vector<double> make_twiddles(long long n);
-// W_S^j for every span S = 2..n at complex index S/2 + j (2n doubles).
-std::vector<double> make_level_twiddles(long long n);
+// W_S^j for every span S = 2..n at complex index S/2 + j (2n doubles); the
+// table for n begins with the table of every smaller power of two.
+std::vector<double> make_level_twiddles(long long n, int threads = 1);
 
 long long fft_size_for(long long biggest);  // xcorr.zig:64-69
 int num_frames(long long len, int window, int hop);

@@ -6,6 +6,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <thread>
 
 #include "host.h"
 
@@ -147,16 +148,35 @@ std::vector<double> make_twiddles(long long n) {
 }
 
 // Twiddles of every radix-2 span S = 2, 4, ..., n in one table: W_S^j at
-// complex index S/2 + j (j < S/2), copied from the n-point table (W_S^j =
-// W_n^{j·n/S}, the same value), so that a stage's twiddles for consecutive j
-// are consecutive in memory.  Index 0 is unused.
-std::vector<double> make_level_twiddles(long long n) {
-    const std::vector<double> base = make_twiddles(n);
+// complex index S/2 + j (j < S/2), so that a stage's twiddles for
+// consecutive j are consecutive in memory.  Index 0 is unused.  The top
+// span holds W_n^k as make_twiddles computes it (on up to `threads` host
+// threads); every smaller span copies W_S^j = W_n^{j·n/S} from it.  The
+// angle -2π·(j·n/S)/n differs from -2π·j/S only by exact power-of-two
+// scalings, so the table for n starts with the table of every smaller
+// power of two, bit for bit.
+std::vector<double> make_level_twiddles(long long n, int threads) {
     std::vector<double> t(2 * std::max(2LL, n), 0.0);
-    for (long long S = 2; S <= n; S <<= 1)
+    const double two_pi = 2.0 * M_PI;
+    const double n_f = (double)n;
+    double* top = t.data() + n;  // complex index n/2
+    const long long half = n / 2;
+    auto fill = [&](long long k0, long long k1) {
+        for (long long k = k0; k < k1; k++) {
+            const double angle = -two_pi * (double)k / n_f;
+            top[2 * k] = cos(angle);
+            top[2 * k + 1] = sin(angle);
+        }
+    };
+    const long long nt = std::max(1LL, std::min<long long>(threads, half >> 16));
+    std::vector<std::thread> pool;
+    for (long long i = 1; i < nt; i++) pool.emplace_back(fill, half * i / nt, half * (i + 1) / nt);
+    fill(0, half / nt);
+    for (auto& th : pool) th.join();
+    for (long long S = 2; S < n; S <<= 1)
         for (long long j = 0; j < S / 2; j++) {
-            t[2 * (S / 2 + j)] = base[2 * (j * (n / S))];
-            t[2 * (S / 2 + j) + 1] = base[2 * (j * (n / S)) + 1];
+            t[2 * (S / 2 + j)] = top[2 * (j * (n / S))];
+            t[2 * (S / 2 + j) + 1] = top[2 * (j * (n / S)) + 1];
         }
     return t;
 }

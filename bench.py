@@ -133,6 +133,11 @@ def serve_level(srv, reqs, qps: float, seconds: float, seed: int, submitters: in
                mean_batch=round(float(batch.mean()), 2),
                tail_phase_ms={k: round(float(phases[tail, j].mean()), 2)
                               for j, k in enumerate(("admit", "upload", "queue", "run"))})
+    worst = int(np.argmax(lat))
+    wref, _, wsr, wmode = reqs[order[worst]]
+    rep["max_request"] = dict(seconds=round(wref.size / wsr, 2), mode=int(wmode), batch=int(batch[worst]),
+                              phase_ms={k: round(float(phases[worst, j]), 2)
+                                        for j, k in enumerate(("admit", "upload", "queue", "run"))})
     return rep, order, mos
 
 
