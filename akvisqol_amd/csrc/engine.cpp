@@ -1779,7 +1779,10 @@ void Engine::xcorr32(const double* a, long long la, const double* b, long long l
     readback(p, z, sizeof(float2) * nfft);
     sync();
     out.resize(nfft);
-    for (long long i = 0; i < nfft; i++) out[i] = p[2 * i];
+    for (long long h = 0; h < nfft / 2; h++) {
+        out[2 * h] = p[2 * lag32_xcorr_slot(h)];
+        out[2 * h + 1] = p[2 * lag32_xcorr_slot(h) + 1];
+    }
 }
 
 void Engine::fft_probe(double* z, long long n, bool dit) {

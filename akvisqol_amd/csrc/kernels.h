@@ -101,9 +101,11 @@ void launch_fft_dit(const LagJob* d_jobs, int njobs, long long n, const double2*
 // fallback[j] = 0, else fallback[j] = 1 and lags[j] is left for the f64
 // search.  d_tw32: the f32 twiddle table (W_S^j at S/2 + j, rounded from
 // the f64 one); each job's z holds at least n float2.  keep_xcorr: leave
-// the f32 circular correlation (times n, as float2 with Re first) in each
-// job's z instead, for checks.
+// the f32 circular correlation c (times n) in each job's z instead, for
+// checks: c[2h] and c[2h + 1] are the two floats of the float2 at
+// lag32_xcorr_slot(h), h < n/2.
 constexpr int kMaxLagCands = 64;
+__host__ __device__ inline long long lag32_xcorr_slot(long long h) { return h + (h & ~2047LL); }
 bool lag32_handles(long long n);  // n a power of two in [2^14, 2^30]
 size_t lag32_scratch_bytes(int njobs, long long n);
 void launch_lag_search32(const LagJob* d_jobs, int njobs, long long n, const float2* d_tw32, const double* d_sums,

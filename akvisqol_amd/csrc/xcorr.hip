@@ -91,7 +91,13 @@ constexpr int kTile = 2048;  // complex doubles per workgroup (32 KiB LDS)
 constexpr int kThreads = 256;
 constexpr int kBfPerThread = kTile / 2 / kThreads;
 
-enum : int { kSrcZ = 0, kSrcEnvelope = 1 };
+// Tile sources: the row z, the two signals' envelopes, or a row holding
+// the nominating transform's half-length inverse (half position h at
+// h + (h & ~(kHalfChunk32 - 1)), value c[2h] + i·c[2h + 1]).
+enum : int { kSrcZ = 0, kSrcEnvelope = 1, kSrcHalf = 2 };
+constexpr long long kHalfChunk32 = 2048;
+__device__ inline long long half_addr(long long h) { return lag32_xcorr_slot(h); }
+static_assert(kHalfChunk32 == 2048, "lag32_xcorr_slot's chunk");
 
 // Scan position of circular index i (xcorr.zig:233-250), or -1 outside
 // the lag range.
@@ -388,7 +394,7 @@ enum : int { kEpiStore = 0, kEpiArgmax = 1, kEpiCandidates = 2 };
 // the tile's largest in-range |Re|, how many in-range elements lie within
 // the row's slack of it, and the first kTileCands of them (scan position
 // and |Re|).  The rows' candidates are a subset of the union of these.
-constexpr int kTileCands = 14;
+constexpr int kTileCands = 30;
 struct TileCands {
     float top;
     int count;
@@ -414,17 +420,24 @@ __device__ inline double block_sum(double v, double* red) {
 // tile's own top.  The row's floor (row top - slack) is at least this
 // tile's floor, so every row candidate in this tile is among those counted
 // here; their order in out->pos is arbitrary (the row merge sorts them).
-template <int THREADS, typename GIDX>
+// HALF: element e holds c[2i] + i·c[2i + 1] of half position i = gidx(e),
+// two lags each; n is the full transform size either way.
+template <int THREADS, bool HALF, typename GIDX>
 __device__ inline void tile_candidates(const float2* sm, GIDX gidx, long long n, const LagJob& job, float slack,
                                        TileCands* out) {
     constexpr int kPer = 8;
+    constexpr int kVals = HALF ? 2 : 1;
     const long long big = job.la > job.lb ? job.la : job.lb;
     const long long max_lag = big - 1;
+    auto value = [&](int e, int h) -> float { return fabsf(h ? sm[e].y : sm[e].x); };
+    auto index = [&](int e, int h) -> long long { return HALF ? 2 * gidx(e) + h : gidx(e); };
     unsigned top_bits = 0;
 #pragma unroll
     for (int u = 0; u < kPer; u++) {
         const int e = threadIdx.x + u * THREADS;
-        if (scan_order(gidx(e), n, max_lag) >= 0) top_bits = max(top_bits, __float_as_uint(fabsf(sm[e].x)));
+#pragma unroll
+        for (int h = 0; h < kVals; h++)
+            if (scan_order(index(e, h), n, max_lag) >= 0) top_bits = max(top_bits, __float_as_uint(value(e, h)));
     }
     __shared__ unsigned s_top;
     __shared__ int s_count;
@@ -441,13 +454,16 @@ __device__ inline void tile_candidates(const float2* sm, GIDX gidx, long long n,
 #pragma unroll
     for (int u = 0; u < kPer; u++) {
         const int e = threadIdx.x + u * THREADS;
-        const long long so = scan_order(gidx(e), n, max_lag);
-        const float v = fabsf(sm[e].x);
-        if (so < 0 || !(v >= floor)) continue;
-        const int slot = atomicAdd(&s_count, 1);
-        if (slot < kTileCands) {
-            out->pos[slot] = (int)so;
-            out->val[slot] = v;
+#pragma unroll
+        for (int h = 0; h < kVals; h++) {
+            const long long so = scan_order(index(e, h), n, max_lag);
+            const float v = value(e, h);
+            if (so < 0 || !(v >= floor)) continue;
+            const int slot = atomicAdd(&s_count, 1);
+            if (slot < kTileCands) {
+                out->pos[slot] = (int)so;
+                out->val[slot] = v;
+            }
         }
     }
     __syncthreads();
@@ -469,7 +485,10 @@ struct TileWaves {
 
 // SRC == kSrcEnvelope builds the tile from the two signals' envelopes
 // (computed in f64; a float2 tile rounds them to f32 and adds the tile's
-// share of ‖env_a‖² + ‖env_b‖² to aux[job·nwg + wg], as doubles).
+// share of ‖env_a‖² + ‖env_b‖² to aux[job·nwg + wg], as doubles);
+// kSrcHalf works on the half-length row of the nominating transform's
+// inverse (lgn its length, half the transform's), loading and storing it
+// where the middle pass put it.
 // EPI == kEpiArgmax writes the (bits, position) pair of the tile's first
 // maximum to aux; kEpiCandidates writes a TileCands to aux against the
 // row's slack slack[job].
@@ -497,6 +516,11 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TileWa
         const int rr = e & ((1 << lgR) - 1);
         return base + ((long long)bb << lgS) + ((long long)m << lgq) + rr;
     };
+    // where row position g is stored
+    auto addr = [&](long long g) -> long long {
+        if constexpr (SRC == kSrcHalf) return half_addr(g);
+        else return g;
+    };
     constexpr int kPer = 8;
     if (SRC == kSrcEnvelope) {
         // both signals' samples for the tile, loads first, envelope after
@@ -508,6 +532,13 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TileWa
         double energy = 0.0;
 #pragma unroll
         for (int g = 0; g < kPer; g += kGroup) {
+            if (sizeof(C) == 8 && g >= kPer / 2) {
+                // the first pass's tile spans the whole row, its second
+                // half the positions >= n/2, past both signals (n >= 2·len - 1)
+#pragma unroll
+                for (int u = 0; u < kGroup; u++) sm[threadIdx.x + (g + u) * THREADS] = Cx<C>::make(0, 0);
+                continue;
+            }
             double xa[kGroup], xb[kGroup];
 #pragma unroll
             for (int u = 0; u < kGroup; u++) {
@@ -532,7 +563,7 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TileWa
     } else {
         C v[kPer];
 #pragma unroll
-        for (int u = 0; u < kPer; u++) v[u] = load2(z, gidx(threadIdx.x + u * THREADS));
+        for (int u = 0; u < kPer; u++) v[u] = load2(z, addr(gidx(threadIdx.x + u * THREADS)));
 #pragma unroll
         for (int u = 0; u < kPer; u++) sm[threadIdx.x + u * THREADS] = v[u];
     }
@@ -542,14 +573,15 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TileWa
 #pragma unroll
         for (int u = 0; u < kPer; u++) {
             const int e = threadIdx.x + u * THREADS;
-            store2(z, gidx(e), sm[e]);
+            store2(z, addr(gidx(e)), sm[e]);
         }
         return;
     } else if constexpr (EPI == kEpiArgmax) {
         tile_argmax<THREADS>(sm, TILE, gidx, n, job, static_cast<unsigned long long*>(aux) + 2 * (jb * nwg + wg));
         return;
     } else {
-        tile_candidates<THREADS>(sm, gidx, n, job, slack[jb], static_cast<TileCands*>(aux) + (jb * nwg + wg));
+        tile_candidates<THREADS, SRC == kSrcHalf>(sm, gidx, SRC == kSrcHalf ? 2 * n : n, job, slack[jb],
+                                                  static_cast<TileCands*>(aux) + (jb * nwg + wg));
     }
 }
 
@@ -597,18 +629,19 @@ __global__ __launch_bounds__(256) void k_cross_terms(const LagJob* __restrict__ 
 // Workgroup w holds the chunk pair (0, 1), or for w >= 1 with
 // 2^k <= w < 2^(k+1): chunks 2^(k+1) + t and 2^(k+2) - 1 - t, t = w - 2^k.
 
-// One radix-2^B step (B = 1, 2, 3) over the eight elements a thread holds in
-// a pass over contiguous blocks (one residue): 8 >> B groups, element c of
+// One radix-2^B step (B = 1, 2, 3) over the N (eight or four) elements a
+// thread holds in a pass over contiguous blocks (one residue): N >> B
+// groups, element c of
 // group gi in x[gi·2^B + c] at block position m0 + c·2^lgH.  The stage with
 // distance d·2^lgH (d = 2^s, s < B) pairs c with c + d (bit s of c clear)
 // and takes W_{2d·2^lgH} at position lo + (c mod d)·2^lgH, i.e. table index
 // (d << lgH) + lo + ((c mod d) << lgH).  DIF runs the stages largest
 // distance first, DIT smallest first: the radix-2 butterflies of the
 // stage-by-stage transform.
-template <int B, bool DIT, typename C>
-__device__ inline void radix_step(C (&x)[8], const int (&lo)[8 >> B], int lgH, const C* __restrict__ tw) {
+template <int B, bool DIT, typename C, int N>
+__device__ inline void radix_step(C (&x)[N], const int (&lo)[N >> B], int lgH, const C* __restrict__ tw) {
     constexpr int G = 1 << B;
-    constexpr int NG = 8 >> B;
+    constexpr int NG = N >> B;
     C w[NG][G];  // w[gi][d + t]: stage distance d, position t (t < d)
 #pragma unroll
     for (int gi = 0; gi < NG; gi++)
@@ -636,13 +669,13 @@ __device__ inline void radix_step(C (&x)[8], const int (&lo)[8 >> B], int lgH, c
 }
 
 // Tile positions (block bb of 2^K elements, position m) of the thread's
-// eight elements for a radix-2^B step with smallest distance 2^lgH: group
+// N elements for a radix-2^B step with smallest distance 2^lgH: group
 // G = thread + gi·THREADS, q = G mod 2^(K-B) its index in the block, lo = q
 // mod 2^lgH, m0 = q with B zero bits inserted at bit lgH.
-template <int K, int B, int THREADS>
-__device__ inline void group_layout(int lgH, int (&idx)[8], int (&lo)[8 >> B]) {
+template <int K, int B, int THREADS, int N = 8>
+__device__ inline void group_layout(int lgH, int (&idx)[N], int (&lo)[N >> B]) {
 #pragma unroll
-    for (int gi = 0; gi < (8 >> B); gi++) {
+    for (int gi = 0; gi < (N >> B); gi++) {
         const int g = (int)threadIdx.x + gi * THREADS;
         const int q = g & ((1 << (K - B)) - 1);
         const int bb = g >> (K - B);
@@ -650,6 +683,61 @@ __device__ inline void group_layout(int lgH, int (&idx)[8], int (&lo)[8 >> B]) {
         const int m0 = ((q >> lgH) << (lgH + B)) | lo[gi];
 #pragma unroll
         for (int c = 0; c < (1 << B); c++) idx[gi * (1 << B) + c] = (bb << K) + m0 + (c << lgH);
+    }
+}
+
+// The half-size inverse of the nominating transform's middle pass (see
+// k_fft_mid, HALF): x holds the cross terms of the aligned run of eight at
+// row position g0; makes the thread's four Y values, runs the n/2-point
+// inverse stages of spans 2 .. 2^(K-1) over the workgroup's two half
+// chunks (4 + 4 + 4 + ... + a last radix-2 step when K - 1 is odd) and
+// stores them.  LDS holds the half chunks at h ^ (((h >> 5)·5) & 31): every
+// radix-4/2 access of 32 lanes then falls on distinct 8-byte columns of a
+// 256-byte bank row, but for a few of the smallest-distance steps.
+template <int K, int THREADS>
+__device__ inline void mid_half_inverse(float2 (&x)[8], long long g0, long long ca, long long cb, int lgn,
+                                        float2* sm, float2* z, const float2* __restrict__ tw) {
+    constexpr int KH = K - 1;
+    constexpr int kQuarter = 1 << KH;  // half-array elements per chunk
+    float2 y[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const unsigned long long k = __builtin_bitreverse64((unsigned long long)(g0 + 2 * c)) >> (64 - lgn);
+        const float2 w = tw[(1LL << (lgn - 1)) + (long long)k];
+        const float2 sum = cadd(x[2 * c], x[2 * c + 1]);
+        const float2 dif = cmul_conj(csub(x[2 * c], x[2 * c + 1]), w);
+        y[c] = make_float2(sum.x - dif.y, sum.y + dif.x);  // sum + i·dif
+    }
+    auto hswz = [](int h) { return h ^ (((h >> 5) * 5) & 31); };
+    int hidx[4];
+    {
+        int lo[1];
+        group_layout<KH, 2, THREADS, 4>(0, hidx, lo);  // the run of eight's four half positions
+        radix_step<2, true>(y, lo, 0, tw);
+    }
+#pragma unroll
+    for (int lgH = 2; lgH < KH; lgH += 2) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) sm[hswz(hidx[c])] = y[c];
+        __syncthreads();
+        if (lgH + 2 <= KH) {
+            int lo[1];
+            group_layout<KH, 2, THREADS, 4>(lgH, hidx, lo);
+#pragma unroll
+            for (int c = 0; c < 4; c++) y[c] = sm[hswz(hidx[c])];
+            radix_step<2, true>(y, lo, lgH, tw);
+        } else {
+            int lo[2];
+            group_layout<KH, 1, THREADS, 4>(lgH, hidx, lo);
+#pragma unroll
+            for (int c = 0; c < 4; c++) y[c] = sm[hswz(hidx[c])];
+            radix_step<1, true>(y, lo, lgH, tw);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int h = hidx[c];
+        store2(z, (h < kQuarter ? ca : cb) * (2LL * kQuarter) + (h & (kQuarter - 1)), y[c]);
     }
 }
 
@@ -684,12 +772,26 @@ struct MidPlan {
 // Four waves per SIMD (at most 128 VGPRs): 2048-element tiles then fit four
 // workgroups per CU instead of three (a few spilled registers cost less
 // than the lost occupancy); 4096-element tiles are held to two by LDS.
-template <int K, int TILE, typename C>
+//
+// HALF (the nominating transform, blocks of one chunk): the correlation c
+// is real, so its unscaled n-point inverse is taken as an n/2-point one.
+// With C the cross-term spectrum and, for k < n/2,
+//   Y[k] = (C[k] + C[k + n/2]) + i·conj(W_n^k)·(C[k] - C[k + n/2]),
+// the unscaled n/2-point inverse of Y is y[m] = c[2m] + i·c[2m + 1].  In
+// the forward output's bit-reversed order C[k] and C[k + n/2] sit side by
+// side, at positions 2p and 2p + 1 with k = rev(2p), and Y[k] belongs at
+// position p of a half-length row in the bit-reversed order of n/2 points
+// (rev_n(2p) = rev_{n/2}(p)), so the inverse goes on as an n/2-point DIT:
+// the stages of spans 2 .. 2^(K-1) here, four elements per thread, the rest
+// in the strided passes.  Chunk c's 2^(K-1) results go to the first half of
+// the chunk's own storage: half position h at h + (h & ~(2^(K-1) - 1)).
+template <int K, int TILE, typename C, bool HALF = false>
 __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_mid(const LagJob* __restrict__ jobs, const C* __restrict__ tw,
-                                                      int njobs) {
+                                                      int njobs, int lgn) {
     constexpr int THREADS = TILE / 8;
     constexpr int kHalfTile = TILE / 2;
     static_assert((1 << K) <= kHalfTile, "blocks of 2^K elements must fit a chunk");
+    static_assert(!HALF || (1 << K) == kHalfTile, "the half-size inverse works on whole chunks");
     __shared__ C sm[TILE];
     using P = MidPlan<K>;
     const int jb = (int)(blockIdx.x % (unsigned)njobs);
@@ -778,7 +880,12 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < 8; c++) x[c] = cross_term(x[c], mirror[c]);
-        radix_step<3, true>(x, lo, 0, tw);
+        if constexpr (HALF) {
+            mid_half_inverse<K, THREADS>(x, gidx(idx[0]), ca, cb, lgn, sm, z, tw);
+            return;
+        } else {
+            radix_step<3, true>(x, lo, 0, tw);
+        }
     }
     // inverse steps, mirroring the forward ones
     int bottom = 3;
@@ -965,13 +1072,13 @@ void launch_lag_search(const LagJob* d_jobs, int njobs, long long n, const doubl
     if (fuse) {
         const dim3 g((unsigned)((n / kTile) * njobs));
         switch (ks[np - 1]) {
-            case 7: hipLaunchKernelGGL((k_fft_mid<7, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            case 8: hipLaunchKernelGGL((k_fft_mid<8, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            case 9: hipLaunchKernelGGL((k_fft_mid<9, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
-            case 10: hipLaunchKernelGGL((k_fft_mid<10, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs); break;
+            case 7: hipLaunchKernelGGL((k_fft_mid<7, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs, 0); break;
+            case 8: hipLaunchKernelGGL((k_fft_mid<8, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs, 0); break;
+            case 9: hipLaunchKernelGGL((k_fft_mid<9, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs, 0); break;
+            case 10: hipLaunchKernelGGL((k_fft_mid<10, kTile, double2>), g, dim3(kThreads), 0, stream, d_jobs, d_tw, njobs, 0); break;
             default:
                 hipLaunchKernelGGL((k_fft_mid<11, kBigTile, double2>), dim3((unsigned)((n / kBigTile) * njobs)),
-                                   dim3(kBigTile / 8), 0, stream, d_jobs, d_tw, njobs);
+                                   dim3(kBigTile / 8), 0, stream, d_jobs, d_tw, njobs, 0);
                 break;
         }
     } else {
@@ -1024,6 +1131,7 @@ void launch_fft_dit(const LagJob* d_jobs, int njobs, long long n, const double2*
 namespace {
 constexpr int kTile32 = 8192;   // float2 per workgroup (64 KiB LDS, 1024 threads)
 constexpr int kMid32 = 12;      // stages of the fused middle (blocks of 4096)
+static_assert(kHalfChunk32 == (1 << (kMid32 - 1)), "half-length chunks are the middle blocks' halves");
 constexpr int kMaxStrided32 = 9;
 constexpr int kRescoreThreads = 256;
 constexpr int kRescorePer = 16;
@@ -1081,7 +1189,7 @@ Lag32Layout lag32_layout(int njobs, long long n) {
     Lag32Layout l{};
     if (np == 0) return l;
     l.wg_first = pass_shape(n, n, ks[0], kTile32).wgs;
-    l.wg_last = l.wg_first;
+    l.wg_last = pass_shape(n / 2, n / 2, ks[0], kTile32).wgs;  // over the half-length inverse
     l.nseg = (n / 2 + kRescoreSeg - 1) / kRescoreSeg;
     size_t at = 0;
     l.energy = at;
@@ -1289,15 +1397,18 @@ void launch_lag_search32(const LagJob* d_jobs, int njobs, long long n, const flo
     if (np == 2) pass32<false, kSrcZ, kEpiStore>(ks[1], d_jobs, njobs, n, n >> ks[0], d_tw, d_sums, nullptr, nullptr, stream);
     hipLaunchKernelGGL(k_lag32_slack, dim3(njobs), dim3(64), 0, stream, (const double*)energy, (int)l.wg_first, lgn,
                        slack);
-    hipLaunchKernelGGL((k_fft_mid<kMid32, kTile32, float2>), dim3((unsigned)((n / kTile32) * njobs)),
-                       dim3(kTile32 / 8), 0, stream, d_jobs, d_tw, njobs);
+    // forward middle, cross terms, and the first stages of the half-length inverse
+    hipLaunchKernelGGL((k_fft_mid<kMid32, kTile32, float2, true>), dim3((unsigned)((n / kTile32) * njobs)),
+                       dim3(kTile32 / 8), 0, stream, d_jobs, d_tw, njobs, lgn);
+    // the half-length inverse's strided passes (the same stage counts)
+    const long long h = n / 2;
     if (np == 2)
-        pass32<true, kSrcZ, kEpiStore>(ks[1], d_jobs, njobs, n, n >> ks[0], d_tw, d_sums, nullptr, nullptr, stream);
+        pass32<true, kSrcHalf, kEpiStore>(ks[1], d_jobs, njobs, h, h >> ks[0], d_tw, d_sums, nullptr, nullptr, stream);
     if (keep_xcorr) {
-        pass32<true, kSrcZ, kEpiStore>(ks[0], d_jobs, njobs, n, n, d_tw, d_sums, nullptr, nullptr, stream);
+        pass32<true, kSrcHalf, kEpiStore>(ks[0], d_jobs, njobs, h, h, d_tw, d_sums, nullptr, nullptr, stream);
         return;
     }
-    pass32<true, kSrcZ, kEpiCandidates>(ks[0], d_jobs, njobs, n, n, d_tw, d_sums, tiles, slack, stream);
+    pass32<true, kSrcHalf, kEpiCandidates>(ks[0], d_jobs, njobs, h, h, d_tw, d_sums, tiles, slack, stream);
     hipLaunchKernelGGL(k_lag32_select, dim3(njobs), dim3(256), 0, stream, d_jobs, (const TileCands*)tiles,
                        (int)l.wg_last, (const float*)slack, cap < kMaxLagCands ? cap : kMaxLagCands, count, cands);
     hipLaunchKernelGGL(k_lag32_rescore, dim3((unsigned)l.nseg, njobs), dim3(kRescoreThreads), 0, stream, d_jobs,
