@@ -148,9 +148,10 @@ void launch_planes(const PlaneJob* d_jobs, int njobs, int nb, int patch, int max
 }
 
 // ---------------------------------------------------------------- similarity table
-// Ref-patch terms (nsim.zig:318-329): mu_ref, mu_ref², conv(ref²) - mu_ref²
-// (fast: its signed square root), with the patch's own replicate padding,
-// and the patch's values; terms[(p*4 + t)*nb*S + r*S + c].
+// Ref-patch terms (nsim.zig:318-329): mu_ref, mu_ref² (fast: mu_ref² + c1,
+// the sum every cell forms), conv(ref²) - mu_ref² (fast: its signed square
+// root), with the patch's own replicate padding, and the patch's values;
+// terms[(p*4 + t)*nb*S + r*S + c].
 template <bool FAST>
 __global__ __launch_bounds__(256) void k_ref_terms(const SimJob* __restrict__ jobs, int nb, int S, double* __restrict__ terms,
                                                    int max_np) {
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(256) void k_ref_terms(const SimJob* __restrict__ jo
     const double rms = mu * mu;
     double* t = terms + ((long long)blockIdx.z * max_np + p) * 4 * nb * S;
     t[e] = mu;
-    t[nb * S + e] = rms;
+    t[nb * S + e] = FAST ? rms + nsim_c1() : rms;
     t[2 * nb * S + e] = FAST ? signed_sqrt(crs - rms) : crs - rms;
     t[3 * nb * S + e] = ref[r * st + c];
 }
@@ -177,14 +178,15 @@ __global__ __launch_bounds__(256) void k_ref_terms(const SimJob* __restrict__ jo
 // One similarity cell in the throughput mode: simCell's value (nsim.zig:
 // 293-308) from the signed roots of the two variances (their product is
 // ±sqrt(|srs·sds|)), with fused multiply-adds and one reciprocal refined by
-// two Newton steps in place of the two divisions.
-__device__ inline double sim_cell_fast(double mr, double rms, double ssr, double md, double ssd, double crd) {
+// two Newton steps in place of the two divisions.  rmsc1 = mu_ref² + c1;
+// crd3 = the windowed covariance term plus c3 (accumulated from c3).
+__device__ inline double sim_cell_fast(double mr, double rmsc1, double ssr, double md, double ssd, double crd3) {
     const double c1 = nsim_c1(), c3 = nsim_c3();
     const double mrd = mr * md;
-    const double num = __builtin_fma(2.0, mrd, c1) * ((crd - mrd) + c3);
+    const double num = __builtin_fma(2.0, mrd, c1) * (crd3 - mrd);
     const double t = ssr * ssd;
     const double den = fmax(t, 0.0) + c3;  // the same value as (t < 0 ? 0 : t) + c3 for every non-NaN t
-    const double d = __builtin_fma(md, md, rms + c1) * den;
+    const double d = __builtin_fma(md, md, rmsc1) * den;
     double q = __builtin_amdgcn_rcp(d);
     double err = __builtin_fma(-d, q, 1.0);
     q = __builtin_fma(q, err, q);
@@ -207,7 +209,10 @@ constexpr int kSimThreads = 512;
 // group's patches, which see the same degraded columns at the same offset.
 // Each (patch, offset) keeps its own sum in row-major order.  Exact: the
 // reference's tap order and serial sum, every product rounded on its own.
-// FAST: the same cells with fused multiply-adds.
+// FAST: per column the products keep only T = X[rm] + X[rp] (one FMA) and
+// M = X[r]; the window's weights (corner WC, edge WE, middle WM) then give
+// c3 + WC·(T[c-1] + T[c+1]) + WE·(T[c] + M[c-1] + M[c+1]) + WM·M[c]:
+// six operations a cell instead of nine, with fused multiply-adds.
 template <bool FAST>
 __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restrict__ jobs, int nb, int S,
                                                            const double* __restrict__ terms, int max_np) {
@@ -265,16 +270,21 @@ __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restr
 #pragma unroll
             for (int q = 0; q < PP; q++) {
                 const double* f = sh + (q * 4 + 3) * cells;
-                Xd[q][0] = f[rm * S + col] * d0;
-                Xd[q][1] = f[r * S + col] * d1;
-                Xd[q][2] = f[rp * S + col] * d2;
+                if (FAST) {
+                    Xd[q][0] = __builtin_fma(f[rp * S + col], d2, f[rm * S + col] * d0);  // T
+                    Xd[q][1] = f[r * S + col] * d1;                                       // M
+                } else {
+                    Xd[q][0] = f[rm * S + col] * d0;
+                    Xd[q][1] = f[r * S + col] * d1;
+                    Xd[q][2] = f[rp * S + col] * d2;
+                }
             }
         };
         fill(X[1], 0);
 #pragma unroll
         for (int q = 0; q < PP; q++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) X[0][q][k] = X[1][q][k];
+            for (int k = 0; k < (FAST ? 2 : 3); k++) X[0][q][k] = X[1][q][k];
         fill(X[2], S > 1 ? 1 : 0);
         auto cell = [&](auto kind, int c, double (&Xa)[PP][3], double (&Xb)[PP][3], double (&Xc)[PP][3]) {
             constexpr int K = decltype(kind)::value;
@@ -287,15 +297,11 @@ __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restr
             for (int q = 0; q < PP; q++) {
                 double crd;
                 if (FAST) {
-                    crd = VQ_WC * Xa[q][0];
-                    crd = __builtin_fma(VQ_WE, Xb[q][0], crd);
-                    crd = __builtin_fma(VQ_WC, Xc[q][0], crd);
-                    crd = __builtin_fma(VQ_WE, Xa[q][1], crd);
-                    crd = __builtin_fma(VQ_WM, Xb[q][1], crd);
-                    crd = __builtin_fma(VQ_WE, Xc[q][1], crd);
-                    crd = __builtin_fma(VQ_WC, Xa[q][2], crd);
-                    crd = __builtin_fma(VQ_WE, Xb[q][2], crd);
-                    crd = __builtin_fma(VQ_WC, Xc[q][2], crd);
+                    const double corners = Xa[q][0] + Xc[q][0];
+                    const double edges = Xb[q][0] + (Xa[q][1] + Xc[q][1]);
+                    crd = __builtin_fma(VQ_WM, Xb[q][1], nsim_c3());
+                    crd = __builtin_fma(VQ_WE, edges, crd);
+                    crd = __builtin_fma(VQ_WC, corners, crd);
                 } else {
                     crd = VQ_WC * Xa[q][0] + VQ_WE * Xb[q][0] + VQ_WC * Xc[q][0] + VQ_WE * Xa[q][1] +
                           VQ_WM * Xb[q][1] + VQ_WE * Xc[q][1] + VQ_WC * Xa[q][2] + VQ_WE * Xb[q][2] +

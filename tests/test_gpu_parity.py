@@ -72,12 +72,18 @@ def test_spectrogram_exact_bit_identical(native, vo, mode, sr, n):
     assert np.array_equal(got, want)
 
 
-def test_spectrogram_fast_close(native, vo):
-    say("spectrogram fast 48k")
-    rng = np.random.default_rng(5)
-    x = rng.uniform(-1, 1, 3840 + 960 * 40)
-    got = native.spectrogram(x, 48000, 0, "fast")
-    want = vo.spectrogram(x, 48000, 32)
+@pytest.mark.parametrize("mode,sr", [(0, 48000), (1, 16000)])
+def test_spectrogram_fast_close(native, vo, mode, sr):
+    """The throughput spectrogram (fused cascade below band 8, FIR +
+    all-pole above) against the oracle's, audio at 48 kHz and speech at
+    16 kHz (21 bands up to 8 kHz)."""
+    say(f"spectrogram fast mode={mode} sr={sr}")
+    rng = np.random.default_rng(5 + mode)
+    window, hop = (3840, 960) if sr == 48000 else (1280, 320)
+    x = rng.uniform(-1, 1, window + hop * 40)
+    got = native.spectrogram(x, sr, mode, "fast")
+    want = vo.spectrogram(x, sr, 32 if mode == 0 else 21, None if mode == 0 else 8000.0)
+    assert got.shape == want.shape
     rel = np.max(np.abs(got - want) / np.maximum(want, 1e-300))
     say("fast spectrogram max rel err", rel)
     assert rel < 1e-9
