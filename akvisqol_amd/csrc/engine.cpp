@@ -119,6 +119,11 @@ Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
         const long v = strtol(e, &end, 10);
         if (end != e && *end == 0 && v >= 0 && v <= kMaxLagCands) opts_.lag_cap = (int)v;
     }
+    if (const char* e = getenv("VISQOL_HIP_SCORERS")) {
+        char* end = nullptr;
+        const long v = strtol(e, &end, 10);
+        if (end != e && *end == 0 && v >= 2 && v <= 4) opts_.scorers = (int)v;
+    }
     if (const char* e = getenv("VISQOL_HIP_GAMMATONE_SHAPE")) {
         char* end = nullptr;
         const long v = strtol(e, &end, 10);
@@ -147,7 +152,7 @@ void Engine::merge_profile(const Profile& p) {
 }
 
 Engine::~Engine() {
-    helper_.reset();  // before the stream they share
+    helpers_.clear();  // before the stream they share
     uploader_.reset();
     for (double* p : slots_)
         if (p) (void)hipFree(p);
@@ -492,19 +497,21 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
                       grads ? grads + c.first : nullptr);
         return;
     }
-    // Odd chunks on the helper's thread, even ones here; both enqueue to the
-    // one in-order stream, so kernels still run one at a time while either
-    // side's host work proceeds.
-    if (!helper_) helper_.reset(new Engine(HelperTag{}, *this));
-    helper_->profiling_ = profiling_;
+    // Chunk k on scorer k mod S (scorer 0 this thread, the others helper
+    // engines on threads of their own); all enqueue to the one in-order
+    // stream, so kernels still run one at a time while the others' host work
+    // proceeds.
+    const size_t nscore = (size_t)std::max(2, std::min(4, opts_.scorers));
+    while (helpers_.size() + 1 < nscore) helpers_.emplace_back(new Engine(HelperTag{}, *this));
+    for (auto& h : helpers_) h->profiling_ = profiling_;
     int dev = 0;  // the calling thread's device, made current on every thread here
     check(hipGetDevice(&dev), "hipGetDevice");
 
-    // Host-resident signals: a third thread uploads chunk after chunk into a
-    // ring of kSlots device buffers ahead of the two scoring threads, so the
+    // Host-resident signals: one more thread uploads chunk after chunk into a
+    // ring of kSlots device buffers ahead of the scoring threads, so the
     // link carries the next chunks while earlier ones are scored.  Slot
     // k % kSlots holds chunk k from its upload until its scoring returns.
-    constexpr size_t kSlots = 3;
+    const size_t kSlots = nscore + 1;
     std::vector<std::vector<PairInput>> staged(host_side ? chunks.size() : 0);
     std::mutex mu;
     std::condition_variable cv;
@@ -522,7 +529,7 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
             }
             need = std::max(need, sz);
         }
-        if (need > slot_len_) {
+        if (need > slot_len_ || slots_.size() != kSlots) {
             for (double*& p : slots_) {
                 if (p) (void)hipFree(p);
                 p = nullptr;
@@ -612,29 +619,33 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
         busy[k % kSlots] = 0;
         cv.notify_all();
     };
-    std::exception_ptr helper_error;
-    std::thread t([&] {
-        (void)hipSetDevice(dev);
-        try {
-            for (size_t k = 1; k < chunks.size(); k += 2) score(*helper_, k);
-        } catch (...) {
-            helper_error = std::current_exception();
-        }
-    });
+    std::vector<std::exception_ptr> helper_error(nscore - 1);
+    std::vector<std::thread> ts;
+    for (size_t h = 1; h < nscore; h++)
+        ts.emplace_back([&, h] {
+            (void)hipSetDevice(dev);
+            try {
+                for (size_t k = h; k < chunks.size(); k += nscore) score(*helpers_[h - 1], k);
+            } catch (...) {
+                helper_error[h - 1] = std::current_exception();
+            }
+        });
     std::exception_ptr own_error;
     try {
-        for (size_t k = 0; k < chunks.size(); k += 2) score(*this, k);
+        for (size_t k = 0; k < chunks.size(); k += nscore) score(*this, k);
     } catch (...) {
         own_error = std::current_exception();
     }
-    t.join();
+    for (auto& t : ts) t.join();
     if (up.joinable()) up.join();
-    if (profiling_) {
-        merge_profile(helper_->prof_);
-        helper_->prof_ = Profile();
-    }
+    if (profiling_)
+        for (auto& h : helpers_) {
+            merge_profile(h->prof_);
+            h->prof_ = Profile();
+        }
     if (own_error) std::rethrow_exception(own_error);
-    if (helper_error) std::rethrow_exception(helper_error);
+    for (auto& e : helper_error)
+        if (e) std::rethrow_exception(e);
     if (upload_error) std::rethrow_exception(upload_error);
 }
 

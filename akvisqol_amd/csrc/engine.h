@@ -26,10 +26,14 @@ struct EngineOptions {
     bool no_realign = false;
     bool lattice = true;
     bool fast = false;
-    // Batches of several chunks: a second engine on the same stream scores
-    // every other chunk from its own host thread, so one chunk's host work
-    // (planning, decoding, aggregation) overlaps the other's kernels.
+    // Batches of several chunks: `scorers` engines on the same stream, each
+    // on its own host thread, take the chunks in turn, so one chunk's host
+    // work (planning, decoding, aggregation) overlaps the others' kernels
+    // (VISQOL_HIP_SCORERS, 2-4, read when an engine is created).  Two
+    // measured best on 1024 ten-second pairs (8300 pairs/s against 8100
+    // with three and 7840 with four).
     bool interleave_chunks = true;
+    int scorers = 2;
     // Throughput mode: rows of the lag search with more nominated lags than
     // this (at most kMaxLagCands) take the f64 search instead; 0 sends every
     // row there (VISQOL_HIP_LAG_CAP, read when an engine is created).
@@ -212,12 +216,12 @@ class Engine {
 
     void set_profiling(bool on) {
         profiling_ = on;
-        if (helper_) helper_->set_profiling(on);
+        for (auto& h : helpers_) h->set_profiling(on);
     }
     Profile profile() const { return prof_; }
     void reset_profile() {
         prof_ = Profile();
-        if (helper_) helper_->reset_profile();
+        for (auto& h : helpers_) h->reset_profile();
     }
 
    private:
@@ -225,7 +229,7 @@ class Engine {
     // stream and enqueue lock; owns its device arena, staging and tables.
     struct HelperTag {};
     Engine(HelperTag, Engine& parent);
-    std::unique_ptr<Engine> helper_;
+    std::vector<std::unique_ptr<Engine>> helpers_;  // scorers - 1 of them, made on first use
     // Uploads host-resident chunks ahead of scoring into slots_ (device
     // buffers of slot_len_ samples each, kept between calls).
     std::unique_ptr<Engine> uploader_;
