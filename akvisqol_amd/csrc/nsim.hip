@@ -6,6 +6,7 @@
 // measurePatchSimilarity), dp.zig:205-355 (DP + backtrack),
 // accel.zig:126-146 (table over the search windows).
 #include <float.h>
+#include <limits.h>
 
 #include <type_traits>
 
@@ -374,11 +375,35 @@ struct ArgMax {
 };
 __device__ inline ArgMax later_max(ArgMax a, ArgMax b) { return b.v >= a.v ? b : a; }
 
+// Exclusive scan of one ArgMax per thread of a 256-thread block under
+// later_max, in thread order: shuffles within each wave, then the waves'
+// totals through LDS.  later_max is associative (max never rounds; ties go
+// to the later operand), so the result is the serial left-to-right one.
+__device__ inline ArgMax block_exclusive_later_max(ArgMax x, double* wv, int* wi) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double v = __shfl_up(x.v, off, 64);
+        const int i = __shfl_up(x.i, off, 64);
+        if (lane >= off) x = later_max(ArgMax{v, i}, x);
+    }
+    if (lane == 63) {
+        wv[wave] = x.v;
+        wi[wave] = x.i;
+    }
+    const double ev = __shfl_up(x.v, 1, 64);
+    const int ei = __shfl_up(x.i, 1, 64);
+    __syncthreads();
+    ArgMax pre = {-INFINITY, -1};
+    for (int w = 0; w < wave; w++) pre = later_max(pre, ArgMax{wv[w], wi[w]});
+    return lane == 0 ? pre : later_max(pre, ArgMax{ev, ei});
+}
+
 __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
     const DpJob j = jobs[blockIdx.x];
     const int cols = j.nf_deg;
-    __shared__ double agg_v[256];
-    __shared__ int agg_i[256];
+    __shared__ double wv[4];
+    __shared__ int wi[4];
     for (int p = 0; p < j.np; p++) {
         const int lo = j.lo[p], hi = j.hi[p];
         double* row = j.dp + (long long)p * cols;
@@ -398,20 +423,7 @@ __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
             const int b = min(a + per, hi + 1);
             ArgMax acc = {-INFINITY, -1};
             for (int t = a; t < b; t++) acc = later_max(acc, ArgMax{prev[t], t});
-            agg_v[threadIdx.x] = acc.v;
-            agg_i[threadIdx.x] = acc.i;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                ArgMax run = {-INFINITY, -1};
-                for (int k = 0; k < (int)blockDim.x; k++) {
-                    const ArgMax cur = {agg_v[k], agg_i[k]};
-                    agg_v[k] = run.v;
-                    agg_i[k] = run.i;
-                    run = later_max(run, cur);
-                }
-            }
-            __syncthreads();
-            ArgMax run = {agg_v[threadIdx.x], agg_i[threadIdx.x]};
+            ArgMax run = block_exclusive_later_max(acc, wv, wi);
             for (int t = a; t < b; t++) {
                 if (t >= lo) {
                     double sim = srow[t] + run.v;
@@ -427,18 +439,36 @@ __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
                 run = later_max(run, ArgMax{prev[t], t});
             }
         }
-        __syncthreads();
+        __syncthreads();  // the row is complete (and wv free) before the next reads it
     }
+    // first maximum of the last row (strictly greater wins), over
+    // contiguous per-thread runs combined in thread order
+    const int last = j.np - 1;
+    const double* lrow = j.dp + (long long)last * cols;
+    const int llo = j.lo[last], lhi = j.hi[last];
+    const int lm = lhi - llo + 1;
+    const int lper = (lm + blockDim.x - 1) / blockDim.x;
+    const int la = llo + threadIdx.x * lper;
+    const int lb = min(la + lper, lhi + 1);
+    ArgMax best = {-INFINITY, INT_MAX};
+    for (int s = la; s < lb; s++)
+        if (lrow[s] > best.v) best = ArgMax{lrow[s], s};
+    for (int off = 32; off > 0; off >>= 1) {
+        const double v = __shfl_down(best.v, off, 64);
+        const int i = __shfl_down(best.i, off, 64);
+        // the partner covers later elements: it wins only when strictly greater
+        if (v > best.v || (v == best.v && i < best.i)) best = ArgMax{v, i};
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wv[threadIdx.x >> 6] = best.v;
+        wi[threadIdx.x >> 6] = best.i;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const int last = j.np - 1;
-        const double* row = j.dp + (long long)last * cols;
-        double best = -INFINITY;
-        int cur = 0;
-        for (int s = j.lo[last]; s <= j.hi[last]; s++)
-            if (row[s] > best) {
-                best = row[s];
-                cur = s;
-            }
+        ArgMax r = {wv[0], wi[0]};
+        for (int w = 1; w < 4; w++)
+            if (wv[w] > r.v) r = ArgMax{wv[w], wi[w]};
+        int cur = r.i == INT_MAX ? 0 : r.i;
         for (int p = 0; p < j.np; p++) {
             j.offsets[p] = 0;
             j.state[p] = 0;
