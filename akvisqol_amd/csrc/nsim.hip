@@ -399,53 +399,130 @@ __device__ inline ArgMax block_exclusive_later_max(ArgMax x, double* wv, int* wi
     return lane == 0 ? pre : later_max(pre, ArgMax{ev, ei});
 }
 
+// Rows whose scan range (lo of the row before .. hi) spans at most kDpLds
+// frames keep the previous DP row and the current similarity row in LDS,
+// in rings indexed by frame mod kDpLds: the similarity row comes in with
+// coalesced loads, the scan and update read LDS, and the DP row and the
+// backtrack pointers still go to global memory (the backtrack and the last
+// row's search read them there).  Longer rows (search radii far above the
+// default) read and write global memory only.
+constexpr int kDpLds = 4096;
+
 __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
     const DpJob j = jobs[blockIdx.x];
     const int cols = j.nf_deg;
+    const auto* wlo = as_global(j.lo);
+    const auto* whi = as_global(j.hi);
     __shared__ double wv[4];
     __shared__ int wi[4];
+    __shared__ double ring[2][kDpLds];
+    __shared__ int s_span;
+    constexpr int kMask = kDpLds - 1;
+    // widest scan range of the job: every row fits the rings or none uses them
+    if (threadIdx.x == 0) s_span = 0;
+    __syncthreads();
+    {
+        int span = 0;
+        for (int p = threadIdx.x; p < j.np; p += blockDim.x) {
+            const int from = p == 0 ? wlo[0] : min(wlo[p - 1], wlo[p]);
+            span = max(span, whi[p] - from + 1);
+        }
+        atomicMax(&s_span, span);
+    }
+    __syncthreads();
+    const bool in_lds = s_span <= kDpLds;
+    // LDS rows: each thread's share of the next similarity row is loaded
+    // into registers while the current row is scanned
+    constexpr int kPre = kDpLds / 256;
+    double pre[kPre];
+    // window bounds run one row ahead, so their loads overlap a row's work
+    int lo_prev = 0, hi_prev = 0, lo_cur = wlo[0], hi_cur = whi[0];
+    int lo_next = j.np > 1 ? wlo[1] : 0, hi_next = j.np > 1 ? whi[1] : 0;
+    auto prefetch = [&](int q, int qlo, int qhi) {
+        if (!in_lds || q >= j.np) return;
+        const auto* sr = as_global(j.sims + (long long)q * cols);
+#pragma unroll
+        for (int k = 0; k < kPre; k++) {
+            const int s = qlo + (int)threadIdx.x + k * 256;
+            pre[k] = s <= qhi ? sr[s] : 0.0;
+        }
+    };
+    prefetch(1, lo_next, hi_next);
     for (int p = 0; p < j.np; p++) {
-        const int lo = j.lo[p], hi = j.hi[p];
-        double* row = j.dp + (long long)p * cols;
-        int* brow = j.bt + (long long)p * cols;
-        const double* srow = j.sims + (long long)p * cols;
+        const int lo = lo_cur, hi = hi_cur;
+        const int lo_after = p + 2 < j.np ? wlo[p + 2] : 0, hi_after = p + 2 < j.np ? whi[p + 2] : 0;
+        // global views, so these are global (not flat) memory instructions
+        auto* row = as_global(j.dp + (long long)p * cols);
+        auto* brow = as_global(j.bt + (long long)p * cols);
+        const auto* srow = as_global(j.sims + (long long)p * cols);
+        double* cur = ring[p & 1];         // this row: similarities, then DP values
+        const double* prv = ring[(p + 1) & 1];  // the previous row's DP values
         if (p == 0) {
             for (int s = lo + threadIdx.x; s <= hi; s += blockDim.x) {
-                row[s] = srow[s];
+                const double v = srow[s];
+                row[s] = v;
                 brow[s] = -1;
+                if (in_lds) cur[s & kMask] = v;
             }
         } else {
-            const double* prev = j.dp + (long long)(p - 1) * cols;
-            const int t0 = j.lo[p - 1];
+            const auto* prev = as_global(j.dp + (long long)(p - 1) * cols);
+            const int t0 = lo_prev;
             const int m = hi - t0 + 1;  // scan elements t0..hi
-            const int per = (m + blockDim.x - 1) / blockDim.x;
-            const int a = t0 + threadIdx.x * per;
-            const int b = min(a + per, hi + 1);
-            ArgMax acc = {-INFINITY, -1};
-            for (int t = a; t < b; t++) acc = later_max(acc, ArgMax{prev[t], t});
-            ArgMax run = block_exclusive_later_max(acc, wv, wi);
-            for (int t = a; t < b; t++) {
-                if (t >= lo) {
-                    double sim = srow[t] + run.v;
-                    int past = run.i;
-                    const double no_move = prev[t];
-                    if (no_move > sim) {
-                        sim = no_move;
-                        past = t;
-                    }
-                    row[t] = sim;
-                    brow[t] = past < 0 ? -1 : past;
+            if (in_lds) {
+                // this row's similarities (prefetched); the previous row is
+                // zero past its hi
+                const int phi = hi_prev;
+#pragma unroll
+                for (int k = 0; k < kPre; k++) {
+                    const int s = lo + (int)threadIdx.x + k * 256;
+                    if (s <= hi) cur[s & kMask] = pre[k];
                 }
-                run = later_max(run, ArgMax{prev[t], t});
+                for (int s = phi + 1 + threadIdx.x; s <= hi; s += blockDim.x) ring[(p + 1) & 1][s & kMask] = 0.0;
+                __syncthreads();
+                prefetch(p + 1, lo_next, hi_next);
             }
+            // one body per source, so LDS reads stay LDS instructions
+            auto body = [&](auto prev_at, auto sim_at) {
+                const int per = (m + blockDim.x - 1) / blockDim.x;
+                const int a = t0 + threadIdx.x * per;
+                const int b = min(a + per, hi + 1);
+                ArgMax acc = {-INFINITY, -1};
+                for (int t = a; t < b; t++) acc = later_max(acc, ArgMax{prev_at(t), t});
+                ArgMax run = block_exclusive_later_max(acc, wv, wi);
+                for (int t = a; t < b; t++) {
+                    const double pv = prev_at(t);
+                    if (t >= lo) {
+                        double sim = sim_at(t) + run.v;
+                        int past = run.i;
+                        if (pv > sim) {
+                            sim = pv;
+                            past = t;
+                        }
+                        row[t] = sim;
+                        brow[t] = past < 0 ? -1 : past;
+                        if (in_lds) cur[t & kMask] = sim;
+                    }
+                    run = later_max(run, ArgMax{pv, t});
+                }
+            };
+            if (in_lds)
+                body([&](int t) { return prv[t & kMask]; }, [&](int t) { return cur[t & kMask]; });
+            else
+                body([&](int t) { return prev[t]; }, [&](int t) { return srow[t]; });
         }
         __syncthreads();  // the row is complete (and wv free) before the next reads it
+        lo_prev = lo_cur;
+        hi_prev = hi_cur;
+        lo_cur = lo_next;
+        hi_cur = hi_next;
+        lo_next = lo_after;
+        hi_next = hi_after;
     }
     // first maximum of the last row (strictly greater wins), over
     // contiguous per-thread runs combined in thread order
     const int last = j.np - 1;
-    const double* lrow = j.dp + (long long)last * cols;
-    const int llo = j.lo[last], lhi = j.hi[last];
+    const auto* lrow = as_global(j.dp + (long long)last * cols);
+    const int llo = wlo[last], lhi = whi[last];
     const int lm = lhi - llo + 1;
     const int lper = (lm + blockDim.x - 1) / blockDim.x;
     const int la = llo + threadIdx.x * lper;
