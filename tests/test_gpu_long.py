@@ -4,6 +4,8 @@
   per patch) and one 60 s 16 kHz speech pair: exact mode bit-identical to
   the oracle on every decision and score, fast mode within |dMOS| <= 1e-6
   with the same alignments.
+* A 90 s pair with search radius 80, whose DP rows exceed the DP kernel's
+  LDS rings (the global-memory path): exact bit-identical, fast within 1e-6.
 * Prepared references in exact mode: bit-identical to the oracle on the
   same (reference, degraded) pairs.
 * The server under Poisson arrivals (above its capacity) of a few hundred
@@ -67,6 +69,28 @@ def test_long_pair_both_modes(case):
     _bits_equal(res, ores)
     fres, ftr = native.Context(mode, precision="fast").measure_trace(ref, deg, sr)
     say(f"  fast MOS {fres['moslqo']!r} |dMOS| {abs(fres['moslqo'] - ores['moslqo']):.3g}")
+    _trace_equal(ftr, otr)
+    assert abs(fres["moslqo"] - ores["moslqo"]) <= FAST_MOS_TOL
+
+
+def test_wide_search_window_exact_and_fast():
+    """A 90 s audio pair searched with radius 80 (windows of up to 4801
+    frames): the DP's scan ranges exceed its 4096-frame LDS rings, so every
+    row takes the global-memory path.  Exact mode bit-identical to the
+    oracle, fast mode within 1e-6 with the same alignments."""
+    from akvisqol_amd import native, synth
+    from oracle import vo
+
+    ref, deg = synth.audio_pair(79, duration=90.0, delay=4321, snr_db=18.0)
+    say("wide window: oracle (single thread) ...")
+    t = time.perf_counter()
+    ores, otr = vo.Oracle(mode=0, search_window_radius=80).measure(ref, deg, 48000)
+    say(f"  oracle {time.perf_counter() - t:.1f} s, MOS {ores['moslqo']!r}, {otr['num_patches']} patches")
+    res, tr = native.Context(0, precision="exact", search_window_radius=80).measure_trace(ref, deg, 48000)
+    _trace_equal(tr, otr)
+    _bits_equal(res, ores)
+    fres, ftr = native.Context(0, precision="fast", search_window_radius=80).measure_trace(ref, deg, 48000)
+    say(f"  fast |dMOS| {abs(fres['moslqo'] - ores['moslqo']):.3g}")
     _trace_equal(ftr, otr)
     assert abs(fres["moslqo"] - ores["moslqo"]) <= FAST_MOS_TOL
 
