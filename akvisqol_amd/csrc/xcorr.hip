@@ -17,6 +17,8 @@
 // pass keeps no output, only each workgroup's first maximum in scan order.
 #include <float.h>
 
+#include <type_traits>
+
 #include "dev_common.h"
 #include "kernels.h"
 
@@ -46,6 +48,15 @@ template <typename C>
 __device__ inline C cmul_conj(C x, C w) {
     const typename Cx<C>::R wi = -w.y;
     return Cx<C>::make(x.x * w.x - x.y * wi, x.x * wi + x.y * w.x);
+}
+// The nominating transform's f32 products with fused multiply-adds (its
+// error bound covers the separately rounded form, so it covers these); the
+// f64 transform keeps the reference's separate roundings above.
+__device__ inline float2 cmul(float2 x, float2 w) {
+    return make_float2(__builtin_fmaf(x.x, w.x, -(x.y * w.y)), __builtin_fmaf(x.x, w.y, x.y * w.x));
+}
+__device__ inline float2 cmul_conj(float2 x, float2 w) {
+    return make_float2(__builtin_fmaf(x.x, w.x, x.y * w.y), __builtin_fmaf(x.y, w.x, -(x.x * w.y)));
 }
 
 __device__ inline long long env_fft_size(long long l) {
@@ -95,9 +106,7 @@ constexpr int kBfPerThread = kTile / 2 / kThreads;
 // the nominating transform's half-length inverse (half position h at
 // h + (h & ~(kHalfChunk32 - 1)), value c[2h] + i·c[2h + 1]).
 enum : int { kSrcZ = 0, kSrcEnvelope = 1, kSrcHalf = 2 };
-constexpr long long kHalfChunk32 = 2048;
-__device__ inline long long half_addr(long long h) { return lag32_xcorr_slot(h); }
-static_assert(kHalfChunk32 == 2048, "lag32_xcorr_slot's chunk");
+constexpr long long kHalfChunk32 = 2048;  // lag32_xcorr_slot's chunk (kernels.h)
 
 // Scan position of circular index i (xcorr.zig:233-250), or -1 outside
 // the lag range.
@@ -427,17 +436,19 @@ __device__ inline void tile_candidates(const float2* sm, GIDX gidx, long long n,
                                        TileCands* out) {
     constexpr int kPer = 8;
     constexpr int kVals = HALF ? 2 : 1;
-    const long long big = job.la > job.lb ? job.la : job.lb;
-    const long long max_lag = big - 1;
+    // 32-bit scan positions (n <= 2^30): scan_order's two ranges
+    const int max_lag = (int)((job.la > job.lb ? job.la : job.lb) - 1);
+    const int neg0 = (int)n - max_lag;  // first index of the negative lags
+    auto order = [&](int j) -> int { return j >= neg0 ? j - neg0 : (j <= max_lag ? max_lag + j : -1); };
     auto value = [&](int e, int h) -> float { return fabsf(h ? sm[e].y : sm[e].x); };
-    auto index = [&](int e, int h) -> long long { return HALF ? 2 * gidx(e) + h : gidx(e); };
+    auto index = [&](int e, int h) -> int { return HALF ? 2 * (int)gidx(e) + h : (int)gidx(e); };
     unsigned top_bits = 0;
 #pragma unroll
     for (int u = 0; u < kPer; u++) {
         const int e = threadIdx.x + u * THREADS;
 #pragma unroll
         for (int h = 0; h < kVals; h++)
-            if (scan_order(index(e, h), n, max_lag) >= 0) top_bits = max(top_bits, __float_as_uint(value(e, h)));
+            if (order(index(e, h)) >= 0) top_bits = max(top_bits, __float_as_uint(value(e, h)));
     }
     __shared__ unsigned s_top;
     __shared__ int s_count;
@@ -456,12 +467,13 @@ __device__ inline void tile_candidates(const float2* sm, GIDX gidx, long long n,
         const int e = threadIdx.x + u * THREADS;
 #pragma unroll
         for (int h = 0; h < kVals; h++) {
-            const long long so = scan_order(index(e, h), n, max_lag);
             const float v = value(e, h);
-            if (so < 0 || !(v >= floor)) continue;
+            if (!(v >= floor)) continue;
+            const int so = order(index(e, h));
+            if (so < 0) continue;
             const int slot = atomicAdd(&s_count, 1);
             if (slot < kTileCands) {
-                out->pos[slot] = (int)so;
+                out->pos[slot] = so;
                 out->val[slot] = v;
             }
         }
@@ -475,12 +487,14 @@ __device__ inline void tile_candidates(const float2* sm, GIDX gidx, long long n,
 
 // Workgroups per CU: f64 tiles at least two (TILE / 1024 waves per SIMD;
 // 4096-element tiles of 8 waves at most 128 VGPRs); f32 tiles of 8192
-// elements (64 KiB, 16 waves) one, within 128 VGPRs: held to 64 VGPRs for
-// two per CU, the envelope and candidate passes spill 70-160 bytes per
-// lane and run 1.3-2.3x slower (measured on the 279 ten-second pairs).
+// elements (64 KiB, 16 waves) two, within 64 VGPRs.  With fused complex
+// products and 32-bit row positions the f32 passes need 51-67 VGPRs, and
+// two workgroups per CU took one 279-pair chunk's passes from 7.4 to 6.7 ms
+// (the candidate passes 0.78 -> 0.52 and 0.60 -> 0.43 ms); the earlier
+// 105-120 VGPR form spilled 70-160 bytes per lane under the same cap.
 template <typename C, int TILE>
 struct TileWaves {
-    static constexpr int value = sizeof(C) == 16 ? TILE / 1024 : TILE / 2048;
+    static constexpr int value = TILE / 1024;
 };
 
 // SRC == kSrcEnvelope builds the tile from the two signals' envelopes
@@ -508,17 +522,19 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(TileWa
     const int lgPer = lgq - lgR;
     const long long blk0 = (wg >> lgPer) << lgBn;
     const long long r0 = (wg & ((1LL << lgPer) - 1)) << lgR;
-    const long long base = (blk0 << lgS) + r0;
+    // row positions: 32-bit for the nominating transform (n <= 2^30)
+    using Idx = typename std::conditional<sizeof(C) == 8, int, long long>::type;
+    const Idx base = (Idx)((blk0 << lgS) + r0);
     C* z = reinterpret_cast<C*>(job.z);
-    auto gidx = [&](int e) -> long long {
+    auto gidx = [&](int e) -> Idx {
         const int bb = e >> lgGR;
         const int m = (e >> lgR) & ((1 << K) - 1);
         const int rr = e & ((1 << lgR) - 1);
-        return base + ((long long)bb << lgS) + ((long long)m << lgq) + rr;
+        return base + ((Idx)bb << lgS) + ((Idx)m << lgq) + rr;
     };
     // where row position g is stored
-    auto addr = [&](long long g) -> long long {
-        if constexpr (SRC == kSrcHalf) return half_addr(g);
+    auto addr = [&](Idx g) -> Idx {
+        if constexpr (SRC == kSrcHalf) return g + (g & ~(Idx)(kHalfChunk32 - 1));
         else return g;
     };
     constexpr int kPer = 8;
