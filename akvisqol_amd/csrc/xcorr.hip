@@ -702,6 +702,55 @@ __device__ inline void group_layout(int lgH, int (&idx)[N], int (&lo)[N >> B]) {
     }
 }
 
+// The nominating transform's steps of smallest distance (lgH = 0), whose
+// twiddles are W_2^0, W_4^0..1 and W_8^0..3: as constants, 1 and -i exact
+// (the table's W_4^1 has a 6e-17 real part) and W_8^1, W_8^3 = (±r, -r)
+// with r = cos(pi/4) rounded to f32.  The same butterflies as radix_step
+// with lgH = 0, the trivial products left out.
+constexpr float kR8 = 0.70710677f;
+__device__ inline float2 mul_neg_i(float2 t) { return make_float2(t.y, -t.x); }  // t·(-i)
+__device__ inline float2 mul_pos_i(float2 t) { return make_float2(-t.y, t.x); }  // t·conj(-i)
+__device__ inline void dif8_central(float2 (&x)[8]) {
+    float2 t;
+    t = csub(x[0], x[4]); x[0] = cadd(x[0], x[4]); x[4] = t;
+    t = csub(x[1], x[5]); x[1] = cadd(x[1], x[5]); x[5] = cmul(t, make_float2(kR8, -kR8));
+    t = csub(x[2], x[6]); x[2] = cadd(x[2], x[6]); x[6] = mul_neg_i(t);
+    t = csub(x[3], x[7]); x[3] = cadd(x[3], x[7]); x[7] = cmul(t, make_float2(-kR8, -kR8));
+#pragma unroll
+    for (int b = 0; b < 8; b += 4) {
+        t = csub(x[b], x[b + 2]); x[b] = cadd(x[b], x[b + 2]); x[b + 2] = t;
+        t = csub(x[b + 1], x[b + 3]); x[b + 1] = cadd(x[b + 1], x[b + 3]); x[b + 3] = mul_neg_i(t);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; c += 2) {
+        t = csub(x[c], x[c + 1]); x[c] = cadd(x[c], x[c + 1]); x[c + 1] = t;
+    }
+}
+// DIT butterfly (a, b) -> (a + b·conj(w), a - b·conj(w)) given bw = b·conj(w)
+__device__ inline void dit_with(float2& a, float2& b, float2 bw) {
+    b = csub(a, bw);
+    a = cadd(a, bw);
+}
+__device__ inline void dit8_central(float2 (&x)[8]) {
+#pragma unroll
+    for (int c = 0; c < 8; c += 2) dit_with(x[c], x[c + 1], x[c + 1]);
+#pragma unroll
+    for (int b = 0; b < 8; b += 4) {
+        dit_with(x[b], x[b + 2], x[b + 2]);
+        dit_with(x[b + 1], x[b + 3], mul_pos_i(x[b + 3]));
+    }
+    dit_with(x[0], x[4], x[4]);
+    dit_with(x[1], x[5], cmul_conj(x[5], make_float2(kR8, -kR8)));
+    dit_with(x[2], x[6], mul_pos_i(x[6]));
+    dit_with(x[3], x[7], cmul_conj(x[7], make_float2(-kR8, -kR8)));
+}
+__device__ inline void dit4_central(float2 (&y)[4]) {
+    dit_with(y[0], y[1], y[1]);
+    dit_with(y[2], y[3], y[3]);
+    dit_with(y[0], y[2], y[2]);
+    dit_with(y[1], y[3], mul_pos_i(y[3]));
+}
+
 // The half-size inverse of the nominating transform's middle pass (see
 // k_fft_mid, HALF): x holds the cross terms of the aligned run of eight at
 // row position g0; makes the thread's four Y values, runs the n/2-point
@@ -711,15 +760,15 @@ __device__ inline void group_layout(int lgH, int (&idx)[N], int (&lo)[N >> B]) {
 // radix-4/2 access of 32 lanes then falls on distinct 8-byte columns of a
 // 256-byte bank row, but for a few of the smallest-distance steps.
 template <int K, int THREADS>
-__device__ inline void mid_half_inverse(float2 (&x)[8], long long g0, long long ca, long long cb, int lgn,
+__device__ inline void mid_half_inverse(float2 (&x)[8], int g0, int ca, int cb, int lgn,
                                         float2* sm, float2* z, const float2* __restrict__ tw) {
     constexpr int KH = K - 1;
     constexpr int kQuarter = 1 << KH;  // half-array elements per chunk
     float2 y[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-        const unsigned long long k = __builtin_bitreverse64((unsigned long long)(g0 + 2 * c)) >> (64 - lgn);
-        const float2 w = tw[(1LL << (lgn - 1)) + (long long)k];
+        const unsigned k = __builtin_bitreverse32((unsigned)(g0 + 2 * c)) >> (32 - lgn);
+        const float2 w = tw[(1u << (lgn - 1)) + k];
         const float2 sum = cadd(x[2 * c], x[2 * c + 1]);
         const float2 dif = cmul_conj(csub(x[2 * c], x[2 * c + 1]), w);
         y[c] = make_float2(sum.x - dif.y, sum.y + dif.x);  // sum + i·dif
@@ -729,7 +778,7 @@ __device__ inline void mid_half_inverse(float2 (&x)[8], long long g0, long long 
     {
         int lo[1];
         group_layout<KH, 2, THREADS, 4>(0, hidx, lo);  // the run of eight's four half positions
-        radix_step<2, true>(y, lo, 0, tw);
+        dit4_central(y);
     }
 #pragma unroll
     for (int lgH = 2; lgH < KH; lgH += 2) {
@@ -753,7 +802,7 @@ __device__ inline void mid_half_inverse(float2 (&x)[8], long long g0, long long 
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const int h = hidx[c];
-        store2(z, (h < kQuarter ? ca : cb) * (2LL * kQuarter) + (h & (kQuarter - 1)), y[c]);
+        store2(z, (h < kQuarter ? ca : cb) * (2 * kQuarter) + (h & (kQuarter - 1)), y[c]);
     }
 }
 
@@ -813,14 +862,16 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
     const int jb = (int)(blockIdx.x % (unsigned)njobs);
     const long long wg = blockIdx.x / (unsigned)njobs;
     C* z = reinterpret_cast<C*>(jobs[jb].z);
-    long long ca = 0, cb = 1;
+    // row positions: 32-bit for the nominating transform (n <= 2^30)
+    using Idx = typename std::conditional<sizeof(C) == 8, int, long long>::type;
+    Idx ca = 0, cb = 1;
     if (wg > 0) {
         const int k = 63 - __builtin_clzll((unsigned long long)wg);
-        const long long t = wg - (1LL << k);
-        ca = (2LL << k) + t;
-        cb = (4LL << k) - 1 - t;
+        const Idx t = (Idx)(wg - (1LL << k));
+        ca = ((Idx)2 << k) + t;
+        cb = ((Idx)4 << k) - 1 - t;
     }
-    auto gidx = [&](int e) -> long long { return (e < kHalfTile ? ca : cb) * kHalfTile + (e & (kHalfTile - 1)); };
+    auto gidx = [&](int e) -> Idx { return (e < kHalfTile ? ca : cb) * kHalfTile + (e & (kHalfTile - 1)); };
     // Tile element e sits in LDS at e ^ h(e), h taking bits 0-2 from e's
     // bits 3-5 and bit 3 from bits 5 ^ 6: a permutation within each run of
     // 16 (bit 3 of the result recovers e's bit 3 from bits 5, 6, then bits
@@ -878,16 +929,17 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
         int lo[1];
         group_layout<K, 3, THREADS>(0, idx, lo);
         get();
-        radix_step<3, false>(x, lo, 0, tw);
+        if constexpr (sizeof(C) == 8) dif8_central(x);
+        else radix_step<3, false>(x, lo, 0, tw);
         put();
         __syncthreads();
         C mirror[8];
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            const long long g = gidx(idx[c]);
-            long long pg = g;
+            const Idx g = gidx(idx[c]);
+            Idx pg = g;
             if (g >= 2) {
-                const long long h = 1LL << (63 - __builtin_clzll((unsigned long long)g));
+                const Idx h = (Idx)1 << (63 - __builtin_clzll((unsigned long long)g));
                 pg = 3 * h - 1 - g;
             }
             const int ep = ((pg / kHalfTile) == ca ? 0 : kHalfTile) + (int)(pg & (kHalfTile - 1));
@@ -899,6 +951,8 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
         if constexpr (HALF) {
             mid_half_inverse<K, THREADS>(x, gidx(idx[0]), ca, cb, lgn, sm, z, tw);
             return;
+        } else if constexpr (sizeof(C) == 8) {
+            dit8_central(x);
         } else {
             radix_step<3, true>(x, lo, 0, tw);
         }
