@@ -124,6 +124,10 @@ Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
         const long v = strtol(e, &end, 10);
         if (end != e && *end == 0 && v >= 2 && v <= 4) opts_.scorers = (int)v;
     }
+    if (const char* e = getenv("VISQOL_HIP_SCORER_STREAMS")) opts_.scorer_streams = strcmp(e, "0") != 0;
+    if (const char* e = getenv("VISQOL_HIP_GT_EXCLUSIVE")) opts_.gt_exclusive = strcmp(e, "0") != 0;
+    fence_ = std::make_shared<StreamFence>();
+    fence_->streams.push_back(stream_);
     if (const char* e = getenv("VISQOL_HIP_GAMMATONE_SHAPE")) {
         char* end = nullptr;
         const long v = strtol(e, &end, 10);
@@ -134,8 +138,15 @@ Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
 Engine::Engine(HelperTag, Engine& parent)
     : gt_force_(parent.gt_force_), opts_(parent.opts_), svr_(parent.svr_), lattice_(parent.lattice_) {
     opts_.interleave_chunks = false;
-    stream_ = parent.stream_;
-    owns_stream_ = false;
+    fence_ = parent.fence_;
+    if (parent.opts_.scorer_streams) {
+        check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        std::lock_guard<std::mutex> lk(fence_->mu);
+        fence_->streams.push_back(stream_);
+    } else {
+        stream_ = parent.stream_;
+        owns_stream_ = false;
+    }
     enqueue_mu_ = parent.enqueue_mu_;
     profiling_ = parent.profiling_;
 }
@@ -166,7 +177,50 @@ Engine::~Engine() {
     }
     for (hipStream_t cs : copy_streams_)
         if (cs) (void)hipStreamDestroy(cs);
+    if (fence_) {
+        std::lock_guard<std::mutex> lk(fence_->mu);
+        auto& ss = fence_->streams;
+        for (size_t i = 0; i < ss.size(); i++)
+            if (ss[i] == stream_ && owns_stream_) {
+                if (i < fence_->events.size()) {
+                    if (fence_->events[i]) (void)hipEventDestroy(fence_->events[i]);
+                    fence_->events.erase(fence_->events.begin() + i);
+                }
+                ss.erase(ss.begin() + i);
+                break;
+            }
+        if (ss.empty())
+            for (hipEvent_t e : fence_->events)
+                if (e) (void)hipEventDestroy(e);
+    }
     if (stream_ && owns_stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <typename F>
+void Engine::exclusive(F&& enqueue) {
+    if (!opts_.scorer_streams || !opts_.gt_exclusive) return enqueue();
+    std::lock_guard<std::mutex> lk(fence_->mu);
+    auto& ss = fence_->streams;
+    auto& ev = fence_->events;
+    while (ev.size() < ss.size()) {
+        hipEvent_t e = nullptr;
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        ev.push_back(e);
+    }
+    size_t self = ss.size();
+    for (size_t i = 0; i < ss.size(); i++) {
+        if (ss[i] == stream_) {
+            self = i;
+            continue;
+        }
+        check(hipEventRecord(ev[i], ss[i]), "hipEventRecord");
+        check(hipStreamWaitEvent(stream_, ev[i], 0), "hipStreamWaitEvent");
+    }
+    enqueue();
+    if (self == ss.size()) return;
+    check(hipEventRecord(ev[self], stream_), "hipEventRecord");
+    for (size_t i = 0; i < ss.size(); i++)
+        if (i != self) check(hipStreamWaitEvent(ss[i], ev[self], 0), "hipStreamWaitEvent");
 }
 
 void Engine::check(hipError_t e, const char* what) {
@@ -498,9 +552,9 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
         return;
     }
     // Chunk k on scorer k mod S (scorer 0 this thread, the others helper
-    // engines on threads of their own); all enqueue to the one in-order
-    // stream, so kernels still run one at a time while the others' host work
-    // proceeds.
+    // engines on threads of their own), each on its own stream (or all on
+    // the one in-order stream without scorer streams), so one chunk's
+    // kernels run while the others' host work proceeds.
     const size_t nscore = (size_t)std::max(2, std::min(4, opts_.scorers));
     while (helpers_.size() + 1 < nscore) helpers_.emplace_back(new Engine(HelperTag{}, *this));
     for (auto& h : helpers_) h->profiling_ = profiling_;
@@ -985,7 +1039,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         GtJob* d_gj = upload(gj);
         if (profiling_ && !ev_[0])
             for (auto& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
-        {
+        exclusive([&] {
             // the events bracket this launch alone on a shared stream
             std::unique_lock<std::recursive_mutex> g(*enqueue_mu_, std::defer_lock);
             if (profiling_) {
@@ -994,7 +1048,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
             }
             gammatone(d_gj, (int)gj.size(), max_nf, dr, 0);
             if (profiling_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
-        }
+        });
         if (profiling_) {
             timed = true;
             for (const GtJob& g : gj) {

@@ -26,14 +26,27 @@ struct EngineOptions {
     bool no_realign = false;
     bool lattice = true;
     bool fast = false;
-    // Batches of several chunks: `scorers` engines on the same stream, each
-    // on its own host thread, take the chunks in turn, so one chunk's host
-    // work (planning, decoding, aggregation) overlaps the others' kernels
+    // Batches of several chunks: `scorers` engines, each on its own host
+    // thread, take the chunks in turn, so one chunk's host work (planning,
+    // decoding, aggregation) overlaps the others' kernels
     // (VISQOL_HIP_SCORERS, 2-4, read when an engine is created).  Two
-    // measured best on 1024 ten-second pairs (8300 pairs/s against 8100
-    // with three and 7840 with four).
+    // measured best on 1024 ten-second pairs (on one shared stream 8300
+    // pairs/s against 8100 with three and 7840 with four; on streams of
+    // their own, three no faster than two).
     bool interleave_chunks = true;
     int scorers = 2;
+    // Each scorer on a stream of its own, so one chunk's lag searches,
+    // similarity tables and small kernels run beside another's and in the
+    // gaps of its host work (VISQOL_HIP_SCORER_STREAMS=0: one shared
+    // in-order stream instead).  1024 ten-second pairs: 9000-9100 against
+    // 8600-8670 pairs/s on one shared stream.
+    bool scorer_streams = true;
+    // With scorer streams: whole-signal spectrogram launches run with
+    // nothing from the other scorers' streams beside them, so their own
+    // duration is measurable (VISQOL_HIP_GT_EXCLUSIVE=0 lets other kernels
+    // share the device with them: ~2% more pairs/s, each spectrogram launch
+    // then stretched from ~16 to ~23 ms).
+    bool gt_exclusive = true;
     // Throughput mode: rows of the lag search with more nominated lags than
     // this (at most kMaxLagCands) take the f64 search instead; 0 sends every
     // row there (VISQOL_HIP_LAG_CAP, read when an engine is created).
@@ -226,7 +239,8 @@ class Engine {
 
    private:
     // A helper for interleaved chunks: shares the parent's options, models,
-    // stream and enqueue lock; owns its device arena, staging and tables.
+    // stream fence and enqueue lock (and stream, without scorer streams);
+    // owns its device arena, staging and tables.
     struct HelperTag {};
     Engine(HelperTag, Engine& parent);
     std::vector<std::unique_ptr<Engine>> helpers_;  // scorers - 1 of them, made on first use
@@ -235,6 +249,18 @@ class Engine {
     std::unique_ptr<Engine> uploader_;
     std::vector<double*> slots_;
     size_t slot_len_ = 0;
+    // The streams of an engine and its helpers, for launches that must run
+    // with nothing beside them on the device: such a launch waits for the
+    // work already enqueued on the other streams, and their later work
+    // waits for it.
+    struct StreamFence {
+        std::mutex mu;
+        std::vector<hipStream_t> streams;
+        std::vector<hipEvent_t> events;  // one per stream, created on first use
+    };
+    std::shared_ptr<StreamFence> fence_;
+    template <typename F>
+    void exclusive(F&& enqueue);
     std::recursive_mutex own_enqueue_mu_;
     std::recursive_mutex* enqueue_mu_ = &own_enqueue_mu_;  // held around timed launches on a shared stream
     bool owns_stream_ = true;
