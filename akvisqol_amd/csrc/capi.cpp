@@ -9,8 +9,10 @@
 #include <string.h>
 
 #include <fstream>
+#include <map>
 #include <mutex>
 #include <sstream>
+#include <thread>
 
 #include "../../include/visqol.h"
 #include "../../include/visqol_hip.h"
@@ -555,6 +557,71 @@ int visqol_hip_measure_batch_device(visqol_ctx* c, size_t n, const double* const
         }
         return rc;
     }
+    return finish_batch(res, out, status);
+} catch (...) {
+    return exception_code();
+}
+
+int visqol_hip_measure_mixed_batch_device(visqol_ctx* const* ctxs, size_t num_ctxs, size_t n,
+                                          const uint32_t* ctx_index, const double* const* dev_refs,
+                                          const size_t* ref_lens, const double* const* dev_degs,
+                                          const size_t* deg_lens, const uint32_t* sample_rates,
+                                          visqol_result** out, int32_t* status) try {
+    if (n == 0) return VISQOL_OK;
+    if (!ctxs || num_ctxs == 0 || !ctx_index || !out || !dev_refs || !ref_lens || !dev_degs || !deg_lens ||
+        !sample_rates)
+        return invalid("NULL argument");
+    for (size_t k = 0; k < num_ctxs; k++) {
+        if (!ctxs[k]) return invalid("ctx is NULL");
+        for (size_t j = 0; j < k; j++)
+            if (ctxs[j] == ctxs[k]) return invalid("contexts must be distinct");
+    }
+    for (size_t i = 0; i < n; i++)
+        if (ctx_index[i] >= num_ctxs) return invalid("ctx_index out of range");
+    // per context, its pairs of each sample rate in pair order
+    std::vector<std::map<uint32_t, std::vector<size_t>>> groups(num_ctxs);
+    for (size_t i = 0; i < n; i++) groups[ctx_index[i]][sample_rates[i]].push_back(i);
+    std::vector<PairResult> res(n);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipGetDevice failed");
+        return VISQOL_E_BACKEND;
+    }
+    // A batch that fails as a whole marks its pairs with its code and the
+    // message its thread recorded.
+    auto fail = [&](const std::vector<size_t>& idx, int rc) {
+        const std::string msg = visqol_last_error();
+        for (size_t i : idx) {
+            res[i].status = rc;
+            res[i].error = msg;
+        }
+    };
+    auto work = [&](size_t k) {
+        (void)hipSetDevice(dev);
+        for (const auto& g : groups[k]) try {
+            const std::vector<size_t>& idx = g.second;
+            std::vector<PairInput> pairs(idx.size());
+            for (size_t j = 0; j < idx.size(); j++) {
+                const size_t i = idx[j];
+                pairs[j] = PairInput{dev_refs[i], (long long)ref_lens[i], dev_degs[i], (long long)deg_lens[i], true};
+            }
+            std::vector<PairResult> r;
+            const int rc = run_pairs(ctxs[k], pairs.data(), pairs.size(), g.first, r);
+            if (rc != VISQOL_OK) {
+                fail(idx, rc);
+                continue;
+            }
+            for (size_t j = 0; j < idx.size(); j++) res[idx[j]] = std::move(r[j]);
+        } catch (...) {
+            fail(g.second, exception_code());
+        }
+    };
+    std::vector<std::thread> ts;
+    for (size_t k = 1; k < num_ctxs; k++)
+        if (!groups[k].empty()) ts.emplace_back(work, k);
+    work(0);
+    for (auto& t : ts) t.join();
     return finish_batch(res, out, status);
 } catch (...) {
     return exception_code();

@@ -140,6 +140,10 @@ def _load() -> ctypes.CDLL:
                                              POINTER(Trace), D, D, D]),
         "visqol_hip_measure_batch_device": (c_int, [c_void_p, c_size_t, POINTER(c_void_p), S, POINTER(c_void_p), S,
                                                     c_uint32, POINTER(POINTER(Result)), POINTER(c_int32)]),
+        "visqol_hip_measure_mixed_batch_device": (c_int, [POINTER(c_void_p), c_size_t, c_size_t, POINTER(c_uint32),
+                                                          POINTER(c_void_p), S, POINTER(c_void_p), S,
+                                                          POINTER(c_uint32), POINTER(POINTER(Result)),
+                                                          POINTER(c_int32)]),
         "visqol_hip_profile": (c_int, [c_void_p, c_int, c_int]),
         "visqol_hip_profile_read": (c_int, [c_void_p, D, D, D, POINTER(c_int64)]),
         "visqol_hip_profile_phases": (c_int, [c_void_p, D, c_int]),
@@ -559,6 +563,46 @@ def measure_batch_device(ctx: Context, refs: list[DeviceBuffer], degs: list[Devi
     st = (c_int32 * n)()
     with ctx._lock:
         rc = lib.visqol_hip_measure_batch_device(ctx._h, n, rp, rl, dp, dl, sample_rate, out, st)
+    if not convert:
+        mos = np.full(n, np.nan)
+        for i in range(n):
+            if out[i]:
+                mos[i] = out[i].contents.moslqo
+                lib.visqol_result_free(out[i])
+        return rc, mos, list(st)
+    results = []
+    for i in range(n):
+        if out[i]:
+            results.append(result_dict(out[i].contents))
+            lib.visqol_result_free(out[i])
+        else:
+            results.append(None)
+    return rc, results, list(st)
+
+
+def measure_mixed_batch_device(ctxs: list[Context], ctx_index, refs: list[DeviceBuffer], degs: list[DeviceBuffer],
+                               sample_rates, convert: bool = True):
+    """visqol_hip_measure_mixed_batch_device: pair i scored by
+    ctxs[ctx_index[i]] at sample_rates[i], the contexts' batches at once.
+    Returns (rc, results, status) as measure_batch_device."""
+    n = len(refs)
+    hs = (c_void_p * len(ctxs))(*[c._h for c in ctxs])
+    ci = (c_uint32 * n)(*[int(k) for k in ctx_index])
+    rp = (c_void_p * n)(*[r.ptr for r in refs])
+    dp = (c_void_p * n)(*[d.ptr for d in degs])
+    rl = (c_size_t * n)(*[r.size for r in refs])
+    dl = (c_size_t * n)(*[d.size for d in degs])
+    sr = (c_uint32 * n)(*[int(x) for x in sample_rates])
+    out = (POINTER(Result) * n)()
+    st = (c_int32 * n)()
+    locks = sorted({id(c): c._lock for c in ctxs}.items())
+    for _, lk in locks:
+        lk.acquire()
+    try:
+        rc = lib.visqol_hip_measure_mixed_batch_device(hs, len(ctxs), n, ci, rp, rl, dp, dl, sr, out, st)
+    finally:
+        for _, lk in reversed(locks):
+            lk.release()
     if not convert:
         mos = np.full(n, np.nan)
         for i in range(n):

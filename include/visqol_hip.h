@@ -48,6 +48,22 @@ int visqol_hip_measure_batch_device(visqol_ctx *ctx, size_t n, const double *con
                                     const double *const *dev_degs, const size_t *deg_lens, uint32_t sample_rate,
                                     visqol_result **out, int32_t *status);
 
+/* One batch of pairs of several contexts (modes, options) and sample
+ * rates, signals resident in device memory: pair i is scored by context
+ * ctxs[ctx_index[i]] at sample_rates[i].  Each context scores its pairs
+ * (one batch per sample rate) on a host thread and device streams of its
+ * own, so the contexts' batches run on the device at once: 48 kHz audio
+ * and 16 kHz speech pairs of one uneven mix in one call.  Results equal
+ * each context's visqol_hip_measure_batch_device on its own pairs.  The
+ * contexts must be distinct and not in use elsewhere during the call.
+ * Per-pair failures land in status[i]; returns the first failure in pair
+ * order. */
+int visqol_hip_measure_mixed_batch_device(visqol_ctx *const *ctxs, size_t num_ctxs, size_t n,
+                                          const uint32_t *ctx_index, const double *const *dev_refs,
+                                          const size_t *ref_lens, const double *const *dev_degs,
+                                          const size_t *deg_lens, const uint32_t *sample_rates,
+                                          visqol_result **out, int32_t *status);
+
 /* Per-context kernel timing: when enabled, HIP events on the context's
  * stream bracket every full-signal gammatone launch; *ms is their summed
  * device time, *flop the algorithmic work (30 flop per sample and band of

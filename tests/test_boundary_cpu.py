@@ -113,6 +113,16 @@ def test_invalid_arguments_before_device():
     assert L.visqol_create(ctypes.byref(o), None) == native.E_INVALID_ARGUMENT
     assert L.visqol_measure(None, None, 0, None, 0, 48000, None) == native.E_INVALID_ARGUMENT
     assert L.visqol_measure_batch(None, 0, None, None, None, None, 48000, 0, None, None) == native.E_INVALID_ARGUMENT
+    M = L.visqol_hip_measure_mixed_batch_device
+    assert M(None, 0, 0, None, None, None, None, None, None, None, None) == native.OK  # n == 0
+    assert M(None, 0, 1, None, None, None, None, None, None, None, None) == native.E_INVALID_ARGUMENT
+    hs = (ctypes.c_void_p * 2)(None, None)
+    one = (ctypes.c_uint32 * 1)(0)
+    ptrs = (ctypes.c_void_p * 1)(None)
+    lens = (ctypes.c_size_t * 1)(0)
+    out = (ctypes.POINTER(native.Result) * 1)()
+    assert M(hs, 2, 1, one, ptrs, lens, ptrs, lens, one, out, None) == native.E_INVALID_ARGUMENT
+    assert b"ctx is NULL" in L.visqol_last_error()
     L.visqol_result_free(None)
     L.visqol_destroy(None)
 

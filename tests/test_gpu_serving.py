@@ -85,6 +85,33 @@ def test_uneven_mixed_batch_exact_matches_oracle(native, vo, mix):
             _same(r, o)
 
 
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_mixed_batch_equals_per_mode_batches(native, mix, precision):
+    """visqol_hip_measure_mixed_batch_device over audio and speech pairs
+    (two contexts, their batches at once, interleaved in pair order, plus
+    one pair too short to score): every result equals the same context's
+    own device batch bit for bit, and the short pair fails alone with
+    VISQOL_E_SIGNAL."""
+    pairs = list(mix) + [(mix[0][0][:2000], mix[0][1][:2000], mix[0][2], mix[0][3])]
+    ctxs = [native.Context(0, precision=precision), native.Context(1, precision=precision)]
+    refs = [native.DeviceBuffer(p[0]) for p in pairs]
+    degs = [native.DeviceBuffer(p[1]) for p in pairs]
+    say(f"mixed batch {precision}: n={len(pairs)} modes={[p[3] for p in pairs]}")
+    rc, res, st = native.measure_mixed_batch_device(ctxs, [p[3] for p in pairs], refs, degs, [p[2] for p in pairs])
+    assert st[-1] == native.E_SIGNAL and res[-1] is None
+    assert rc == native.E_SIGNAL
+    assert all(s == 0 for s in st[:-1]), st
+    for mode in (0, 1):
+        idx = [i for i, p in enumerate(pairs[:-1]) if p[3] == mode]
+        sr = pairs[idx[0]][2]
+        rc1, own, st1 = native.measure_batch_device(ctxs[mode], [refs[i] for i in idx], [degs[i] for i in idx], sr)
+        assert rc1 == 0 and all(s == 0 for s in st1)
+        for i, r in zip(idx, own):
+            _same(res[i], r)
+    for b in refs + degs:
+        b.free()
+
+
 def _serve(native, pairs, **cfg):
     with native.Server(precision="fast", **cfg) as srv:
         tickets = [None] * len(pairs)
