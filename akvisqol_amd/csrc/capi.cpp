@@ -12,6 +12,7 @@
 #include <map>
 #include <mutex>
 #include <sstream>
+#include <system_error>
 #include <thread>
 
 #include "../../include/visqol.h"
@@ -618,8 +619,14 @@ int visqol_hip_measure_mixed_batch_device(visqol_ctx* const* ctxs, size_t num_ct
         }
     };
     std::vector<std::thread> ts;
-    for (size_t k = 1; k < num_ctxs; k++)
-        if (!groups[k].empty()) ts.emplace_back(work, k);
+    for (size_t k = 1; k < num_ctxs; k++) {
+        if (groups[k].empty()) continue;
+        try {
+            ts.emplace_back(work, k);
+        } catch (const std::system_error&) {
+            work(k);  // no thread to be had: this context's batches run here
+        }
+    }
     work(0);
     for (auto& t : ts) t.join();
     return finish_batch(res, out, status);

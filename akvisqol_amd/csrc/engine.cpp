@@ -25,6 +25,7 @@
 #include <condition_variable>
 #include <map>
 #include <mutex>
+#include <system_error>
 #include <thread>
 
 namespace vq {
@@ -704,22 +705,42 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
         cv.notify_all();
     };
     std::vector<std::exception_ptr> helper_error(nscore - 1);
+    auto helper = [&](size_t h) {
+        (void)hipSetDevice(dev);
+        try {
+            for (size_t k = h; k < chunks.size(); k += nscore) score(*helpers_[h - 1], k);
+        } catch (...) {
+            helper_error[h - 1] = std::current_exception();
+        }
+    };
     std::vector<std::thread> ts;
-    for (size_t h = 1; h < nscore; h++)
-        ts.emplace_back([&, h] {
-            (void)hipSetDevice(dev);
-            try {
-                for (size_t k = h; k < chunks.size(); k += nscore) score(*helpers_[h - 1], k);
-            } catch (...) {
-                helper_error[h - 1] = std::current_exception();
+    std::vector<size_t> inline_helpers;  // scorers no thread could be made for
+    for (size_t h = 1; h < nscore; h++) {
+        try {
+            ts.emplace_back(helper, h);
+        } catch (const std::system_error&) {
+            if (host_side) {
+                // the upload ring needs every scorer draining its slots:
+                // stop the uploader and the started scorers, fail the batch
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    abort = true;
+                    cv.notify_all();
+                }
+                for (auto& t : ts) t.join();
+                if (up.joinable()) up.join();
+                throw;
             }
-        });
+            inline_helpers.push_back(h);
+        }
+    }
     std::exception_ptr own_error;
     try {
         for (size_t k = 0; k < chunks.size(); k += nscore) score(*this, k);
     } catch (...) {
         own_error = std::current_exception();
     }
+    for (size_t h : inline_helpers) helper(h);
     for (auto& t : ts) t.join();
     if (up.joinable()) up.join();
     if (profiling_)

@@ -216,8 +216,8 @@ def test_prepared_references_equal_batch(native, synth, mode, precision):
 @pytest.mark.parametrize("precision", ["fast", "exact"])
 def test_multi_chunk_batch_equals_single_chunk(native, synth, precision):
     """A batch of more than 512 pairs runs as several chunks, every other
-    chunk on a helper engine sharing the stream: each pair's result equals
-    the same pair scored in a one-chunk batch, bit for bit."""
+    chunk on a helper engine with a stream of its own: each pair's result
+    equals the same pair scored in a one-chunk batch, bit for bit."""
     base = [synth.audio_pair(50 + s, duration=1.2, snr_db=12.0 + 3 * s, delay=20 * s) for s in range(8)]
     pairs = base * 65  # 520 pairs: chunks of 512 and 8
     ctx = native.Context(0, precision=precision)
@@ -228,6 +228,29 @@ def test_multi_chunk_batch_equals_single_chunk(native, synth, precision):
     assert rc == 0
     for i, g in enumerate(got):
         _same(g, want[i % len(base)])
+
+
+@pytest.mark.parametrize("precision", ["fast", "exact"])
+def test_device_multi_chunk_batch_equals_single_chunk(native, synth, precision):
+    """A device-resident batch of 1030 pairs: greedy chunks of 512, 512
+    and 6 are re-cut into four equal chunks of about 258 for the two
+    scorers (each on its own stream, the spectrogram launches fenced); each
+    pair's result equals the same pair in a one-chunk batch, bit for bit."""
+    base = [synth.audio_pair(90 + s, duration=1.1, snr_db=14.0 + 2 * s, delay=15 * s) for s in range(10)]
+    refs = [native.DeviceBuffer(p[0]) for p in base]
+    degs = [native.DeviceBuffer(p[1]) for p in base]
+    ctx = native.Context(0, precision=precision)
+    rc, want, _ = native.measure_batch_device(ctx, refs, degs, 48000)
+    assert rc == 0
+    n = 1030
+    say(f"device multi-chunk batch precision={precision}: {n} pairs")
+    rc, got, st = native.measure_batch_device(ctx, [refs[i % 10] for i in range(n)], [degs[i % 10] for i in range(n)],
+                                              48000)
+    assert rc == 0 and set(st) == {0}
+    for i, g in enumerate(got):
+        _same(g, want[i % len(base)])
+    for b in refs + degs:
+        b.free()
 
 
 def test_prefetched_chunks_equal_single_chunk(native, synth):
