@@ -883,15 +883,28 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
     auto swz = [](int e) { return e ^ (((e >> 3) & 7) | ((((e >> 5) ^ (e >> 6)) & 1) << 3)); };
     C x[8];
     int idx[8];
+    // The swizzle is linear over the index bits (each result bit an XOR of
+    // index bits), and a radix-2^B group's positions are its first position
+    // with c << lgH OR-ed into bits group_layout leaves zero, so slot c is
+    // swz(first) ^ swz(c << lgH): one XOR with a constant per element.
+    int sw[8];
+    auto slots = [&](int B, int lgH) {
+#pragma unroll
+        for (int g0 = 0; g0 < 8; g0 += 1 << B) {
+            const int s0 = swz(idx[g0]);
+#pragma unroll
+            for (int c = 0; c < (1 << B); c++) sw[g0 + c] = s0 ^ swz(c << lgH);
+        }
+    };
     // LDS hand-over between steps: store at the current layout, barrier,
     // load at the next.
     auto put = [&]() {
 #pragma unroll
-        for (int c = 0; c < 8; c++) sm[swz(idx[c])] = x[c];
+        for (int c = 0; c < 8; c++) sm[sw[c]] = x[c];
     };
     auto get = [&]() {
 #pragma unroll
-        for (int c = 0; c < 8; c++) x[c] = sm[swz(idx[c])];
+        for (int c = 0; c < 8; c++) x[c] = sm[sw[c]];
     };
     // forward steps before the central one
     int top = K;
@@ -903,6 +916,7 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
         if (B == 3) {
             int lo[1];
             group_layout<K, 3, THREADS>(lgH, idx, lo);
+            slots(3, lgH);
             if (i == 0) {
 #pragma unroll
                 for (int c = 0; c < 8; c++) x[c] = load2(z, gidx(idx[c]));
@@ -913,6 +927,7 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
         } else {
             int lo[2];
             group_layout<K, 2, THREADS>(lgH, idx, lo);
+            slots(2, lgH);
             if (i == 0) {
 #pragma unroll
                 for (int c = 0; c < 8; c++) x[c] = load2(z, gidx(idx[c]));
@@ -928,6 +943,7 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
     {
         int lo[1];
         group_layout<K, 3, THREADS>(0, idx, lo);
+        slots(3, 0);
         get();
         if constexpr (sizeof(C) == 8) dif8_central(x);
         else radix_step<3, false>(x, lo, 0, tw);
@@ -969,11 +985,13 @@ __global__ __launch_bounds__(TILE / 8) __attribute__((amdgpu_waves_per_eu(4))) v
         if (B == 3) {
             int lo[1];
             group_layout<K, 3, THREADS>(lgH, idx, lo);
+            slots(3, lgH);
             get();
             radix_step<3, true>(x, lo, lgH, tw);
         } else {
             int lo[2];
             group_layout<K, 2, THREADS>(lgH, idx, lo);
+            slots(2, lgH);
             get();
             radix_step<2, true>(x, lo, lgH, tw);
         }
