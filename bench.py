@@ -7,7 +7,9 @@ freeing the results) in the throughput mode (precision "fast": f64 with
 fused multiply-adds, lags nominated from an f32 correlation and rescored in
 f64; |dMOS| <= 1e-6 against the f64 oracle, checked every run).  Beside it:
 the gammatone kernel's roofline fraction from HIP events on the library's
-stream; the same batch as host arrays; the plain per-pair exact f64 path;
+stream; the same batch as host arrays and in the exact f64 mode (the path
+the reference's default backend is served with); the plain per-pair exact
+f64 path;
 per-stage fast-vs-f64 error for audio and speech; uneven batches of the
 1-60 s mix; the continuous-batching server under Poisson arrivals of the
 1-120 s mix (p50/p99 latency and where the tail's time went), with |dMOS|
@@ -191,6 +193,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--exact-pairs", type=int, default=8, help="pairs timed on the per-pair exact f64 path")
+    ap.add_argument("--exact-batch", type=int, default=1,
+                    help="also time the headline batch in the exact f64 mode (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=0, help="oracle pairs per round (0: 2 per host thread)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle checks")
     ap.add_argument("--qps", default="200,500,1000", help="serving loads (requests/s); empty: no serving run")
@@ -277,6 +281,19 @@ def main() -> int:
                                                for sh, m in zip(t["shapes"], t["ms"])}}
     say(f"autotune {autotune}")
     value = args.batch / step_s
+
+    # The exact f64 mode (what the reference's default backend, CPU, is
+    # served with) on the same device-resident batch: one warm, one timed.
+    exact_batch = None
+    exact_batch_mos = None
+    if args.exact_batch:
+        ectx = native.Context(native.MODE_AUDIO, precision="exact")
+        native.measure_batch_device(ectx, refs, degs, 48000, convert=False)
+        t = time.perf_counter()
+        rc_e, exact_batch_mos, _ = native.measure_batch_device(ectx, refs, degs, 48000, convert=False)
+        te = time.perf_counter() - t
+        exact_batch = {"pairs_per_s": round(args.batch / te, 2), "ms_per_batch": round(te * 1e3, 2), "rc": rc_e}
+        say(f"exact batch {exact_batch}")
     for b in refs + degs:
         b.free()
 
@@ -444,6 +461,8 @@ def main() -> int:
         "host_buffers_pairs_per_s": round(host_pps, 3) if host_pps else None,
         "host_buffers_scores_match": host_match,
         "exact_per_pair_pairs_per_s": round(exact_pps, 3) if exact_pps else None,
+        "exact_batch": (dict(exact_batch, matches_per_pair=bool(np.array_equal(exact_batch_mos[:ne], exact_mos)))
+                        if exact_batch else None),
         "speedup_vs_exact_per_pair": round(value / exact_pps, 2) if exact_pps else None,
         "max_abs_dmos_fast_vs_exact_gpu": (float(np.max(np.abs(fast_mos[:ne] - np.array(exact_mos)))) if ne > 0
                                            else None),
