@@ -21,6 +21,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 
@@ -378,7 +379,13 @@ int visqol_hip_server_create(const visqol_options* opts, const visqol_hip_server
         delete s;
         return VISQOL_E_BACKEND;
     }
-    for (uint32_t w = 0; w < c.workers; w++) s->workers.emplace_back([s] { s->work(); });
+    try {
+        for (uint32_t w = 0; w < c.workers; w++) s->workers.emplace_back([s] { s->work(); });
+    } catch (const std::system_error&) {
+        visqol_hip_server_destroy(s);  // stops and joins the workers already started
+        set_error("cannot start %u server worker threads", c.workers);
+        return VISQOL_E_BACKEND;
+    }
     *out = s;
     return VISQOL_OK;
 } catch (...) {
