@@ -170,10 +170,17 @@ __global__ __launch_bounds__(256) void k_ref_terms(const SimJob* __restrict__ jo
     const double crs = tap9([&](int rr, int cc) { const double v = ref[rr * st + cc]; return v * v; }, rm, r, rp, cm, c, cp);
     const double rms = mu * mu;
     double* t = terms + ((long long)blockIdx.z * max_np + p) * 4 * nb * S;
-    t[e] = mu;
-    t[nb * S + e] = FAST ? rms + nsim_c1() : rms;
-    t[2 * nb * S + e] = FAST ? signed_sqrt(crs - rms) : crs - rms;
-    t[3 * nb * S + e] = ref[r * st + c];
+    if (FAST) {  // cell-major: a cell's four terms side by side
+        t[4 * e] = mu;
+        t[4 * e + 1] = rms + nsim_c1();
+        t[4 * e + 2] = signed_sqrt(crs - rms);
+        t[4 * e + 3] = ref[r * st + c];
+    } else {
+        t[e] = mu;
+        t[nb * S + e] = rms;
+        t[2 * nb * S + e] = crs - rms;
+        t[3 * nb * S + e] = ref[r * st + c];
+    }
 }
 
 // One similarity cell in the throughput mode: simCell's value (nsim.zig:
@@ -238,8 +245,17 @@ __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restr
     if (first > hi_u) return;  // uniform
     const int cells = nb * S;
     const double* t = terms + ((long long)blockIdx.z * max_np + p0) * 4 * cells;
-    for (int e = threadIdx.x; e < 4 * cells * npres; e += kSimThreads) sh[e] = t[e];
-    __syncthreads();
+    if (!FAST) {
+        for (int e = threadIdx.x; e < 4 * cells * npres; e += kSimThreads) sh[e] = t[e];
+        __syncthreads();
+    }
+    // Term k of cell e of the group's patch q: exact mode from LDS (four
+    // planes per patch), fast mode straight from the cell-major global
+    // table with wave-uniform indices (scalar loads, no LDS traffic).
+    auto term = [&](int q, int k, int e) -> double {
+        if (FAST) return t[((long long)q * cells + e) * 4 + k];
+        return sh[(q * 4 + k) * cells + e];
+    };
 
     const int off = first + (int)threadIdx.x;
     const int o = off <= hi_u ? off : lo_u;  // idle lanes shadow a valid column
@@ -270,14 +286,14 @@ __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restr
             const double d0 = g[q0], d1 = g[q1], d2 = g[q2];
 #pragma unroll
             for (int q = 0; q < PP; q++) {
-                const double* f = sh + (q * 4 + 3) * cells;
+                const double fm = term(q, 3, rm * S + col), f0 = term(q, 3, r * S + col), fp = term(q, 3, rp * S + col);
                 if (FAST) {
-                    Xd[q][0] = __builtin_fma(f[rp * S + col], d2, f[rm * S + col] * d0);  // T
-                    Xd[q][1] = f[r * S + col] * d1;                                       // M
+                    Xd[q][0] = __builtin_fma(fp, d2, fm * d0);  // T
+                    Xd[q][1] = f0 * d1;                         // M
                 } else {
-                    Xd[q][0] = f[rm * S + col] * d0;
-                    Xd[q][1] = f[r * S + col] * d1;
-                    Xd[q][2] = f[rp * S + col] * d2;
+                    Xd[q][0] = fm * d0;
+                    Xd[q][1] = f0 * d1;
+                    Xd[q][2] = fp * d2;
                 }
             }
         };
@@ -308,9 +324,8 @@ __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restr
                           VQ_WM * Xb[q][1] + VQ_WE * Xc[q][1] + VQ_WC * Xa[q][2] + VQ_WE * Xb[q][2] +
                           VQ_WC * Xc[q][2];
                 }
-                const double* tq = sh + q * 4 * cells;
-                if (FAST) sum[q] += sim_cell_fast(tq[e], tq[cells + e], tq[2 * cells + e], md, v, crd);
-                else sum[q] += sim_cell(tq[e], tq[cells + e], tq[2 * cells + e], md, md * md, v, crd);
+                if (FAST) sum[q] += sim_cell_fast(term(q, 0, e), term(q, 1, e), term(q, 2, e), md, v, crd);
+                else sum[q] += sim_cell(term(q, 0, e), term(q, 1, e), term(q, 2, e), md, md * md, v, crd);
             }
             if (K != kLast) fill(Xa, c + 2 < S ? c + 2 : S - 1);
         };
@@ -351,7 +366,7 @@ void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int ma
     // blocks past a job's patches or search windows exit before any work
     const dim3 grid(xcd_spread((max_window + kSimThreads - 1) / kSimThreads),
                     xcd_spread((max_np + kSimPatchGroup - 1) / kSimPatchGroup), njobs);
-    const size_t lds = sizeof(double) * 4 * nb * patch * kSimPatchGroup;
+    const size_t lds = fast ? 0 : sizeof(double) * 4 * nb * patch * kSimPatchGroup;
     if (fast) {
         hipLaunchKernelGGL(k_ref_terms<true>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
         hipLaunchKernelGGL(k_sim_table<true>, grid, dim3(kSimThreads), lds, stream, d_jobs, nb, patch,
