@@ -1678,6 +1678,9 @@ std::map<std::array<int, 5>, GtTuneEntry> g_gt_tune;
 // units, run the first shape untuned: too small to fill 256 CUs several
 // times over.
 constexpr long long kTuneUnits = 4096;
+// Launches of at most this many units (one block of the first shape each,
+// at most one per CU) run the wide shape.
+constexpr long long kWideUnits = 256;
 }  // namespace
 
 void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate& dr, int kind) {
@@ -1694,6 +1697,10 @@ void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate
     {
         std::lock_guard<std::mutex> g(g_gt_mu);
         GtTuneEntry& t = g_gt_tune[key];
+        // Launches too small to give every CU a block (a few served pairs,
+        // their realignment patches) run the wide shape: their time is one
+        // block's serial work, which the most blocks split finest.
+        if (units <= kWideUnits) return launch(gammatone_wide_shape());
         if (t.chosen >= 0) return launch(t.chosen);
         // Realignment launches (kind 1) keep the first shape: their sizes
         // vary from batch to batch, and a timing run in the middle of a

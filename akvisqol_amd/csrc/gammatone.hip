@@ -35,10 +35,15 @@ namespace vq {
 constexpr int kFramesPerWave = 64;
 
 // Block shapes the launcher offers; index 0 runs until a shape is chosen.
+// The last, one band per lane and eight bands per block, spreads a launch
+// over the most blocks: launches too small to fill the device (a served
+// pair or two) run it, since their time is one block's serial work.
 constexpr GtShape kGtShapes[] = {
-    {4, 4, 32}, {8, 4, 32}, {4, 2, 32}, {8, 2, 32}, {4, 4, 64}, {8, 4, 16},
+    {4, 4, 32}, {8, 4, 32}, {4, 2, 32}, {8, 2, 32}, {4, 4, 64}, {8, 4, 16}, {8, 1, 32},
 };
 constexpr int kGtShapeCount = (int)(sizeof(kGtShapes) / sizeof(kGtShapes[0]));
+constexpr int kGtWideShape = kGtShapeCount - 1;
+static_assert(kGtShapes[kGtWideShape].bands_per_lane == 1, "the wide shape runs one band per lane");
 
 // One band's recurrence state on one frame and the sum of squares of its
 // output: the four DF2T biquads' two state values each, or in the FIR +
@@ -269,14 +274,21 @@ constexpr auto kExactLaunch = shape_table<false>(std::make_integer_sequence<int,
 
 int gammatone_shape_count() { return kGtShapeCount; }
 
+int gammatone_wide_shape() { return kGtWideShape; }
+
 GtShape gammatone_shape(int v) { return kGtShapes[v >= 0 && v < kGtShapeCount ? v : 0]; }
 
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const FirCoef* d_fir,
                       const double* d_hann, int nb, int window, int hop, bool fast, int shape, hipStream_t stream) {
     if (njobs == 0 || max_nf == 0) return;
     if (max_nf <= kFramesPerWave / 2 && njobs > 1) {  // short jobs (realignment patches): two per block
-        if (fast) launch_shape<true, 0, true>(d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
-        else launch_shape<false, 0, true>(d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
+        if (shape == kGtWideShape) {
+            if (fast) launch_shape<true, kGtWideShape, true>(d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
+            else launch_shape<false, kGtWideShape, true>(d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
+        } else {
+            if (fast) launch_shape<true, 0, true>(d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
+            else launch_shape<false, 0, true>(d_jobs, njobs, max_nf, d_coef, d_fir, d_hann, nb, window, hop, stream);
+        }
         return;
     }
     const int v = shape >= 0 && shape < kGtShapeCount ? shape : 0;

@@ -26,6 +26,10 @@ struct GtShape {
 };
 int gammatone_shape_count();
 GtShape gammatone_shape(int v);
+// The shape with the most blocks per launch (one band per lane), for
+// launches too small to fill the device.  Short jobs (two per block) run
+// shape 0 or this one.
+int gammatone_wide_shape();
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const FirCoef* d_fir,
                       const double* d_hann, int nb, int window, int hop, bool fast, int shape, hipStream_t stream);
 
