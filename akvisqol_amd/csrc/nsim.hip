@@ -204,6 +204,9 @@ __device__ inline double sim_cell_fast(double mr, double rmsc1, double ssr, doub
 }
 
 constexpr int kSimThreads = 512;
+// Launches of fewer blocks of kSimPatchGroup patches and kSimThreads lanes
+// than this (fewer than the device's CUs) take 64-lane one-patch blocks.
+constexpr long long kSimSmallLaunch = 256;
 
 // measureSimilarityAt (nsim.zig:364-424) for kSimPatchGroup patches of one
 // pair at one offset per lane, lanes on consecutive offsets of the union of
@@ -221,10 +224,15 @@ constexpr int kSimThreads = 512;
 // M = X[r]; the window's weights (corner WC, edge WE, middle WM) then give
 // c3 + WC·(T[c-1] + T[c+1]) + WE·(T[c] + M[c-1] + M[c+1]) + WM·M[c]:
 // six operations a cell instead of nine, with fused multiply-adds.
-template <bool FAST>
-__global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restrict__ jobs, int nb, int S,
-                                                           const double* __restrict__ terms, int max_np) {
-    constexpr int PP = kSimPatchGroup;
+//
+// PP patches per block and THREADS lanes: launches that fill the device run
+// kSimPatchGroup patches on kSimThreads lanes; launches of a few pairs, whose
+// time is one block's serial cell loop, run one patch per 64-lane block, so
+// their (patch, offset) lanes spread over the most CUs.  Every grouping
+// gives the same bits (each lane's sum keeps its order).
+template <bool FAST, int PP, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_sim_table(const SimJob* __restrict__ jobs, int nb, int S,
+                                                       const double* __restrict__ terms, int max_np) {
     extern __shared__ double sh[];  // [PP][4][nb*S]: mu_ref, mu_ref², sigma_ref² (or root), ref
     const SimJob j = jobs[blockIdx.z];
     const int p0 = blockIdx.y * PP;
@@ -241,12 +249,12 @@ __global__ __launch_bounds__(kSimThreads) void k_sim_table(const SimJob* __restr
             hi_u = hi[q] > hi_u ? hi[q] : hi_u;
         }
     }
-    const int first = lo_u + (int)blockIdx.x * kSimThreads;
+    const int first = lo_u + (int)blockIdx.x * THREADS;
     if (first > hi_u) return;  // uniform
     const int cells = nb * S;
     const double* t = terms + ((long long)blockIdx.z * max_np + p0) * 4 * cells;
     if (!FAST) {
-        for (int e = threadIdx.x; e < 4 * cells * npres; e += kSimThreads) sh[e] = t[e];
+        for (int e = threadIdx.x; e < 4 * cells * npres; e += THREADS) sh[e] = t[e];
         __syncthreads();
     }
     // Term k of cell e of the group's patch q: exact mode from LDS (four
@@ -363,19 +371,26 @@ void launch_sim_table(const SimJob* d_jobs, int njobs, int nb, int patch, int ma
                       bool fast, hipStream_t stream) {
     if (njobs == 0 || max_np == 0 || max_window == 0) return;
     const dim3 gt((nb * patch + 255) / 256, max_np, njobs);
+    if (fast) hipLaunchKernelGGL(k_ref_terms<true>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
+    else hipLaunchKernelGGL(k_ref_terms<false>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
     // blocks past a job's patches or search windows exit before any work
-    const dim3 grid(xcd_spread((max_window + kSimThreads - 1) / kSimThreads),
-                    xcd_spread((max_np + kSimPatchGroup - 1) / kSimPatchGroup), njobs);
-    const size_t lds = fast ? 0 : sizeof(double) * 4 * nb * patch * kSimPatchGroup;
-    if (fast) {
-        hipLaunchKernelGGL(k_ref_terms<true>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
-        hipLaunchKernelGGL(k_sim_table<true>, grid, dim3(kSimThreads), lds, stream, d_jobs, nb, patch,
-                           (const double*)d_terms, max_np);
-    } else {
-        hipLaunchKernelGGL(k_ref_terms<false>, gt, dim3(256), 0, stream, d_jobs, nb, patch, d_terms, max_np);
-        hipLaunchKernelGGL(k_sim_table<false>, grid, dim3(kSimThreads), lds, stream, d_jobs, nb, patch,
-                           (const double*)d_terms, max_np);
-    }
+    auto launch = [&](auto pp, auto threads) {
+        constexpr int PP = decltype(pp)::value, T = decltype(threads)::value;
+        const dim3 grid(xcd_spread((max_window + T - 1) / T), xcd_spread((max_np + PP - 1) / PP), njobs);
+        const size_t lds = fast ? 0 : sizeof(double) * 4 * nb * patch * PP;
+        if (fast)
+            hipLaunchKernelGGL((k_sim_table<true, PP, T>), grid, dim3(T), lds, stream, d_jobs, nb, patch,
+                               (const double*)d_terms, max_np);
+        else
+            hipLaunchKernelGGL((k_sim_table<false, PP, T>), grid, dim3(T), lds, stream, d_jobs, nb, patch,
+                               (const double*)d_terms, max_np);
+    };
+    const long long big_blocks = (long long)njobs * ((max_np + kSimPatchGroup - 1) / kSimPatchGroup) *
+                                 ((max_window + kSimThreads - 1) / kSimThreads);
+    if (big_blocks < kSimSmallLaunch)
+        launch(std::integral_constant<int, 1>(), std::integral_constant<int, 64>());
+    else
+        launch(std::integral_constant<int, kSimPatchGroup>(), std::integral_constant<int, kSimThreads>());
 }
 
 // ---------------------------------------------------------------- DP
