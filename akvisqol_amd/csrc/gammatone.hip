@@ -23,7 +23,8 @@
 // operations per sample and band instead of 30); the other bands run the
 // equivalent FIR + shared all-pole form (14 operations).  With WAVES
 // dividing kDf2tBands, the DF2T bands are exactly the first kDf2tBands/WAVES
-// slots of block row 0, fixed at compile time.
+// slots of block row 0 (in blocks of fewer bands than kDf2tBands, the
+// first block rows), fixed at compile time.
 #include <array>
 #include <utility>
 
@@ -35,11 +36,12 @@ namespace vq {
 constexpr int kFramesPerWave = 64;
 
 // Block shapes the launcher offers; index 0 runs until a shape is chosen.
-// The last, one band per lane and eight bands per block, spreads a launch
-// over the most blocks: launches too small to fill the device (a served
-// pair or two) run it, since their time is one block's serial work.
+// The last, one band per lane and four bands per block (one wave per SIMD),
+// spreads a launch over the most blocks: launches too small to fill the
+// device (a served pair or two) run it, since their time is one block's
+// serial work.
 constexpr GtShape kGtShapes[] = {
-    {4, 4, 32}, {8, 4, 32}, {4, 2, 32}, {8, 2, 32}, {4, 4, 64}, {8, 4, 16}, {8, 1, 32},
+    {4, 4, 32}, {8, 4, 32}, {4, 2, 32}, {8, 2, 32}, {4, 4, 64}, {8, 4, 16}, {4, 1, 32},
 };
 constexpr int kGtShapeCount = (int)(sizeof(kGtShapes) / sizeof(kGtShapes[0]));
 constexpr int kGtWideShape = kGtShapeCount - 1;
@@ -226,8 +228,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_gammatone(const GtJob* __restric
                                                           const FirCoef* __restrict__ fir,
                                                           const double* __restrict__ hann, int nb, int window, int hop) {
     static_assert(kFramesPerWave * CHUNK % (64 * WAVES) == 0, "tile must split evenly over the block");
-    static_assert(kDf2tBands % WAVES == 0 && WAVES * BPL >= kDf2tBands,
-                  "the DF2T bands must fill whole slots of block row 0");
+    // The DF2T bands fill whole slots of block row 0, or whole block rows
+    // when a block holds fewer bands than there are DF2T bands.
+    constexpr bool kNarrow = WAVES * BPL < kDf2tBands;
+    static_assert(kNarrow ? kDf2tBands % (WAVES * BPL) == 0 : kDf2tBands % WAVES == 0,
+                  "the DF2T bands must fill whole slots of block row 0, or whole block rows");
     const int za = PAIR ? 2 * (int)blockIdx.z : (int)blockIdx.z;
     const GtJob ja = jobs[za];
     GtJob jb = ja;
@@ -242,7 +247,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_gammatone(const GtJob* __restric
     const int band0 = blockIdx.y * (WAVES * BPL) + wave;
     const int v = band0 < nb ? min(BPL, (nb - band0 + WAVES - 1) / WAVES) : 0;
     constexpr int kM0 = FAST ? (BPL < kDf2tBands / WAVES ? BPL : kDf2tBands / WAVES) : BPL;
-    if (!FAST || blockIdx.y == 0)
+    if (!FAST || (kNarrow ? (int)blockIdx.y * WAVES * BPL < kDf2tBands : blockIdx.y == 0))
         gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, kM0>(v, ja, jb, coef, fir, hann, band0, window, hop, tile);
     else
         gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, 0>(v, ja, jb, coef, fir, hann, band0, window, hop, tile);
