@@ -41,7 +41,7 @@ constexpr int kFramesPerWave = 64;
 // device (a served pair or two) run it, since their time is one block's
 // serial work.
 constexpr GtShape kGtShapes[] = {
-    {4, 4, 32}, {8, 4, 32}, {4, 2, 32}, {8, 2, 32}, {4, 4, 64}, {8, 4, 16}, {4, 1, 32},
+    {4, 4, 32}, {8, 4, 32}, {4, 2, 32}, {8, 2, 32}, {4, 4, 64}, {8, 4, 16}, {4, 1, 64},
 };
 constexpr int kGtShapeCount = (int)(sizeof(kGtShapes) / sizeof(kGtShapes[0]));
 constexpr int kGtWideShape = kGtShapeCount - 1;
