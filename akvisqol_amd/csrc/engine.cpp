@@ -996,7 +996,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     std::vector<PlaneJob> plj;
     std::vector<SimJob> sj;
     std::vector<DpJob> dj;
-    int max_nf = 0, max_width = 0, max_np = 0, max_win = 0;
+    int max_nf = 0, max_width = 0, max_np = 0, max_win = 0, max_span = 0;
     long long max_len = 0;
     // Chunk-wide contiguous blocks: one allocation, upload, memset and
     // readback each instead of one per pair.
@@ -1074,6 +1074,8 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
         sj.push_back(SimJob{s.spec_ref, s.plane, s.sims, s.d_idx, s.d_lo, s.d_hi, s.np, s.nf_ref, s.nf_deg, s.width});
         dj.push_back(DpJob{s.sims, s.dp, s.bt, s.d_lo, s.d_hi, s.d_off, s.d_state, s.np, s.nf_deg});
         max_np = std::max(max_np, s.np);
+        for (int p = 0; p < s.np; p++)  // the DP's scan range of row p
+            max_span = std::max(max_span, s.hi[p] - (p == 0 ? s.lo[0] : std::min(s.lo[p - 1], s.lo[p])) + 1);
         // widest union of the search windows of a similarity-table patch group
         for (int p0 = 0; p0 < s.np; p0 += kSimPatchGroup) {
             int lo = s.lo[p0], hi = s.hi[p0];
@@ -1136,7 +1138,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                         prof_.sim_cells += (double)(st[i].hi[p] - st[i].lo[p] + 1) * nb * S;
             prof_.sim_launches++;
         }
-        launch_dp(upload(dj), (int)dj.size(), stream_);
+        launch_dp(upload(dj), (int)dj.size(), max_span, stream_);
     }
     // Readback of the DP choice (and stage outputs on request).
     std::vector<std::vector<int>> h_off(n), h_state(n);

@@ -220,7 +220,10 @@ struct DpJob {
     int* state;     // [np] 0 untouched, 1 valid, 2 no real match
     int np, nf_deg;
 };
-void launch_dp(const DpJob* d_jobs, int njobs, hipStream_t stream);
+// max_span: the longest scan range of any row (frames from the previous
+// row's lo to the row's hi); from kDpWideSpan up a pair gets 1024 lanes.
+constexpr int kDpWideSpan = 2048;
+void launch_dp(const DpJob* d_jobs, int njobs, int max_span, hipStream_t stream);
 
 // Per-band NSIM statistics of one patch pair (nsim.zig:427-510).
 struct StatJob {

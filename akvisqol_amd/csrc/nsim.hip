@@ -405,9 +405,9 @@ struct ArgMax {
 };
 __device__ inline ArgMax later_max(ArgMax a, ArgMax b) { return b.v >= a.v ? b : a; }
 
-// Exclusive scan of one ArgMax per thread of a 256-thread block under
-// later_max, in thread order: shuffles within each wave, then the waves'
-// totals through LDS.  later_max is associative (max never rounds; ties go
+// Exclusive scan of one ArgMax per thread of a block under later_max, in
+// thread order: shuffles within each wave, then the waves' totals through
+// LDS.  later_max is associative (max never rounds; ties go
 // to the later operand), so the result is the serial left-to-right one.
 __device__ inline ArgMax block_exclusive_later_max(ArgMax x, double* wv, int* wi) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -438,13 +438,19 @@ __device__ inline ArgMax block_exclusive_later_max(ArgMax x, double* wv, int* wi
 // default) read and write global memory only.
 constexpr int kDpLds = 4096;
 
-__global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
+//
+// THREADS lanes per pair: 256, or 1024 when the launch's longest scan range
+// is long enough (long pairs, wide search windows) that a quarter of the
+// per-thread serial run outweighs the longer cross-wave step.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_dp(const DpJob* __restrict__ jobs) {
+    constexpr int kWaves = THREADS / 64;
     const DpJob j = jobs[blockIdx.x];
     const int cols = j.nf_deg;
     const auto* wlo = as_global(j.lo);
     const auto* whi = as_global(j.hi);
-    __shared__ double wv[4];
-    __shared__ int wi[4];
+    __shared__ double wv[kWaves];
+    __shared__ int wi[kWaves];
     __shared__ double ring[2][kDpLds];
     __shared__ int s_span;
     constexpr int kMask = kDpLds - 1;
@@ -463,7 +469,7 @@ __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
     const bool in_lds = s_span <= kDpLds;
     // LDS rows: each thread's share of the next similarity row is loaded
     // into registers while the current row is scanned
-    constexpr int kPre = kDpLds / 256;
+    constexpr int kPre = kDpLds / THREADS;
     double pre[kPre];
     // window bounds run one row ahead, so their loads overlap a row's work
     int lo_prev = 0, hi_prev = 0, lo_cur = wlo[0], hi_cur = whi[0];
@@ -473,7 +479,7 @@ __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
         const auto* sr = as_global(j.sims + (long long)q * cols);
 #pragma unroll
         for (int k = 0; k < kPre; k++) {
-            const int s = qlo + (int)threadIdx.x + k * 256;
+            const int s = qlo + (int)threadIdx.x + k * THREADS;
             pre[k] = s <= qhi ? sr[s] : 0.0;
         }
     };
@@ -504,7 +510,7 @@ __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
                 const int phi = hi_prev;
 #pragma unroll
                 for (int k = 0; k < kPre; k++) {
-                    const int s = lo + (int)threadIdx.x + k * 256;
+                    const int s = lo + (int)threadIdx.x + k * THREADS;
                     if (s <= hi) cur[s & kMask] = pre[k];
                 }
                 for (int s = phi + 1 + threadIdx.x; s <= hi; s += blockDim.x) ring[(p + 1) & 1][s & kMask] = 0.0;
@@ -573,7 +579,7 @@ __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
     __syncthreads();
     if (threadIdx.x == 0) {
         ArgMax r = {wv[0], wi[0]};
-        for (int w = 1; w < 4; w++)
+        for (int w = 1; w < kWaves; w++)
             if (wv[w] > r.v) r = ArgMax{wv[w], wi[w]};
         int cur = r.i == INT_MAX ? 0 : r.i;
         for (int p = 0; p < j.np; p++) {
@@ -590,9 +596,10 @@ __global__ __launch_bounds__(256) void k_dp(const DpJob* __restrict__ jobs) {
     }
 }
 
-void launch_dp(const DpJob* d_jobs, int njobs, hipStream_t stream) {
+void launch_dp(const DpJob* d_jobs, int njobs, int max_span, hipStream_t stream) {
     if (njobs == 0) return;
-    hipLaunchKernelGGL(k_dp, dim3(njobs), dim3(256), 0, stream, d_jobs);
+    if (max_span >= kDpWideSpan) hipLaunchKernelGGL(k_dp<1024>, dim3(njobs), dim3(1024), 0, stream, d_jobs);
+    else hipLaunchKernelGGL(k_dp<256>, dim3(njobs), dim3(256), 0, stream, d_jobs);
 }
 
 // ---------------------------------------------------------------- patch statistics
