@@ -205,8 +205,9 @@ __device__ inline double sim_cell_fast(double mr, double rmsc1, double ssr, doub
 
 constexpr int kSimThreads = 512;
 // Launches of fewer blocks of kSimPatchGroup patches and kSimThreads lanes
-// than this (fewer than the device's CUs) take 64-lane one-patch blocks.
-constexpr long long kSimSmallLaunch = 256;
+// than this (up to four per CU) take 64-lane one-patch blocks (one 60 s
+// pair alone: 2.47 -> 2.38 ms against a threshold of one block per CU).
+constexpr long long kSimSmallLaunch = 1024;
 
 // measureSimilarityAt (nsim.zig:364-424) for kSimPatchGroup patches of one
 // pair at one offset per lane, lanes on consecutive offsets of the union of

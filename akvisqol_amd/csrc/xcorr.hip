@@ -1304,10 +1304,13 @@ Lag32Layout lag32_layout(int njobs, long long n) {
 // partials in tile order.
 __global__ __launch_bounds__(64) void k_lag32_slack(const double* __restrict__ energy, int nparts, int lgn,
                                                     float* __restrict__ slack) {
-    if (threadIdx.x != 0) return;
+    // the row's energy: lane l sums partials l, l + 64, ..., then a fixed
+    // shuffle tree (the same order on every run, whatever the batch)
     const double* e = energy + (long long)blockIdx.x * nparts;
     double acc = 0.0;
-    for (int p = 0; p < nparts; p++) acc += e[p];
+    for (int p = (int)threadIdx.x; p < nparts; p += 64) acc += e[p];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (threadIdx.x != 0) return;
     const double tol = 8.0 * (double)FLT_EPSILON * (double)lgn;
     slack[blockIdx.x] = (float)(2.0 * tol * acc * (double)(1LL << lgn));
 }
