@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -24,6 +25,7 @@
 #include <system_error>
 #include <thread>
 #include <unordered_map>
+#include <vector>
 
 #include "../../include/visqol.h"
 #include "../../include/visqol_hip.h"
@@ -94,9 +96,11 @@ class DeviceSlab {
 // kStageBytes, each held by one submitting thread for the length of its
 // upload.  A request's bytes pass through its buffer in pieces of half the
 // buffer, the halves alternating, so the thread copies one piece while the
-// DMA of the previous one runs; every DMA goes on the server's one copy
-// stream (pinned sources copy at ~45 GB/s in such pieces, pageable ones at
-// ~26 GB/s).
+// DMA of the previous one runs; the DMAs go on the server's copy streams
+// (pinned sources copy at ~45 GB/s in such pieces, pageable ones at
+// ~26 GB/s).  Requests of 32 MiB and up are cut into stripes uploaded by
+// several threads at once (upload_striped): one thread's host copy
+// (~30 GB/s) would otherwise bound them.
 class Stager {
    public:
     static constexpr int kStageBuffers = 8;
@@ -172,6 +176,44 @@ class Stager {
         return e;
     }
 
+    // A large request: its bytes cut into `parts` stripes (256-byte
+    // aligned), each uploaded as above through a staging buffer of its own
+    // on a host thread of its own (the caller's the first), the stripes'
+    // DMAs alternating between two copy streams, so several host copies
+    // feed the link at once.  Returns once every stripe is on the device.
+    hipError_t upload_striped(char* dst, const std::pair<const char*, size_t>* seg, int nseg,
+                              const hipStream_t* streams, int parts) {
+        size_t total = 0;
+        for (int i = 0; i < nseg; i++) total += seg[i].second;
+        std::vector<hipError_t> err(parts, hipSuccess);
+        auto stripe = [&](int t) {
+            const size_t a = t == 0 ? 0 : (total * t / parts) & ~(size_t)255;
+            const size_t b = t + 1 == parts ? total : (total * (t + 1) / parts) & ~(size_t)255;
+            std::vector<std::pair<const char*, size_t>> sub;
+            size_t pos = 0;
+            for (int i = 0; i < nseg; i++) {
+                const size_t s0 = pos, s1 = pos + seg[i].second;
+                const size_t lo = std::max(a, s0), hi = std::min(b, s1);
+                if (lo < hi) sub.emplace_back(seg[i].first + (lo - s0), hi - lo);
+                pos = s1;
+            }
+            if (!sub.empty()) err[t] = upload(dst + a, sub.data(), (int)sub.size(), streams[t & 1]);
+        };
+        std::vector<std::thread> ts;
+        for (int t = 1; t < parts; t++) {
+            try {
+                ts.emplace_back(stripe, t);
+            } catch (const std::system_error&) {
+                stripe(t);
+            }
+        }
+        stripe(0);
+        for (auto& th : ts) th.join();
+        for (hipError_t e : err)
+            if (e != hipSuccess) return e;
+        return hipSuccess;
+    }
+
    private:
     struct Buf {
         char* host = nullptr;
@@ -204,7 +246,7 @@ struct visqol_hip_server {
     visqol_hip_server_config cfg{};
     DeviceSlab slab;
     Stager stager;
-    hipStream_t copy_stream = nullptr;
+    hipStream_t copy_stream = nullptr, copy_stream2 = nullptr;
     std::mutex mu;
     std::condition_variable cv_work, cv_done, cv_space;
     std::deque<Request*> queue;
@@ -373,7 +415,8 @@ int visqol_hip_server_create(const visqol_options* opts, const visqol_hip_server
         delete s;
         return VISQOL_E_OUT_OF_MEMORY;
     }
-    if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->copy_stream2, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
         set_error("cannot create a HIP stream");
         delete s;
@@ -429,7 +472,13 @@ int visqol_hip_server_submit(visqol_hip_server* s, uint32_t mode, const double* 
     (void)hipSetDevice(s->device);
     const std::pair<const char*, size_t> seg[2] = {{(const char*)ref, sizeof(double) * ref_len},
                                                    {(const char*)deg, sizeof(double) * deg_len}};
-    const hipError_t e = s->stager.upload(s->slab.at(rp->slab_off), seg, 2, s->copy_stream);
+    // requests of 32 MiB and up (pairs longer than ~35 s at 48 kHz) go up
+    // in stripes of at least 16 MiB on up to four threads
+    const size_t bytes = seg[0].second + seg[1].second;
+    const hipStream_t streams[2] = {s->copy_stream, s->copy_stream2};
+    const int parts = (int)std::min<size_t>(4, bytes / ((size_t)16 << 20));
+    const hipError_t e = parts >= 2 ? s->stager.upload_striped(s->slab.at(rp->slab_off), seg, 2, streams, parts)
+                                    : s->stager.upload(s->slab.at(rp->slab_off), seg, 2, s->copy_stream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         std::lock_guard<std::mutex> lk(s->mu);
@@ -523,6 +572,7 @@ void visqol_hip_server_destroy(visqol_hip_server* s) {
     s->cv_work.notify_all();
     for (auto& t : s->workers) t.join();
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+    if (s->copy_stream2) (void)hipStreamDestroy(s->copy_stream2);
     delete s;
 }
 
