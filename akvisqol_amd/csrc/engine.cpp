@@ -1099,7 +1099,7 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
                 g.lock();
                 check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
             }
-            gammatone(d_gj, (int)gj.size(), max_nf, dr, 0);
+            gammatone(d_gj, (int)gj.size(), max_nf, dr, 0, &gj);
             if (profiling_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
         });
         if (profiling_) {
@@ -1642,7 +1642,7 @@ std::unique_ptr<PreparedSet> Engine::prepare(const double* const* refs, const lo
     sums(s1, d_out, !opts_.fast);
     sums(s2, d_out, !opts_.fast);
     if (!gj.empty())
-        gammatone(upload(gj), (int)gj.size(), max_nf, dr, 0);
+        gammatone(upload(gj), (int)gj.size(), max_nf, dr, 0, &gj);
     unsigned char* d_active = vad_bytes ? (unsigned char*)arena_.alloc(vad_bytes) : nullptr;
     if (!vj.empty()) {
         for (int i = 0; i < n; i++)
@@ -1685,9 +1685,36 @@ constexpr long long kTuneUnits = 4096;
 constexpr long long kWideUnits = 256;
 }  // namespace
 
-void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate& dr, int kind) {
+void Engine::gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate& dr, int kind,
+                       const std::vector<GtJob>* host_jobs) {
     const int nb = dr.p.nb, W = dr.p.window, hop = dr.p.hop;
+    // the unit list of each shape's band-row count, made on first use
+    std::map<int, std::pair<const GtUnit*, int>> lists;
     auto launch = [&](int shape) {
+        if (host_jobs && max_nf > kFramesPerUnit) {
+            const GtShape sh = gammatone_shape(shape);
+            const int rows = (nb + sh.waves * sh.bands_per_lane - 1) / (sh.waves * sh.bands_per_lane);
+            long long active = 0;
+            for (const GtJob& j : *host_jobs) active += (j.nf + kFramesPerUnit - 1) / kFramesPerUnit;
+            const long long grid = (long long)xcd_spread((max_nf + kFramesPerUnit - 1) / kFramesPerUnit) * njobs;
+            // a quarter or more of the grid's blocks would exit at once
+            if (4 * active <= 3 * grid) {
+                auto it = lists.find(rows);
+                if (it == lists.end()) {
+                    std::vector<GtUnit> units;
+                    units.reserve((size_t)active * rows);
+                    for (int jb = 0; jb < njobs; jb++) {
+                        const int blocks = ((*host_jobs)[jb].nf + kFramesPerUnit - 1) / kFramesPerUnit;
+                        for (int b = 0; b < blocks; b++)
+                            for (int r = 0; r < rows; r++) units.push_back(GtUnit{jb, b, r, 0});
+                    }
+                    it = lists.emplace(rows, std::make_pair((const GtUnit*)upload(units), (int)units.size())).first;
+                }
+                launch_gammatone_units(d_jobs, it->second.first, it->second.second, dr.coef, dr.fir, dr.hann, nb, W,
+                                       hop, opts_.fast, shape, stream_);
+                return;
+            }
+        }
         launch_gammatone(d_jobs, njobs, max_nf, dr.coef, dr.fir, dr.hann, nb, W, hop, opts_.fast, shape, stream_);
     };
     if (njobs == 0 || max_nf == 0) return;

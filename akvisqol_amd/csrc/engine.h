@@ -319,7 +319,11 @@ class Engine {
     void finish(PairResult& r, const RateParams& rp);
     // Spectrograms of d_jobs with the tuned block shape (tuning on the
     // first large launch of this kind); kind as in gammatone_tuning.
-    void gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate& dr, int kind);
+    // With the host copy of the jobs, whole-signal launches whose grid
+    // would be mostly blocks past a shorter job's end take the persistent
+    // unit-list form instead.
+    void gammatone(const GtJob* d_jobs, int njobs, int max_nf, const DevRate& dr, int kind,
+                   const std::vector<GtJob>* host_jobs = nullptr);
     int gt_force_ = -1;  // block shape forced by VISQOL_HIP_GAMMATONE_SHAPE, else -1
 
     EngineOptions opts_;

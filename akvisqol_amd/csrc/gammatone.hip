@@ -33,7 +33,7 @@
 
 namespace vq {
 
-constexpr int kFramesPerWave = 64;
+constexpr int kFramesPerWave = kFramesPerUnit;
 
 // Block shapes the launcher offers; index 0 runs until a shape is chosen.
 // The last, one band per lane and four bands per block (one wave per SIMD),
@@ -124,12 +124,12 @@ __device__ inline void band_step_fir(BandState& b, FirState& f, const FirCoef& c
 template <bool FAST, int WAVES, int BPL, int CHUNK, bool PAIR, int M, int V>
 __device__ __forceinline__ void gt_run(const GtJob& ja, const GtJob& jb, const BandCoef* __restrict__ coef,
                                        const FirCoef* __restrict__ fir, const double* __restrict__ hann, int band0,
-                                       int window, int hop, double (*tile)[CHUNK + 1]) {
+                                       int window, int hop, double (*tile)[CHUNK + 1], int fblock) {
     constexpr int kThreads = 64 * WAVES;
     constexpr int kStage = kFramesPerWave * CHUNK / kThreads;  // tile elements staged per thread
     constexpr int kSlots = V > 0 ? V : 1;
     constexpr int kFramesPerJob = PAIR ? kFramesPerWave / 2 : kFramesPerWave;
-    const int f0 = blockIdx.x * kFramesPerJob;
+    const int f0 = fblock * kFramesPerJob;
     // tile row (= lane) -> job and frame
     auto row_b = [](int row) { return PAIR && row >= kFramesPerWave / 2; };
     auto row_frame = [&](int row) { return f0 + (PAIR ? row & (kFramesPerWave / 2 - 1) : row); };
@@ -213,13 +213,35 @@ __device__ __forceinline__ void gt_run(const GtJob& ja, const GtJob& jb, const B
 template <bool FAST, int WAVES, int BPL, int CHUNK, bool PAIR, int MM, int VV = BPL>
 __device__ __forceinline__ void gt_dispatch(int v, const GtJob& ja, const GtJob& jb, const BandCoef* __restrict__ coef,
                                             const FirCoef* __restrict__ fir, const double* __restrict__ hann,
-                                            int band0, int window, int hop, double (*tile)[CHUNK + 1]) {
+                                            int band0, int window, int hop, double (*tile)[CHUNK + 1], int fblock) {
     if (v == VV) {
         gt_run<FAST, WAVES, BPL, CHUNK, PAIR, (MM < VV ? MM : VV), VV>(ja, jb, coef, fir, hann, band0, window, hop,
-                                                                        tile);
+                                                                        tile, fblock);
     } else if constexpr (VV > 0) {
-        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, MM, VV - 1>(v, ja, jb, coef, fir, hann, band0, window, hop, tile);
+        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, MM, VV - 1>(v, ja, jb, coef, fir, hann, band0, window, hop, tile,
+                                                              fblock);
     }
+}
+
+// One block's work: 64 frames (frame block fblock; PAIR: 32 of each job)
+// of block row `row` of the bands.
+template <bool FAST, int WAVES, int BPL, int CHUNK, bool PAIR>
+__device__ __forceinline__ void gt_block(const GtJob& ja, const GtJob& jb, const BandCoef* __restrict__ coef,
+                                         const FirCoef* __restrict__ fir, const double* __restrict__ hann, int nb,
+                                         int window, int hop, int fblock, int row, double (*tile)[CHUNK + 1]) {
+    // The DF2T bands fill whole slots of block row 0, or whole block rows
+    // when a block holds fewer bands than there are DF2T bands.
+    constexpr bool kNarrow = WAVES * BPL < kDf2tBands;
+    static_assert(kNarrow ? kDf2tBands % (WAVES * BPL) == 0 : kDf2tBands % WAVES == 0,
+                  "the DF2T bands must fill whole slots of block row 0, or whole block rows");
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int band0 = row * (WAVES * BPL) + wave;
+    const int v = band0 < nb ? min(BPL, (nb - band0 + WAVES - 1) / WAVES) : 0;
+    constexpr int kM0 = FAST ? (BPL < kDf2tBands / WAVES ? BPL : kDf2tBands / WAVES) : BPL;
+    if (!FAST || (kNarrow ? row * WAVES * BPL < kDf2tBands : row == 0))
+        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, kM0>(v, ja, jb, coef, fir, hann, band0, window, hop, tile, fblock);
+    else
+        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, 0>(v, ja, jb, coef, fir, hann, band0, window, hop, tile, fblock);
 }
 
 template <bool FAST, int WAVES, int BPL, int CHUNK, bool PAIR>
@@ -228,11 +250,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_gammatone(const GtJob* __restric
                                                           const FirCoef* __restrict__ fir,
                                                           const double* __restrict__ hann, int nb, int window, int hop) {
     static_assert(kFramesPerWave * CHUNK % (64 * WAVES) == 0, "tile must split evenly over the block");
-    // The DF2T bands fill whole slots of block row 0, or whole block rows
-    // when a block holds fewer bands than there are DF2T bands.
-    constexpr bool kNarrow = WAVES * BPL < kDf2tBands;
-    static_assert(kNarrow ? kDf2tBands % (WAVES * BPL) == 0 : kDf2tBands % WAVES == 0,
-                  "the DF2T bands must fill whole slots of block row 0, or whole block rows");
     const int za = PAIR ? 2 * (int)blockIdx.z : (int)blockIdx.z;
     const GtJob ja = jobs[za];
     GtJob jb = ja;
@@ -243,14 +260,31 @@ __global__ __launch_bounds__(64 * WAVES) void k_gammatone(const GtJob* __restric
     const int frames = PAIR ? kFramesPerWave / 2 : kFramesPerWave;
     if ((int)blockIdx.x * frames >= (ja.nf > jb.nf ? ja.nf : jb.nf)) return;  // whole block idle: uniform exit
     __shared__ double tile[kFramesPerWave][CHUNK + 1];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int band0 = blockIdx.y * (WAVES * BPL) + wave;
-    const int v = band0 < nb ? min(BPL, (nb - band0 + WAVES - 1) / WAVES) : 0;
-    constexpr int kM0 = FAST ? (BPL < kDf2tBands / WAVES ? BPL : kDf2tBands / WAVES) : BPL;
-    if (!FAST || (kNarrow ? (int)blockIdx.y * WAVES * BPL < kDf2tBands : blockIdx.y == 0))
-        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, kM0>(v, ja, jb, coef, fir, hann, band0, window, hop, tile);
-    else
-        gt_dispatch<FAST, WAVES, BPL, CHUNK, PAIR, 0>(v, ja, jb, coef, fir, hann, band0, window, hop, tile);
+    gt_block<FAST, WAVES, BPL, CHUNK, PAIR>(ja, jb, coef, fir, hann, nb, window, hop, (int)blockIdx.x,
+                                            (int)blockIdx.y, tile);
+}
+
+// Persistent form for launches whose jobs differ in length: a grid of at
+// most the blocks the device holds at once walks a compact list of the
+// (job, frame block, band row) units that have frames, unit u on block
+// u mod gridDim.x, instead of a grid sized by the longest job whose blocks
+// past a shorter job's end exit at once.  Each unit runs exactly as that
+// block of the grid form, so the bits are the same.
+template <bool FAST, int WAVES, int BPL, int CHUNK>
+__global__ __launch_bounds__(64 * WAVES) void k_gammatone_units(const GtJob* __restrict__ jobs,
+                                                                const GtUnit* __restrict__ units, int nunits,
+                                                                const BandCoef* __restrict__ coef,
+                                                                const FirCoef* __restrict__ fir,
+                                                                const double* __restrict__ hann, int nb, int window,
+                                                                int hop) {
+    __shared__ double tile[kFramesPerWave][CHUNK + 1];
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const GtUnit w = units[u];
+        const GtJob ja = jobs[w.job];
+        // the previous unit's last reads of the tile precede this one's writes
+        __syncthreads();
+        gt_block<FAST, WAVES, BPL, CHUNK, false>(ja, ja, coef, fir, hann, nb, window, hop, w.block, w.row, tile);
+    }
 }
 
 namespace {
@@ -269,6 +303,38 @@ void launch_shape(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_
 using GtLaunch = void (*)(const GtJob*, int, int, const BandCoef*, const FirCoef*, const double*, int, int, int,
                          hipStream_t);
 
+template <bool FAST, int V>
+void launch_units_shape(const GtJob* d_jobs, const GtUnit* d_units, int nunits, const BandCoef* d_coef,
+                        const FirCoef* d_fir, const double* d_hann, int nb, int window, int hop, hipStream_t stream) {
+    constexpr GtShape sh = kGtShapes[V];
+    auto kern = k_gammatone_units<FAST, sh.waves, sh.bands_per_lane, sh.chunk>;
+    // as many blocks as the device holds at once (queried once per kernel)
+    static int resident = [&] {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * sh.waves, 0) != hipSuccess || cus <= 0 ||
+            per <= 0) {
+            (void)hipGetLastError();
+            return 256;
+        }
+        return cus * per;
+    }();
+    const int grid = nunits < resident ? nunits : resident;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * sh.waves), 0, stream, d_jobs, d_units, nunits, d_coef, d_fir,
+                       d_hann, nb, window, hop);
+}
+
+using GtUnitsLaunch = void (*)(const GtJob*, const GtUnit*, int, const BandCoef*, const FirCoef*, const double*, int,
+                               int, int, hipStream_t);
+
+template <bool FAST, int... V>
+constexpr std::array<GtUnitsLaunch, sizeof...(V)> units_table(std::integer_sequence<int, V...>) {
+    return {&launch_units_shape<FAST, V>...};
+}
+constexpr auto kFastUnits = units_table<true>(std::make_integer_sequence<int, kGtShapeCount>{});
+constexpr auto kExactUnits = units_table<false>(std::make_integer_sequence<int, kGtShapeCount>{});
+
 template <bool FAST, int... V>
 constexpr std::array<GtLaunch, sizeof...(V)> shape_table(std::integer_sequence<int, V...>) {
     return {&launch_shape<FAST, V>...};
@@ -282,6 +348,14 @@ int gammatone_shape_count() { return kGtShapeCount; }
 int gammatone_wide_shape() { return kGtWideShape; }
 
 GtShape gammatone_shape(int v) { return kGtShapes[v >= 0 && v < kGtShapeCount ? v : 0]; }
+
+void launch_gammatone_units(const GtJob* d_jobs, const GtUnit* d_units, int nunits, const BandCoef* d_coef,
+                            const FirCoef* d_fir, const double* d_hann, int nb, int window, int hop, bool fast,
+                            int shape, hipStream_t stream) {
+    if (nunits <= 0) return;
+    const int v = shape >= 0 && shape < kGtShapeCount ? shape : 0;
+    (fast ? kFastUnits : kExactUnits)[v](d_jobs, d_units, nunits, d_coef, d_fir, d_hann, nb, window, hop, stream);
+}
 
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const FirCoef* d_fir,
                       const double* d_hann, int nb, int window, int hop, bool fast, int shape, hipStream_t stream) {

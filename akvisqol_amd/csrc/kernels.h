@@ -33,6 +33,18 @@ int gammatone_wide_shape();
 void launch_gammatone(const GtJob* d_jobs, int njobs, int max_nf, const BandCoef* d_coef, const FirCoef* d_fir,
                       const double* d_hann, int nb, int window, int hop, bool fast, int shape, hipStream_t stream);
 
+// The persistent form over jobs of different lengths: units lists the
+// (job, 64-frame block, band row of WAVES·BPL bands of `shape`) blocks that
+// have frames; a grid of at most the blocks the device holds at once walks
+// them.  Same bits as launch_gammatone with the same shape.
+struct GtUnit {
+    int job, block, row, pad_;
+};
+constexpr int kFramesPerUnit = 64;  // frames of one block (one per lane)
+void launch_gammatone_units(const GtJob* d_jobs, const GtUnit* d_units, int nunits, const BandCoef* d_coef,
+                            const FirCoef* d_fir, const double* d_hann, int nb, int window, int hop, bool fast,
+                            int shape, hipStream_t stream);
+
 // Grid extent for a dimension whose trailing blocks exit at once when a job
 // is shorter than the longest (jobs of uneven size).  Workgroups go to the
 // eight XCDs round-robin by linear block index, so with an even extent the
