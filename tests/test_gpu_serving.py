@@ -112,6 +112,22 @@ def test_mixed_batch_equals_per_mode_batches(native, mix, precision):
         b.free()
 
 
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_uneven_batch_equals_pairs_alone(native, mix, precision):
+    """An uneven batch's whole-signal spectrograms run the persistent
+    unit-list form (most of the grid form's blocks would lie past a shorter
+    job's end); each pair alone runs the grid form: every result is the
+    same, bit for bit."""
+    for mode, sr in ((0, 48000), (1, 16000)):
+        sel = [p for p in mix if p[3] == mode]
+        ctx = native.Context(mode, precision=precision)
+        say(f"uneven batch vs alone: mode={mode} precision={precision} lens={[p[0].size for p in sel]}")
+        rc, batch, st = ctx.measure_batch([p[0] for p in sel], [p[1] for p in sel], sr)
+        assert rc == 0 and set(st) == {0}
+        for p, b in zip(sel, batch):
+            _same(b, ctx.measure(p[0], p[1], sr))
+
+
 def _serve(native, pairs, **cfg):
     with native.Server(precision="fast", **cfg) as srv:
         tickets = [None] * len(pairs)
