@@ -243,6 +243,42 @@ def test_errors(native):
     assert e.value.code == native.E_INVALID_ARGUMENT
 
 
+@pytest.mark.parametrize("precision", ["exact", "fast"])
+def test_degenerate_inputs_match_oracle(native, vo, synth, precision):
+    """Silence, one silent side, DC, identical signals, unequal lengths and
+    extreme levels (below the dB floor, far above full scale), in both
+    modes: exact equals the oracle bit for bit, fast within 1e-6 MOS with
+    the same global lag."""
+    ref, deg = synth.audio_pair(3, duration=2.0, delay=50)
+    sref, sdeg = synth.speech_pair(2, duration=3.0)
+    cases = [
+        ("silence", 0, 48000, np.zeros(96000), np.zeros(96000)),
+        ("silent degraded", 0, 48000, ref, np.zeros(96000)),
+        ("silent reference", 0, 48000, np.zeros(96000), deg),
+        ("dc", 0, 48000, np.full(96000, 0.3), np.full(96000, 0.3)),
+        ("identical", 0, 48000, ref, ref.copy()),
+        ("degraded shorter", 0, 48000, ref, deg[:50000]),
+        ("degraded longer", 0, 48000, ref, np.concatenate([deg, deg])),
+        ("below the floor", 0, 48000, ref * 1e-9, deg * 1e-9),
+        ("loud", 0, 48000, ref * 1e4, deg * 1e4),
+        ("speech silence", 1, 16000, np.zeros(48000), np.zeros(48000)),
+        ("speech identical", 1, 16000, sref, sref.copy()),
+        ("speech silent degraded", 1, 16000, sref, np.zeros(len(sref))),
+    ]
+    ctxs = {m: native.Context(m, precision=precision) for m in (0, 1)}
+    for name, mode, sr, r, d in cases:
+        say(f"degenerate {name} mode={mode} precision={precision}")
+        res, tr = ctxs[mode].measure_trace(r, d, sr)
+        ores, otr = vo.Oracle(mode=mode).measure(r, d, sr)
+        assert tr["lag"] == otr["lag"], name
+        if precision == "exact":
+            assert res["moslqo"] == ores["moslqo"], name
+            assert np.array_equal(res["fvnsim"], ores["fvnsim"]), name
+            assert tr["dp_offset"] == otr["dp_offset"], name
+        else:
+            assert abs(res["moslqo"] - ores["moslqo"]) <= 1e-6, (name, res["moslqo"], ores["moslqo"])
+
+
 def test_fast_mode_close(native, vo, synth):
     say("fast mode |dMOS| <= 1e-6")
     ref, deg = synth.audio_pair(0)
