@@ -619,6 +619,7 @@ int visqol_hip_measure_mixed_batch_device(visqol_ctx* const* ctxs, size_t num_ct
         }
     };
     std::vector<std::thread> ts;
+    ts.reserve(num_ctxs);  // only thread creation can throw below
     for (size_t k = 1; k < num_ctxs; k++) {
         if (groups[k].empty()) continue;
         try {

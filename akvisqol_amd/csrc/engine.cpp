@@ -714,7 +714,9 @@ void Engine::run(const PairInput* pairs, int n, uint32_t sr, PairResult* out, bo
         }
     };
     std::vector<std::thread> ts;
+    ts.reserve(nscore);  // only thread creation can throw below
     std::vector<size_t> inline_helpers;  // scorers no thread could be made for
+    inline_helpers.reserve(nscore);
     for (size_t h = 1; h < nscore; h++) {
         try {
             ts.emplace_back(helper, h);

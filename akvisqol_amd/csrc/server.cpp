@@ -200,6 +200,7 @@ class Stager {
             if (!sub.empty()) err[t] = upload(dst + a, sub.data(), (int)sub.size(), streams[t & 1]);
         };
         std::vector<std::thread> ts;
+        ts.reserve(parts);  // only thread creation can throw below
         for (int t = 1; t < parts; t++) {
             try {
                 ts.emplace_back(stripe, t);
