@@ -921,6 +921,26 @@ int visqol_hip_best_lag(const double* a, size_t la, const double* b, size_t lb, 
     return exception_code();
 }
 
+int visqol_hip_sum(int kind, const double* x, size_t len, double plain_sum, int64_t lag, size_t half, int form,
+                   double* out) try {
+    if ((!x && len) || !out || kind < kSum || kind > kSumSqFrom || form < 0 || form > 2)
+        return invalid("invalid argument");
+    if (kind == kNyquist && len == 0) return invalid("the alternating sum needs at least one sample");
+    std::string why;
+    if (!device_ok(why)) { set_error("%s", why.c_str()); return VISQOL_E_BACKEND; }
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    try {
+        *out = stage_engine(0, false)->reference_sum(kind, x, (long long)len, plain_sum, (long long)lag,
+                                                     (long long)half, form);
+    } catch (const BackendError& e) {
+        set_error("%s", e.msg.c_str());
+        return e.code;
+    }
+    return VISQOL_OK;
+} catch (...) {
+    return exception_code();
+}
+
 int visqol_hip_best_lag_fast(const double* a, size_t la, const double* b, size_t lb, int cap, int64_t* lag,
                              int* nominated) try {
     if ((!a && la) || (!b && lb) || !lag) return invalid("invalid argument");

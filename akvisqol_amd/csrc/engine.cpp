@@ -127,6 +127,7 @@ Engine::Engine(const EngineOptions& o, std::unique_ptr<SvrModel> svr,
     }
     if (const char* e = getenv("VISQOL_HIP_SCORER_STREAMS")) opts_.scorer_streams = strcmp(e, "0") != 0;
     if (const char* e = getenv("VISQOL_HIP_GT_EXCLUSIVE")) opts_.gt_exclusive = strcmp(e, "0") != 0;
+    if (const char* e = getenv("VISQOL_HIP_SPEC_SUMS")) opts_.spec_sums = strcmp(e, "0") != 0;
     fence_ = std::make_shared<StreamFence>();
     fence_->streams.push_back(stream_);
     if (const char* e = getenv("VISQOL_HIP_GAMMATONE_SHAPE")) {
@@ -456,12 +457,23 @@ void Engine::lap(int phase) {
     lap_t_ = now;
 }
 
+// Reference-order sums of few long signals take the speculative form: as
+// one serial chain per signal they would leave all but a handful of CUs
+// idle, while a batch's many chains already fill the device.
+constexpr size_t kSpecMaxJobs = 16;
+constexpr long long kSpecMinLen = 1 << 16;
+
 void Engine::sums(const std::vector<SumJob>& jobs, double* d_out, bool serial) {
     if (jobs.empty()) return;
-    double* parts =
-        serial ? nullptr : (double*)arena_.alloc(sizeof(double) * kSumOutputs * kSumParts * jobs.size());
     long long max_len = 0;
     for (const SumJob& j : jobs) max_len = std::max(max_len, j.len);
+    if (serial && opts_.spec_sums && jobs.size() <= kSpecMaxJobs && max_len >= kSpecMinLen && spec_sums_fit(max_len)) {
+        void* scratch = arena_.alloc(spec_sums_scratch_bytes((int)jobs.size(), max_len));
+        launch_sums_spec(upload(jobs), (int)jobs.size(), max_len, d_out, scratch, stream_);
+        return;
+    }
+    double* parts =
+        serial ? nullptr : (double*)arena_.alloc(sizeof(double) * kSumOutputs * kSumParts * jobs.size());
     launch_sums(upload(jobs), (int)jobs.size(), max_len, d_out, serial, parts, stream_);
 }
 
@@ -1846,6 +1858,35 @@ long long Engine::best_lag(const double* a, long long la, const double* b, long 
     readback(p, d_lag, sizeof(long long));
     sync();
     return *p;
+}
+
+double Engine::reference_sum(int kind, const double* x, long long len, double plain_sum, long long lag, long long half,
+                             int form) {
+    arena_.reset();
+    pinned_.reset();
+    double* dx = (double*)arena_.alloc(sizeof(double) * std::max(1LL, len));
+    if (len) check(hipMemcpyAsync(dx, x, sizeof(double) * len, hipMemcpyHostToDevice, stream_), "upload");
+    // slot 0: the plain sum (kNyquist's mean), slot 1: the result, then the lag
+    double* d_s = (double*)arena_.alloc(sizeof(double) * 2 + sizeof(long long));
+    double* h = (double*)pinned_.alloc(sizeof(double) * 2 + sizeof(long long));
+    h[0] = plain_sum;
+    h[1] = 0.0;
+    memcpy(h + 2, &lag, sizeof(long long));
+    check(hipMemcpyAsync(d_s, h, sizeof(double) * 2 + sizeof(long long), hipMemcpyHostToDevice, stream_), "upload");
+    const long long* d_lag = reinterpret_cast<const long long*>(d_s + 2);
+    std::vector<SumJob> jobs{SumJob{dx, len, kind, 1, 0, 0, kind == kSumSqFrom ? d_lag : nullptr, half}};
+    if (form == 1 && spec_sums_fit(len)) {
+        void* scratch = arena_.alloc(spec_sums_scratch_bytes(1, len));
+        launch_sums_spec(upload(jobs), 1, len, d_s, scratch, stream_);
+    } else if (form == 2) {
+        sums(jobs, d_s, true);
+    } else {
+        launch_sums(upload(jobs), 1, len, d_s, true, nullptr, stream_);
+    }
+    double* r = (double*)pinned_.alloc(sizeof(double));
+    readback(r, d_s + 1, sizeof(double));
+    sync();
+    return *r;
 }
 
 void Engine::xcorr(const double* a, long long la, const double* b, long long lb, std::vector<double>& out) {

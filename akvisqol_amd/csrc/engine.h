@@ -47,6 +47,11 @@ struct EngineOptions {
     // share the device with them: ~2% more pairs/s, each spectrogram launch
     // then stretched from ~16 to ~23 ms).
     bool gt_exclusive = true;
+    // Exact mode: launches of reference-order sums over few long signals
+    // take the speculative form (same bits; a ten-second signal's sum in
+    // ~0.2 ms instead of ~2 ms as one serial chain); VISQOL_HIP_SPEC_SUMS=0
+    // keeps one serial chain per signal.
+    bool spec_sums = true;
     // Throughput mode: rows of the lag search with more nominated lags than
     // this (at most kMaxLagCands) take the f64 search instead; 0 sends every
     // row there (VISQOL_HIP_LAG_CAP, read when an engine is created).
@@ -176,6 +181,12 @@ class Engine {
     // Stage access for checks.
     void spectrogram(uint32_t sample_rate, const double* host_sig, long long len, std::vector<double>& out, int& nf);
     long long best_lag(const double* a, long long la, const double* b, long long lb);
+    // One reference-order sum over host samples x[0..len): kind a SumKind
+    // (kNyquist's mean from plain_sum, kSumSqFrom's start from lag and
+    // half); form 0 the serial chain, 1 the speculative form, 2 as the
+    // engine's own launches choose.
+    double reference_sum(int kind, const double* x, long long len, double plain_sum, long long lag, long long half,
+                         int form);
     // The throughput mode's lag (nominated in f32, rescored in f64) with
     // the engine's cap; *nominated is false when the row fell back to the
     // f64 search (or its size is outside the nominating search's range).

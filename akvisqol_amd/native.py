@@ -162,6 +162,7 @@ def _load() -> ctypes.CDLL:
         "visqol_hip_spectrogram": (c_int, [c_uint32, c_uint32, c_uint32, D, c_size_t, D, S]),
         "visqol_hip_filters": (c_int, [c_uint32, c_uint32, D, D]),
         "visqol_hip_best_lag": (c_int, [D, c_size_t, D, c_size_t, POINTER(c_int64)]),
+        "visqol_hip_sum": (c_int, [c_int, D, c_size_t, c_double, c_int64, c_size_t, c_int, POINTER(c_double)]),
         "visqol_hip_best_lag_noalloc": (c_int, [c_void_p, c_size_t, c_void_p, c_size_t, POINTER(c_int64)]),
         "visqol_hip_math_probe": (c_int, [c_int, D, D, D, c_size_t]),
         "visqol_hip_fft": (c_int, [D, c_size_t, c_int]),
@@ -800,6 +801,18 @@ def filters(sample_rate: int, mode: int = MODE_AUDIO):
     cf = np.zeros(nb)
     _check(lib.visqol_hip_filters(mode, sample_rate, _dptr(b), _dptr(cf)))
     return b.reshape(nb, 8), cf
+
+
+SUM_PLAIN, SUM_SQ, SUM_ALTERNATING, SUM_SQ_FROM_LAG = 0, 1, 2, 3
+SUM_SERIAL, SUM_SPECULATIVE, SUM_AS_SCORED = 0, 1, 2
+
+
+def reference_sum(kind: int, x, plain_sum: float = 0.0, lag: int = 0, half: int = 0, form: int = SUM_SERIAL) -> float:
+    """visqol_hip_sum: one reference-order f64 sum of x on the device."""
+    x = _f64(x)
+    out = c_double()
+    _check(lib.visqol_hip_sum(kind, _dptr(x), x.size, plain_sum, lag, half, form, ctypes.byref(out)))
+    return out.value
 
 
 def best_lag(a, b) -> int:

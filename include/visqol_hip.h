@@ -224,6 +224,15 @@ int visqol_hip_frame_grad(uint32_t mode, uint32_t sample_rate, const double *fra
 int visqol_hip_filters(uint32_t mode, uint32_t sample_rate, double *bands8, double *center_freqs);
 /* FFT cross-correlation lag of the upper envelopes of a and b. */
 int visqol_hip_best_lag(const double *a, size_t la, const double *b, size_t lb, int64_t *lag);
+/* One reference-order (left-to-right, f64) sum over x[0..len) on the
+ * device, as the exact path forms it (envelope.zig:23-53,
+ * realign.zig:21-51): kind 0 sum of x, 1 sum of x*x, 2 the alternating
+ * sum of +-(x[i] - plain_sum/len) (+ for even i), 3 sum of x*x from
+ * sample -lag when -half <= lag < 0, else from 0.  form 0: one serial
+ * chain; 1: the speculative segmented form; 2: as the scoring path chooses.
+ * Every form gives the same bits. */
+int visqol_hip_sum(int kind, const double *x, size_t len, double plain_sum, int64_t lag, size_t half, int form,
+                   double *out);
 /* visqol_hip_best_lag on signals in device memory (dev_a, dev_b on the
  * current device), allocating nothing.  No counterpart in the reference:
  * for callers that cannot allocate device memory (code under graph

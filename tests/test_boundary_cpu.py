@@ -150,6 +150,14 @@ def test_gradient_and_probe_arguments_before_device():
     assert L.visqol_hip_frame_grad(0, 48000, p(frame), 100, p(g), 0, 100, p(out)) == native.E_INVALID_ARGUMENT
     assert L.visqol_hip_frame_grad(0, 48000, p(frame), 3840, p(g), 10, 5, p(out)) == native.E_INVALID_ARGUMENT
     assert L.visqol_hip_frame_grad(0, 48000, p(frame), 3840, p(g), 0, 4000, p(out)) == native.E_INVALID_ARGUMENT
+    # the reference-order sum probe: kind, form, pointers and an empty
+    # alternating sum (its mean would divide by zero) checked first
+    s = ctypes.c_double()
+    assert L.visqol_hip_sum(4, p(frame), 10, 0.0, 0, 0, 0, ctypes.byref(s)) == native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_sum(0, p(frame), 10, 0.0, 0, 0, 3, ctypes.byref(s)) == native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_sum(0, None, 10, 0.0, 0, 0, 0, ctypes.byref(s)) == native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_sum(0, p(frame), 10, 0.0, 0, 0, 0, None) == native.E_INVALID_ARGUMENT
+    assert L.visqol_hip_sum(2, p(frame), 0, 0.0, 0, 0, 0, ctypes.byref(s)) == native.E_INVALID_ARGUMENT
 
 
 def test_noalloc_limit_checked_before_device():
