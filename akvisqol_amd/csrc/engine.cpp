@@ -459,15 +459,20 @@ void Engine::lap(int phase) {
 
 // Reference-order sums of few long signals take the speculative form: as
 // one serial chain per signal they would leave all but a handful of CUs
-// idle, while a batch's many chains already fill the device.
-constexpr size_t kSpecMaxJobs = 16;
-constexpr long long kSpecMinLen = 1 << 16;
+// idle, while a batch's many chains already fill the device.  The choice
+// follows the form's time model (below two thirds of the serial chain's).
+constexpr size_t kSpecMaxJobs = 64;
+constexpr long long kSpecMinLen = 1 << 14;
 
 void Engine::sums(const std::vector<SumJob>& jobs, double* d_out, bool serial) {
     if (jobs.empty()) return;
-    long long max_len = 0;
-    for (const SumJob& j : jobs) max_len = std::max(max_len, j.len);
-    if (serial && opts_.spec_sums && jobs.size() <= kSpecMaxJobs && max_len >= kSpecMinLen && spec_sums_fit(max_len)) {
+    long long max_len = 0, total_len = 0;
+    for (const SumJob& j : jobs) {
+        max_len = std::max(max_len, j.len);
+        total_len += j.len;
+    }
+    if (serial && opts_.spec_sums && jobs.size() <= kSpecMaxJobs && max_len >= kSpecMinLen && spec_sums_fit(max_len) &&
+        spec_sums_cost_ratio((int)jobs.size(), max_len, total_len) < 2.0 / 3.0) {
         void* scratch = arena_.alloc(spec_sums_scratch_bytes((int)jobs.size(), max_len));
         launch_sums_spec(upload(jobs), (int)jobs.size(), max_len, d_out, scratch, stream_);
         return;

@@ -83,16 +83,20 @@ constexpr int kSumOutputs = 3;
 void launch_sums(const SumJob* d_jobs, int njobs, long long max_len, double* d_out, bool serial, double* d_partials,
                  hipStream_t stream);
 // The serial sums' speculative form (the same bits): each job's terms in
-// segments of about 16 K run at once from 2048 candidate starts each,
-// then resolved segment by segment (a segment
-// whose true start falls outside its candidates is run again serially).
-// For launches of few long jobs, where the serial form would run a
-// handful of chains on an otherwise idle device.  d_scratch holds
-// spec_sums_scratch_bytes(njobs, max_len) bytes; spec_sums_fit(max_len):
-// at most kSpecMaxSegs segments.
+// segments (16 K terms from 2048 candidate starts each when the longest
+// job has kSpecLongLen terms or more, else 4 K from 512) run at once,
+// then resolved segment by segment (a segment whose true start falls
+// outside its candidates is run again serially).  For launches of few
+// jobs, where the serial form would run a handful of chains on an
+// otherwise idle device.  d_scratch holds spec_sums_scratch_bytes(njobs,
+// max_len) bytes; spec_sums_fit(max_len): at most kSpecMaxSegs segments;
+// spec_sums_cost_ratio: the form's modelled time over the serial form's
+// (total_len: the jobs' summed lengths).
 constexpr int kSpecMaxSegs = 2048;
+constexpr long long kSpecLongLen = 1 << 16;
 size_t spec_sums_scratch_bytes(int njobs, long long max_len);
 bool spec_sums_fit(long long max_len);
+double spec_sums_cost_ratio(int njobs, long long max_len, long long total_len);
 void launch_sums_spec(const SumJob* d_jobs, int njobs, long long max_len, double* d_out, void* d_scratch,
                       hipStream_t stream);
 

@@ -281,24 +281,17 @@ __device__ inline double from_ordered_bits(long long o) {
     return __longlong_as_double(o < 0 ? ((-o) | (long long)0x8000000000000000ULL) : o);
 }
 
-constexpr int kSpecChunk = 64;         // terms per estimate chunk
-constexpr int kSpecSegChunks = 256;    // nominal segment: 16384 terms
-constexpr int kSpecWindow = 64;        // boundary search: ± chunks around the nominal edge
+constexpr int kSpecChunk = 64;  // terms per estimate chunk
 constexpr int kSpecPlanThreads = 1024;
-// Candidate starts per segment, kSpecPerLane to a lane.  The rounded
-// chain's distance from the estimate at a boundary grows about as the
-// square root of the position; on ten-second music it reached 1776 ulps
-// (1024 candidates missed up to 3 of a signal's 29 boundaries).
-constexpr int kSpecCands = 2048;
-constexpr int kSpecPerLane = 2;
-static_assert(kSpecCands % (256 * kSpecPerLane) == 0, "candidates fill whole blocks");
-static_assert(2 * kSpecWindow <= kSpecSegChunks / 2, "windows of neighbouring boundaries stay apart");
+constexpr int kSpecPerLane = 2;  // candidate starts per lane
 
 // Scratch of one launch: chunk sums, segment starts and estimates, segment
 // counts, and every candidate chain's end value.
 struct SpecLayout {
     long long nchunks;  // chunks of the longest job
     long long tstride;  // terms per job in the terms array (the longest job's)
+    int seg_chunks;     // nominal segment length in chunks
+    int window;         // boundary search: ± chunks around the nominal edge
     int pmax;           // segments of the longest job
     int cands;          // candidate starts per segment
     size_t chunks, terms, begin, est, nseg, starts, ends, bytes;
@@ -306,14 +299,25 @@ struct SpecLayout {
 __host__ __device__ inline long long spec_chunks_for(long long terms) {
     return terms > 0 ? (terms + kSpecChunk - 1) / kSpecChunk : 0;
 }
-__host__ __device__ inline int spec_segments_for(long long nchunks) {
-    return nchunks <= kSpecSegChunks ? 1 : (int)((nchunks + kSpecSegChunks - 1) / kSpecSegChunks);
+__host__ __device__ inline int spec_segments_for(long long nchunks, int seg_chunks) {
+    return nchunks <= seg_chunks ? 1 : (int)((nchunks + seg_chunks - 1) / seg_chunks);
 }
+// Segment length, boundary window and candidate count per launch.  The
+// rounded chain's distance from the estimate at a boundary grows about as
+// the square root of the position: on ten-second music it reached 1776
+// ulps (1024 candidates then missed up to 3 of a signal's 29 boundaries),
+// on 30-45 K-sample slices with 4 K-term segments 487 (2 of 690 beyond
+// ±256, each miss one 4 K-term serial segment).  Windows stay within
+// a quarter of the segment, so neighbouring windows never meet (nominal
+// edges lie at least half a segment apart).
 inline SpecLayout spec_layout(int njobs, long long max_len) {
     SpecLayout L{};
     L.nchunks = spec_chunks_for(max_len);
-    L.pmax = spec_segments_for(L.nchunks);
-    L.cands = kSpecCands;
+    const bool shortjobs = max_len < kSpecLongLen;
+    L.seg_chunks = shortjobs ? 64 : 256;
+    L.window = shortjobs ? 16 : 64;
+    L.cands = shortjobs ? 512 : 2048;
+    L.pmax = spec_segments_for(L.nchunks, L.seg_chunks);
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t off = 0;
     L.tstride = max_len > 0 ? max_len : 1;
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(kSpecPlanThreads) void k_spec_plan(const SumJob* __
     const SumJob j = jobs[job];
     const long long start = sum_start(j);
     const long long nch = spec_chunks_for(j.len - start);
-    const int P = spec_segments_for(nch);  // <= L.pmax: the job is no longer than the longest
+    const int P = spec_segments_for(nch, L.seg_chunks);  // <= L.pmax: the job is no longer than the longest
     const DD* ch = reinterpret_cast<const DD*>(scratch + L.chunks) + (long long)job * L.nchunks;
     long long* begin = reinterpret_cast<long long*>(scratch + L.begin) + (long long)job * (L.pmax + 1);
     double* est = reinterpret_cast<double*>(scratch + L.est) + (long long)job * L.pmax;
@@ -414,20 +418,22 @@ __global__ __launch_bounds__(kSpecPlanThreads) void k_spec_plan(const SumJob* __
     }
     const DD excl = threadIdx.x > 0 ? scan[threadIdx.x - 1] : DD{0.0, 0.0};
     // The boundary whose window holds chunk edge c (edge c: the prefix of
-    // chunks [0, c)), or 0.  Nominal edges q·nch/P lie at least 128
-    // chunks apart, so at most one window holds c, and the nearest q is it.
+    // chunks [0, c)), or 0.  Nominal edges q·nch/P lie at least half a
+    // segment (twice a window) apart, so at most one window holds c, and
+    // the nearest q is it.
+    const int W = L.window;
     auto window = [&](long long c, long long& lo_edge) -> int {
         const long long q = (c * P + nch / 2) / nch;
         if (q < 1 || q > P - 1) return 0;
-        lo_edge = q * nch / P - kSpecWindow;
-        return (c >= lo_edge && c < lo_edge + 2 * kSpecWindow) ? (int)q : 0;
+        lo_edge = q * nch / P - W;
+        return (c >= lo_edge && c < lo_edge + 2 * W) ? (int)q : 0;
     };
     // An edge's key: |prefix| with its 7 lowest bits replaced by the
     // edge's place in the window counted from the far end, so the largest
     // key is an edge of (all but) the largest |prefix|, the first one on a
     // tie.  One walk takes each window's largest key; a second writes the
     // boundary at the edge that holds it.
-    static_assert(2 * kSpecWindow <= 128, "a window's places fit in 7 bits");
+
     for (int pass = 0; pass < 2; pass++) {
         DD pre = excl;
         each([&](long long c, DD v) {
@@ -436,7 +442,7 @@ __global__ __launch_bounds__(kSpecPlanThreads) void k_spec_plan(const SumJob* __
             if (q) {
                 const unsigned long long k =
                     ((unsigned long long)__double_as_longlong(fabs(pre.hi)) & ~0x7FULL) |
-                    (unsigned long long)(2 * kSpecWindow - 1 - (c - lo_edge));
+                    (unsigned long long)(2 * W - 1 - (c - lo_edge));  // < 128: 2W <= 128
                 if (pass == 0) {
                     atomicMax(&key[q], k);
                 } else if (k == key[q]) {
@@ -560,7 +566,20 @@ __global__ __launch_bounds__(256) void k_spec_resolve(const SumJob* __restrict__
 
 size_t spec_sums_scratch_bytes(int njobs, long long max_len) { return spec_layout(njobs, max_len).bytes; }
 
-bool spec_sums_fit(long long max_len) { return spec_segments_for(spec_chunks_for(max_len)) <= kSpecMaxSegs; }
+bool spec_sums_fit(long long max_len) { return spec_layout(1, max_len).pmax <= kSpecMaxSegs; }
+
+double spec_sums_cost_ratio(int njobs, long long max_len, long long total_len) {
+    // time model in cycles: the speculative form's longest candidate chain
+    // (~10 cycles a term) or its LDS broadcasts (8 bytes per lane per term
+    // to C / (64 · kSpecPerLane) waves a segment, 128 bytes a cycle on each
+    // of 256 CUs), whichever is longer, plus ~40 us of fixed launches;
+    // against one serial chain of the longest job
+    const SpecLayout L = spec_layout(njobs, max_len);
+    const double seg = 10.0 * (double)L.seg_chunks * kSpecChunk * 1.5;
+    const double lds = (double)total_len * L.cands * 8.0 / kSpecPerLane / (256.0 * 128.0);
+    const double spec = (seg > lds ? seg : lds) + 40e-6 * 2.4e9;
+    return spec / (10.0 * (double)max_len);
+}
 
 void launch_sums_spec(const SumJob* d_jobs, int njobs, long long max_len, double* d_out, void* d_scratch,
                       hipStream_t stream) {
