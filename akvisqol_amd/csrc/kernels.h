@@ -83,7 +83,7 @@ constexpr int kSumOutputs = 3;
 void launch_sums(const SumJob* d_jobs, int njobs, long long max_len, double* d_out, bool serial, double* d_partials,
                  hipStream_t stream);
 // The serial sums' speculative form (the same bits): each job's terms in
-// segments (16 K terms from 2048 candidate starts each when the longest
+// segments (8 K terms from 2048 candidate starts each when the longest
 // job has kSpecLongLen terms or more, else 4 K from 512) run at once,
 // then resolved segment by segment (a segment whose true start falls
 // outside its candidates is run again serially).  For launches of few

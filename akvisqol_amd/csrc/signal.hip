@@ -243,7 +243,7 @@ __global__ __launch_bounds__(64) void k_sums_finish(const SumJob* __restrict__ j
 //
 //   k_spec_chunks   double-double sums of 64-term chunks
 //   k_spec_plan     their prefix; segment boundaries at chunk edges, each
-//                   the edge of largest |prefix| within ±64 chunks of its
+//                   the edge of largest |prefix| within a window of its
 //                   nominal place (the ulp gap between the rounded chain
 //                   and the exact prefix is widest where the running sum
 //                   passes near zero); est_p = that prefix
@@ -314,8 +314,8 @@ inline SpecLayout spec_layout(int njobs, long long max_len) {
     SpecLayout L{};
     L.nchunks = spec_chunks_for(max_len);
     const bool shortjobs = max_len < kSpecLongLen;
-    L.seg_chunks = shortjobs ? 64 : 256;
-    L.window = shortjobs ? 16 : 64;
+    L.seg_chunks = shortjobs ? 64 : 128;
+    L.window = shortjobs ? 16 : 32;
     L.cands = shortjobs ? 512 : 2048;
     L.pmax = spec_segments_for(L.nchunks, L.seg_chunks);
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
