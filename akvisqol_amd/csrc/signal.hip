@@ -233,8 +233,8 @@ __global__ __launch_bounds__(64) void k_sums_finish(const SumJob* __restrict__ j
 // The reference-order sum of a long signal split into segments that run at
 // once, with the same bits as serial_chain.  Rounded f64 addition is
 // monotonic and a chain's value after a term depends only on its value
-// before it, so segment p is run from C consecutive start values (one
-// lane each: est_p - C/2 ulps .. est_p + C/2 - 1 ulps)
+// before it, so segment p is run from C consecutive start values
+// (kSpecPerLane to a lane: est_p - C/2 ulps .. est_p + C/2 - 1 ulps)
 // centred on an accurate estimate est_p of the true running sum at its
 // first term; once segment p - 1's end value e is known, the lane whose
 // start equals e holds segment p's true end value.  When e lies outside
@@ -316,7 +316,7 @@ inline SpecLayout spec_layout(int njobs, long long max_len) {
     const bool shortjobs = max_len < kSpecLongLen;
     L.seg_chunks = shortjobs ? 64 : 128;
     L.window = shortjobs ? 16 : 32;
-    L.cands = shortjobs ? 512 : 2048;
+    L.cands = shortjobs ? 512 : 2048;  // whole blocks: multiples of 256 · kSpecPerLane
     L.pmax = spec_segments_for(L.nchunks, L.seg_chunks);
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t off = 0;
