@@ -279,8 +279,9 @@ void Engine::readback(void* host_dst, const void* dev_src, size_t bytes) {
 size_t upload_span(long long len) { return ((size_t)std::max(1LL, len) + 31) & ~(size_t)31; }
 
 void Engine::upload_signals(int n, const double* const* src, const long long* len, double* const* dst) {
-    constexpr size_t kGroup = (size_t)1 << 28;  // samples staged at once
-    constexpr size_t kPiece = (size_t)2 << 20;  // samples per piece: 16 MiB
+    constexpr size_t kGroup = (size_t)1 << 28;       // samples staged at once
+    constexpr size_t kMaxPiece = (size_t)2 << 20;    // samples per piece: 16 MiB
+    constexpr size_t kMinPiece = (size_t)128 << 10;  // 1 MiB
     std::vector<int> groups{0};
     size_t stage_len = 0, acc = 0;
     for (int i = 0; i < n; i++) {
@@ -305,6 +306,11 @@ void Engine::upload_signals(int n, const double* const* src, const long long* le
         const size_t total = off[m];
         bool mirrored = true;  // dst[g0 + i] == dst[g0] + off[i]
         for (int i = 1; i < m && mirrored; i++) mirrored = dst[g0 + i] == dst[g0] + off[i];
+        // pieces of about an eighth of the group (1-16 MiB, whole 256-byte
+        // runs): a small upload (a pair or two) is then copied by several
+        // threads, its DMAs overlapping the copies, instead of one thread's
+        // copy followed by one DMA
+        const size_t kPiece = std::min(kMaxPiece, std::max(kMinPiece, (total / 8 + 31) & ~(size_t)31));
         const int T = std::min(8, host_threads((total + kPiece - 1) / kPiece, 2));
         std::atomic<int> failed{(int)hipSuccess};
         auto send = [&](double* to, const double* from, size_t count, size_t piece) {
@@ -339,7 +345,14 @@ void Engine::upload_signals(int n, const double* const* src, const long long* le
             }
         };
         std::vector<std::thread> pool;
-        for (int t = 1; t < T; t++) pool.emplace_back(stripe, t);
+        pool.reserve(T);  // only thread creation can throw below
+        for (int t = 1; t < T; t++) {
+            try {
+                pool.emplace_back(stripe, t);
+            } catch (const std::system_error&) {
+                stripe(t);  // no thread to be had: this stripe on the caller's
+            }
+        }
         stripe(0);
         for (auto& th : pool) th.join();
         // drain what was enqueued before any failure (the staging block is
