@@ -344,17 +344,7 @@ void Engine::upload_signals(int n, const double* const* src, const long long* le
                 if (mirrored && lo < hi) send(dst[g0] + lo, stage + lo, hi - lo, piece);
             }
         };
-        std::vector<std::thread> pool;
-        pool.reserve(T);  // only thread creation can throw below
-        for (int t = 1; t < T; t++) {
-            try {
-                pool.emplace_back(stripe, t);
-            } catch (const std::system_error&) {
-                stripe(t);  // no thread to be had: this stripe on the caller's
-            }
-        }
-        stripe(0);
-        for (auto& th : pool) th.join();
+        run_striped(T, stripe);
         // drain what was enqueued before any failure (the staging block is
         // reused), then report it
         hipError_t drained = hipSuccess;
@@ -1504,13 +1494,10 @@ void Engine::run_chunk(const PairInput* pairs, int n, uint32_t sr, PairResult* o
     const int per = opts_.mode == 1 && opts_.lattice ? 1 : 8;  // pairs per thread worth a thread
     const int nthreads = host_threads(n, per);
     if (nthreads > 1) {
-        std::vector<std::thread> pool;
-        for (int t = 0; t < nthreads; t++)
-            pool.emplace_back([&, t] {
-                for (int i = t; i < n; i += nthreads)
-                    if (st[i].live) finish(out[i], rp);
-            });
-        for (auto& th : pool) th.join();
+        run_striped(nthreads, [&](int t) {
+            for (int i = t; i < n; i += nthreads)
+                if (st[i].live) finish(out[i], rp);
+        });
     } else {
         for (int i = 0; i < n; i++)
             if (st[i].live) finish(out[i], rp);

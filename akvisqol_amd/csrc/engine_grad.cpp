@@ -76,12 +76,9 @@ void Engine::mapper_grad(int nb, const double* f0, const double* f1, const doubl
         };
         const int total = 4 * nb;
         if (threads > 1) {
-            std::vector<std::thread> pool;
-            for (int t = 0; t < threads; t++)
-                pool.emplace_back([&, t] {
-                    for (int idx = t; idx < total; idx += threads) slope(idx);
-                });
-            for (auto& th : pool) th.join();
+            run_striped(threads, [&](int t) {
+                for (int idx = t; idx < total; idx += threads) slope(idx);
+            });
         } else {
             for (int idx = 0; idx < total; idx++) slope(idx);
         }
@@ -252,12 +249,9 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
         }
     };
     if (nthreads > 1) {
-        std::vector<std::thread> pool;
-        for (int t = 0; t < nthreads; t++)
-            pool.emplace_back([&, t] {
-                for (size_t q = t; q < todo.size(); q += nthreads) host_pass(q);
-            });
-        for (auto& th : pool) th.join();
+        run_striped(nthreads, [&](int t) {
+            for (size_t q = t; q < todo.size(); q += nthreads) host_pass(q);
+        });
     } else {
         for (size_t q = 0; q < todo.size(); q++) host_pass(q);
     }
@@ -355,12 +349,9 @@ void Engine::backward(const PairInput* pairs, int n, uint32_t sr, PairResult* ou
         aligned_to_input_grad(deg, s.dl, s.shift, s.dl2, s.factor, s.e_deg, scaled, hs[q].g_al, grads[i]);
     };
     if (nthreads > 1) {
-        std::vector<std::thread> pool;
-        for (int t = 0; t < nthreads; t++)
-            pool.emplace_back([&, t] {
-                for (size_t q = t; q < todo.size(); q += nthreads) input_pass(q);
-            });
-        for (auto& th : pool) th.join();
+        run_striped(nthreads, [&](int t) {
+            for (size_t q = t; q < todo.size(); q += nthreads) input_pass(q);
+        });
     } else {
         for (size_t q = 0; q < todo.size(); q++) input_pass(q);
     }

@@ -8,8 +8,11 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <exception>
 #include <memory>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "dev_common.h"
@@ -100,5 +103,34 @@ std::unique_ptr<LatticeModel, LatticeDeleter> load_lattice(const std::string& by
 // graph error.
 bool lattice_predict(const LatticeModel& m, const double* fvnsim, const double* fvnsim10, const double* fstdnsim,
                      const double* fvdegenergy, double& out);
+
+// Runs f(t) for t = 0 .. n - 1: t >= 1 on threads of their own, t = 0 on
+// the caller's, and any t whose thread cannot be created on the caller's
+// too (after the threads already started), so a host short of threads
+// runs slower instead of terminating.  The first exception of a stripe run
+// on the caller's thread is rethrown once every thread has been joined.
+template <typename F>
+void run_striped(int n, F&& f) {
+    std::vector<std::thread> pool;
+    pool.reserve(n > 1 ? (size_t)(n - 1) : 0);
+    std::exception_ptr err;
+    auto here = [&](int t) {
+        try {
+            f(t);
+        } catch (...) {
+            if (!err) err = std::current_exception();
+        }
+    };
+    for (int t = 1; t < n; t++) {
+        try {
+            pool.emplace_back(f, t);
+        } catch (const std::system_error&) {
+            here(t);
+        }
+    }
+    here(0);
+    for (auto& th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+}
 
 }  // namespace vq

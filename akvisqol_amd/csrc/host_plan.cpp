@@ -169,10 +169,7 @@ std::vector<double> make_level_twiddles(long long n, int threads) {
         }
     };
     const long long nt = std::max(1LL, std::min<long long>(threads, half >> 16));
-    std::vector<std::thread> pool;
-    for (long long i = 1; i < nt; i++) pool.emplace_back(fill, half * i / nt, half * (i + 1) / nt);
-    fill(0, half / nt);
-    for (auto& th : pool) th.join();
+    run_striped((int)nt, [&](int i) { fill(half * i / nt, half * (i + 1) / nt); });
     for (long long S = 2; S < n; S <<= 1)
         for (long long j = 0; j < S / 2; j++) {
             t[2 * (S / 2 + j)] = top[2 * (j * (n / S))];
