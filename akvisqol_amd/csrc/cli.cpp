@@ -15,6 +15,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -106,7 +107,14 @@ int run_batch(visqol_ctx* ctx, const Args& a) {
     };
     const size_t want = a.threads ? a.threads : std::max(1u, std::thread::hardware_concurrency());
     std::vector<std::thread> pool;
-    for (size_t t = 1; t < std::min(want, names.size()); t++) pool.emplace_back(worker);
+    pool.reserve(std::min(want, names.size()));
+    for (size_t t = 1; t < std::min(want, names.size()); t++) {
+        try {
+            pool.emplace_back(worker);
+        } catch (const std::system_error&) {
+            break;  // the queue is shared: the threads there are (and this one) take the rest
+        }
+    }
     worker();
     for (auto& t : pool) t.join();
     // the first failure in CSV order ends the run
