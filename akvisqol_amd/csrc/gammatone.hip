@@ -52,6 +52,9 @@ static_assert(kGtShapes[kGtWideShape].bands_per_lane == 1, "the wide shape runs 
 // all-pole form each all-pole section's last two outputs.
 struct BandState {
     double s1c0 = 0, s1c1 = 0, s2c0 = 0, s2c1 = 0, s3c0 = 0, s3c1 = 0, s4c0 = 0, s4c1 = 0, sum_sq = 0;
+    // pipelined steps: sections 1-3's outputs of the previous step, the
+    // next sections' inputs at this one
+    double p1 = 0, p2 = 0, p3 = 0;
 };
 
 template <bool FAST>
@@ -116,6 +119,79 @@ __device__ inline void band_step_fir(BandState& b, FirState& f, const FirCoef& c
     b.sum_sq = __builtin_fma(y4, y4, b.sum_sq);
 }
 
+// Pipelined forms of band_step and band_step_fir for lanes that carry one
+// band: at step n section k runs on section k-1's output of step n-1
+// (section 1 on sample n), so the four sections' recurrences are
+// independent within a step and a lone wave waits on one section's
+// latency a step instead of the cascade's.  Each section performs the same
+// operations on the same inputs in the same order as in the plain step, so
+// every output is bit-identical; the sum of squares gains only the +0
+// squares of the first three steps (zero-state sections on zero inputs
+// stay zero) and needs three drain steps after the frame's last sample.
+template <bool FAST>
+__device__ inline void band_step_pipe(BandState& b, const BandCoef& k, double s) {
+    const double in2 = b.p1, in3 = b.p2, in4 = b.p3;
+    if (FAST) {
+        const double f1 = __builtin_fma(k.s1n0, s, b.s1c0);
+        b.s1c0 = __builtin_fma(-k.d1, f1, __builtin_fma(k.s1n1, s, b.s1c1));
+        b.s1c1 = -k.d2 * f1;
+        const double f2 = __builtin_fma(k.ca0, in2, b.s2c0);
+        b.s2c0 = __builtin_fma(-k.d1, f2, __builtin_fma(k.s2n1, in2, b.s2c1));
+        b.s2c1 = -k.d2 * f2;
+        const double f3 = __builtin_fma(k.ca0, in3, b.s3c0);
+        b.s3c0 = __builtin_fma(-k.d1, f3, __builtin_fma(k.s3n1, in3, b.s3c1));
+        b.s3c1 = -k.d2 * f3;
+        const double f4 = __builtin_fma(k.ca0, in4, b.s4c0);
+        b.s4c0 = __builtin_fma(-k.d1, f4, __builtin_fma(k.s4n1, in4, b.s4c1));
+        b.s4c1 = -k.d2 * f4;
+        b.sum_sq = __builtin_fma(f4, f4, b.sum_sq);
+        b.p1 = f1;
+        b.p2 = f2;
+        b.p3 = f3;
+    } else {
+        const double f1 = k.s1n0 * s + b.s1c0;
+        b.s1c0 = k.s1n1 * s + b.s1c1 - k.d1 * f1;
+        b.s1c1 = -(k.d2 * f1);
+        const double f2 = k.ca0 * in2 + b.s2c0;
+        b.s2c0 = k.s2n1 * in2 + b.s2c1 - k.d1 * f2;
+        b.s2c1 = -(k.d2 * f2);
+        const double f3 = k.ca0 * in3 + b.s3c0;
+        b.s3c0 = k.s3n1 * in3 + b.s3c1 - k.d1 * f3;
+        b.s3c1 = -(k.d2 * f3);
+        const double f4 = k.ca0 * in4 + b.s4c0;
+        b.s4c0 = k.s4n1 * in4 + b.s4c1 - k.d1 * f4;
+        b.s4c1 = -(k.d2 * f4);
+        b.sum_sq += f4 * f4;
+        b.p1 = f1;
+        b.p2 = f2;
+        b.p3 = f3;
+    }
+}
+
+__device__ inline void band_step_fir_pipe(BandState& b, FirState& f, const FirCoef& c, double s, double x1) {
+    const double in2 = b.p1, in3 = b.p2, in4 = b.p3;
+    const double u = __builtin_fma(c.c0, s, f.w1);
+    f.w1 = __builtin_fma(c.c1, s, f.w2);
+    f.w2 = __builtin_fma(c.c2, s, f.w3);
+    f.w3 = __builtin_fma(c.c3, s, c.c4 * x1);
+    const double y1 = __builtin_fma(-c.d1, b.s1c0, __builtin_fma(-c.d2, b.s1c1, u));
+    b.s1c1 = b.s1c0;
+    b.s1c0 = y1;
+    const double y2 = __builtin_fma(-c.d1, b.s2c0, __builtin_fma(-c.d2, b.s2c1, in2));
+    b.s2c1 = b.s2c0;
+    b.s2c0 = y2;
+    const double y3 = __builtin_fma(-c.d1, b.s3c0, __builtin_fma(-c.d2, b.s3c1, in3));
+    b.s3c1 = b.s3c0;
+    b.s3c0 = y3;
+    const double y4 = __builtin_fma(-c.d1, b.s4c0, __builtin_fma(-c.d2, b.s4c1, in4));
+    b.s4c1 = b.s4c0;
+    b.s4c0 = y4;
+    b.sum_sq = __builtin_fma(y4, y4, b.sum_sq);
+    b.p1 = y1;
+    b.p2 = y2;
+    b.p3 = y3;
+}
+
 // One wave's share of a block: V valid slots, the first M of them in DF2T
 // form.  Every variant stages its part of the tile and meets the block's
 // barriers the same number of times.  PAIR: the block holds 32 frames of
@@ -127,6 +203,10 @@ __device__ __forceinline__ void gt_run(const GtJob& ja, const GtJob& jb, const B
                                        int window, int hop, double (*tile)[CHUNK + 1], int fblock) {
     constexpr int kThreads = 64 * WAVES;
     constexpr int kStage = kFramesPerWave * CHUNK / kThreads;  // tile elements staged per thread
+    // One band a lane: too few lanes to hide a cascade's latency, so the
+    // sections run pipelined (with several bands a lane, the bands'
+    // independent cascades already interleave)
+    constexpr bool kPipe = BPL == 1;
     constexpr int kSlots = V > 0 ? V : 1;
     constexpr int kFramesPerJob = PAIR ? kFramesPerWave / 2 : kFramesPerWave;
     const int f0 = fblock * kFramesPerJob;
@@ -193,11 +273,28 @@ __device__ __forceinline__ void gt_run(const GtJob& ja, const GtJob& jb, const B
                 const double s = tile[lane][i];
 #pragma unroll
                 for (int q = 0; q < V; q++) {
-                    if (q < M) band_step<FAST>(st[q], k[q], s);
-                    else band_step_fir(st[q], fs[q], c[q], s, x1);
+                    if (kPipe) {
+                        if (q < M) band_step_pipe<FAST>(st[q], k[q], s);
+                        else band_step_fir_pipe(st[q], fs[q], c[q], s, x1);
+                    } else {
+                        if (q < M) band_step<FAST>(st[q], k[q], s);
+                        else band_step_fir(st[q], fs[q], c[q], s, x1);
+                    }
                 }
                 x1 = s;
             }
+        }
+    }
+    if (kPipe && V > 0 && active) {
+        // drain: sections 2-4 take the last three samples' pending values
+        // (section 1's outputs of these steps are never used)
+        for (int d = 0; d < 3; d++) {
+#pragma unroll
+            for (int q = 0; q < V; q++) {
+                if (q < M) band_step_pipe<FAST>(st[q], k[q], 0.0);
+                else band_step_fir_pipe(st[q], fs[q], c[q], 0.0, x1);
+            }
+            x1 = 0.0;
         }
     }
     if (V > 0 && active) {
