@@ -33,18 +33,29 @@ __global__ __launch_bounds__(256) void k_frame_floor(const PrepJob* __restrict__
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int mf = j.nfr < j.nfd ? j.nfr : j.nfd;
     if (c >= mf) return;
+    // the column's bands held in registers (nb <= kMaxBands): every load
+    // issued before the first is used, each value read once
+    double rv[kMaxBands], dv[kMaxBands];
+#pragma unroll
+    for (int r = 0; r < kMaxBands; r++)
+        if (r < nb) {
+            rv[r] = j.ref[(long long)r * j.nfr + c];
+            dv[r] = j.deg[(long long)r * j.nfd + c];
+        }
     double col_max = -INFINITY;
-    for (int r = 0; r < nb; r++) {
-        const double m = fmax(j.ref[(long long)r * j.nfr + c], j.deg[(long long)r * j.nfd + c]);
-        if (m > col_max) col_max = m;
-    }
+#pragma unroll
+    for (int r = 0; r < kMaxBands; r++)
+        if (r < nb) {
+            const double m = fmax(rv[r], dv[r]);
+            if (m > col_max) col_max = m;
+        }
     const double floor_db = col_max - 45.0;
-    for (int r = 0; r < nb; r++) {
-        double* a = &j.ref[(long long)r * j.nfr + c];
-        double* b = &j.deg[(long long)r * j.nfd + c];
-        if (*a < floor_db) *a = floor_db;
-        if (*b < floor_db) *b = floor_db;
-    }
+#pragma unroll
+    for (int r = 0; r < kMaxBands; r++)
+        if (r < nb) {
+            if (rv[r] < floor_db) j.ref[(long long)r * j.nfr + c] = floor_db;
+            if (dv[r] < floor_db) j.deg[(long long)r * j.nfd + c] = floor_db;
+        }
 }
 
 __global__ void k_min_init(unsigned long long* scratch, int n) {
@@ -613,11 +624,31 @@ __global__ __launch_bounds__(64) void k_stats(const StatJob* __restrict__ jobs, 
     double* R = lds;
     double* D = lds + nb * S;
     double* SIM = lds + 2 * nb * S;
-    for (int e = threadIdx.x; e < nb * S; e += blockDim.x) {
-        const int r = e / S, c = e % S;
-        R[e] = j.ref[(long long)r * j.ref_stride + j.ref_col + c];
-        const int dc = j.deg_col + c;
-        D[e] = dc < j.deg_valid ? j.deg[(long long)r * j.deg_stride + dc] : 0.0;
+    // staged eight elements a thread at a time, every load of a group
+    // issued before the group's first is stored
+    constexpr int kGroup = 8;
+    for (int e0 = threadIdx.x; e0 < nb * S; e0 += kGroup * blockDim.x) {
+        double rv[kGroup], dv[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; u++) {
+            const int e = e0 + u * (int)blockDim.x;
+            rv[u] = 0.0;
+            dv[u] = 0.0;
+            if (e < nb * S) {
+                const int r = e / S, c = e - r * S;
+                rv[u] = j.ref[(long long)r * j.ref_stride + j.ref_col + c];
+                const int dc = j.deg_col + c;
+                if (dc < j.deg_valid) dv[u] = j.deg[(long long)r * j.deg_stride + dc];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kGroup; u++) {
+            const int e = e0 + u * (int)blockDim.x;
+            if (e < nb * S) {
+                R[e] = rv[u];
+                D[e] = dv[u];
+            }
+        }
     }
     __syncthreads();
     const int r = threadIdx.x;
